@@ -1,0 +1,156 @@
+/* C ABI of the gfx950 propagate-and-screen engine (libkessler_hip.so).
+ *
+ * One engine context per host thread; one HIP stream per context; not
+ * re-entrant. Every function returns 0 on success or a negative status; the
+ * message of the last failure is kept in the context (khip_last_error) or,
+ * for khip_create, written to the caller's buffer. Nothing throws across
+ * this boundary.
+ *
+ * Object state is device-resident between calls: the host uploads it once
+ * (khip_set_objects), runs any number of steps (khip_run), and reads back
+ * states (khip_get_objects), events (khip_take_events) and per-step
+ * statistics (khip_take_stats) only when it needs them.
+ *
+ * Which reference interface each entry point replaces (file:line in the
+ * Kessler_Sim reference):
+ *
+ *   khip_create / khip_destroy     kessler::gpu::create / destroy
+ *                                  (collision_gpu.hpp:47-48) and
+ *                                  Screener::set_gpu (collision.hpp:93)
+ *   khip_set_objects               gpu::staging + the per-step H2D upload
+ *                                  (collision_gpu.hpp:53, collision.cpp:574-583);
+ *                                  here once per run, not once per step
+ *   khip_set_config                ScreeningConfig (collision.hpp:26-37)
+ *   khip_set_time                  Simulation::{epoch_jd, t, dt} (orbital.hpp:145-148)
+ *   khip_run                       the main.cpp step loop body (main.cpp:352-373):
+ *                                  Simulation::step (orbital.cpp:286-300) and
+ *                                  Screener::begin/finish (collision.cpp:706-750)
+ *   khip_screen                    Screener::screen / screen_step on host-held
+ *                                  before/after states (collision.hpp:100-102, 124-128)
+ *   khip_take_events               gpu::collect (collision_gpu.hpp:62) + the
+ *                                  exact filter/refine and (t, i, j) sort of
+ *                                  collision.cpp:622-647, 739-747
+ *   khip_take_stats                ScreeningStats (collision.hpp:47-62)
+ *   khip_get_objects               Simulation::objects[i].{s, alive} read-back
+ *                                  (orbital.hpp:145, 67-84)
+ */
+#ifndef KESSLER_HIP_H
+#define KESSLER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KHIP_OK 0
+#define KHIP_ERR_NO_DEVICE (-1)
+#define KHIP_ERR_HIP (-2)
+#define KHIP_ERR_ARG (-3)
+#define KHIP_ERR_STATE (-4)
+
+typedef struct khip_ctx khip_ctx;
+
+/* ScreeningConfig (collision.hpp:26-37). */
+typedef struct {
+    double threshold_km;             /* default 5 */
+    double radius_scale;             /* default 1 */
+    double min_encounter_speed_km_s; /* default 0.01 */
+} khip_screen_config;
+
+/* Conjunction (collision.hpp:39-45) plus the step index it was found in. */
+typedef struct {
+    int32_t i, j;      /* object indices, i < j */
+    int64_t step;      /* global step index since khip_set_time */
+    double t;          /* TCA, s since epoch */
+    double miss_km;
+    double v_rel_km_s;
+    int32_t collision; /* miss_km < R_i + R_j */
+    int32_t pad;
+} khip_event;
+
+/* The ScreeningStats fields that the device computes (collision.hpp:47-62). */
+typedef struct {
+    int64_t candidate_pairs;
+    int64_t refined_pairs;
+    int64_t co_orbiting;
+    double cell_km;
+    int32_t max_cell_objects;
+    int32_t cell_doublings; /* > 0 only where the grid extent needed a coarser cell (see khip_run) */
+} khip_stats;
+
+/* Device time per kernel family, summed over the steps since the last
+ * khip_reset_timing, measured with HIP events on the engine's stream when
+ * timing is enabled (khip_set_timing). */
+typedef struct {
+    double propagate_ms; /* fused RK4 step */
+    double grid_ms;      /* keys, radix sort, cell table, neighbours */
+    double pairs_ms;     /* fused enumeration + filters + TCA + append */
+    double total_ms;     /* whole steps, launch gaps included */
+    int64_t steps;
+} khip_timing;
+
+/* Number of visible HIP devices (0 when none or the runtime fails). */
+int khip_device_count(void);
+
+/* Creates a context on `device`. On failure *out is NULL and `msg` (if
+ * given) receives the reason; on success `msg` receives the device name. */
+int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap);
+void khip_destroy(khip_ctx* ctx);
+const char* khip_last_error(const khip_ctx* ctx);
+
+/* Uploads n objects: states is n x 6 doubles (x, y, z km; vx, vy, vz km/s),
+ * alive n bytes (NULL = all alive), size_m n doubles (characteristic size L,
+ * collision radius L/2). Replaces any previous population. */
+int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
+                     const double* size_m);
+int64_t khip_object_count(const khip_ctx* ctx);
+/* Reads back states (n x 6) and alive flags; either pointer may be NULL. */
+int khip_get_objects(khip_ctx* ctx, double* states, uint8_t* alive);
+
+int khip_set_config(khip_ctx* ctx, const khip_screen_config* cfg);
+/* Epoch (Julian day), elapsed time t (s since epoch) and step dt (s).
+ * Resets the global step counter to 0. */
+int khip_set_time(khip_ctx* ctx, double epoch_jd, double t, double dt);
+int khip_get_time(const khip_ctx* ctx, double* t, int64_t* step);
+
+/* Runs nsteps steps of the main loop: FP64 RK4 of every alive object
+ * (Earth + J2 + Moon) and, if `screen`, the screening of each step
+ * (grid on start-of-step positions of the objects alive at the end of the
+ * step, FP32 pre-filter, FP64 filter_pair, FP64 Hermite/golden-section TCA,
+ * radius test). Events and per-step statistics accumulate on the host side
+ * of the context until taken. */
+int khip_run(khip_ctx* ctx, int64_t nsteps, int screen);
+
+/* One screening of host-held states: `before` (start of step) and `after`
+ * (end of step) are n x 6 doubles, alive_after n bytes, size_m n doubles.
+ * Uses (and overwrites) the context's device population. Events and stats
+ * are returned directly; *nevents receives the full count even beyond cap. */
+int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* after,
+                const uint8_t* alive_after, const double* size_m, double t0, double dt,
+                khip_event* events, int64_t cap, int64_t* nevents, khip_stats* stats);
+
+/* Moves up to cap accumulated events (sorted by step, t, i, j) to `out`;
+ * *n receives the number moved. Call with out == NULL to query the count. */
+int khip_take_events(khip_ctx* ctx, khip_event* out, int64_t cap, int64_t* n);
+/* Same for per-step statistics, one record per screened step. */
+int khip_take_stats(khip_ctx* ctx, khip_stats* out, int64_t cap, int64_t* n);
+
+int khip_set_timing(khip_ctx* ctx, int enabled);
+int khip_get_timing(const khip_ctx* ctx, khip_timing* out);
+int khip_reset_timing(khip_ctx* ctx);
+
+/* Steps per host synchronisation in khip_run (default 256). */
+int khip_set_chunk(khip_ctx* ctx, int steps);
+
+/* FP64 arithmetic self-check: out[k] = {sqrt(a[k]), a[k] / b[k],
+ * floor(a[k] / b[k])} computed on the device, for comparison with the host's
+ * IEEE results. n <= 1 << 20. */
+int khip_fp64_selfcheck(khip_ctx* ctx, int64_t n, const double* a, const double* b, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

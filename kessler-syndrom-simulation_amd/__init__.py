@@ -1,0 +1,225 @@
+"""Python handle on the MI355X propagate-and-screen engine.
+
+The engine is ``lib/libkessler_hip.so`` (HIP kernels for gfx950 + the C ABI of
+``include/kessler_hip.h``); this module only binds it with ctypes for the
+tests and the benchmark. There is no Python or CPU fallback: if the library
+is missing or no GPU is visible, the calls raise.
+
+The directory name is not a Python identifier; load it with
+``__graft_entry__.load_package()`` (importlib, module name ``kessler_amd``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libkessler_hip.so")
+
+EVENT_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("step", "<i8"), ("t", "<f8"),
+                        ("miss_km", "<f8"), ("v_rel_km_s", "<f8"), ("collision", "<i4"),
+                        ("pad", "<i4")])
+STATS_DTYPE = np.dtype([("candidate_pairs", "<i8"), ("refined_pairs", "<i8"),
+                        ("co_orbiting", "<i8"), ("cell_km", "<f8"),
+                        ("max_cell_objects", "<i4"), ("cell_doublings", "<i4")])
+TIMING_DTYPE = np.dtype([("propagate_ms", "<f8"), ("grid_ms", "<f8"), ("pairs_ms", "<f8"),
+                         ("total_ms", "<f8"), ("steps", "<i8")])
+
+# Every symbol include/kessler_hip.h declares.
+ABI_SYMBOLS = (
+    "khip_device_count", "khip_create", "khip_destroy", "khip_last_error",
+    "khip_set_objects", "khip_object_count", "khip_get_objects", "khip_set_config",
+    "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
+    "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
+    "khip_set_chunk", "khip_fp64_selfcheck",
+)
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_int64
+_D = C.c_double
+
+
+class ScreenConfig(C.Structure):
+    _fields_ = [("threshold_km", _D), ("radius_scale", _D), ("min_encounter_speed_km_s", _D)]
+
+
+def build(jobs: int = 8) -> str:
+    """Compiles the engine for gfx950 (always recompiles) and returns the .so path."""
+    subprocess.run(["make", "-B", "-C", HERE, f"-j{jobs}"], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP engine not built: {LIB_PATH} is missing (run build())")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "khip_device_count": (_I, []),
+        "khip_create": (_I, [_I, C.POINTER(_P), C.c_char_p, C.c_size_t]),
+        "khip_destroy": (None, [_P]),
+        "khip_last_error": (C.c_char_p, [_P]),
+        "khip_set_objects": (_I, [_P, _L, _P, _P, _P]),
+        "khip_object_count": (_L, [_P]),
+        "khip_get_objects": (_I, [_P, _P, _P]),
+        "khip_set_config": (_I, [_P, C.POINTER(ScreenConfig)]),
+        "khip_set_time": (_I, [_P, _D, _D, _D]),
+        "khip_get_time": (_I, [_P, C.POINTER(_D), C.POINTER(_L)]),
+        "khip_run": (_I, [_P, _L, _I]),
+        "khip_screen": (_I, [_P, _L, _P, _P, _P, _P, _D, _D, _P, _L, C.POINTER(_L), _P]),
+        "khip_take_events": (_I, [_P, _P, _L, C.POINTER(_L)]),
+        "khip_take_stats": (_I, [_P, _P, _L, C.POINTER(_L)]),
+        "khip_set_timing": (_I, [_P, _I]),
+        "khip_get_timing": (_I, [_P, _P]),
+        "khip_reset_timing": (_I, [_P]),
+        "khip_set_chunk": (_I, [_P, _I]),
+        "khip_fp64_selfcheck": (_I, [_P, _L, _P, _P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+class Engine:
+    """One device context (khip_ctx). Mirrors the reference's Simulation +
+    Screener pair for one population: upload once, run steps, read back."""
+
+    def __init__(self, device: int = 0):
+        self.lib = library()
+        msg = C.create_string_buffer(512)
+        h = _P()
+        rc = self.lib.khip_create(device, C.byref(h), msg, 512)
+        if rc != 0:
+            raise RuntimeError(f"khip_create failed ({rc}): {msg.value.decode()}")
+        self.h = h
+        self.device_name = msg.value.decode()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.khip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            err = self.lib.khip_last_error(self.h)
+            raise RuntimeError(f"{what} failed ({rc}): {err.decode() if err else ''}")
+
+    def set_objects(self, states, alive=None, size_m=None):
+        states = np.ascontiguousarray(states, dtype=np.float64)
+        n = states.shape[0]
+        alive = None if alive is None else np.ascontiguousarray(alive, dtype=np.uint8)
+        size_m = None if size_m is None else np.ascontiguousarray(size_m, dtype=np.float64)
+        self._check(self.lib.khip_set_objects(self.h, n, _ptr(states), _ptr(alive), _ptr(size_m)),
+                    "khip_set_objects")
+
+    @property
+    def n(self) -> int:
+        return int(self.lib.khip_object_count(self.h))
+
+    def get_objects(self):
+        n = self.n
+        states = np.zeros((n, 6))
+        alive = np.zeros(n, dtype=np.uint8)
+        self._check(self.lib.khip_get_objects(self.h, _ptr(states), _ptr(alive)), "khip_get_objects")
+        return states, alive
+
+    def set_config(self, threshold_km=5.0, radius_scale=1.0, min_encounter_speed_km_s=0.01):
+        cfg = ScreenConfig(threshold_km, radius_scale, min_encounter_speed_km_s)
+        self._check(self.lib.khip_set_config(self.h, C.byref(cfg)), "khip_set_config")
+
+    def set_time(self, epoch_jd, t=0.0, dt=10.0):
+        self._check(self.lib.khip_set_time(self.h, float(epoch_jd), float(t), float(dt)),
+                    "khip_set_time")
+
+    def time(self):
+        t = _D()
+        s = _L()
+        self._check(self.lib.khip_get_time(self.h, C.byref(t), C.byref(s)), "khip_get_time")
+        return t.value, s.value
+
+    def set_chunk(self, steps: int):
+        self._check(self.lib.khip_set_chunk(self.h, int(steps)), "khip_set_chunk")
+
+    def run(self, nsteps: int, screen: bool = True):
+        self._check(self.lib.khip_run(self.h, int(nsteps), 1 if screen else 0), "khip_run")
+
+    def take_events(self) -> np.ndarray:
+        n = _L()
+        self._check(self.lib.khip_take_events(self.h, None, 0, C.byref(n)), "khip_take_events")
+        out = np.zeros(n.value, dtype=EVENT_DTYPE)
+        self._check(self.lib.khip_take_events(self.h, _ptr(out), n.value, C.byref(n)),
+                    "khip_take_events")
+        return out[: n.value]
+
+    def take_stats(self) -> np.ndarray:
+        n = _L()
+        self._check(self.lib.khip_take_stats(self.h, None, 0, C.byref(n)), "khip_take_stats")
+        out = np.zeros(n.value, dtype=STATS_DTYPE)
+        self._check(self.lib.khip_take_stats(self.h, _ptr(out), n.value, C.byref(n)),
+                    "khip_take_stats")
+        return out[: n.value]
+
+    def screen(self, before, after, alive_after, size_m, t0, dt, cap=1 << 20):
+        before = np.ascontiguousarray(before, dtype=np.float64)
+        after = np.ascontiguousarray(after, dtype=np.float64)
+        alive = np.ascontiguousarray(alive_after, dtype=np.uint8)
+        size_m = np.ascontiguousarray(size_m, dtype=np.float64)
+        out = np.zeros(cap, dtype=EVENT_DTYPE)
+        st = np.zeros(1, dtype=STATS_DTYPE)
+        ne = _L()
+        self._check(self.lib.khip_screen(self.h, before.shape[0], _ptr(before), _ptr(after),
+                                         _ptr(alive), _ptr(size_m), float(t0), float(dt),
+                                         _ptr(out), cap, C.byref(ne), _ptr(st)), "khip_screen")
+        assert ne.value <= cap, "event buffer too small"
+        return out[: ne.value].copy(), st[0]
+
+    def set_timing(self, on: bool):
+        self._check(self.lib.khip_set_timing(self.h, 1 if on else 0), "khip_set_timing")
+
+    def timing(self):
+        out = np.zeros(1, dtype=TIMING_DTYPE)
+        self._check(self.lib.khip_get_timing(self.h, _ptr(out)), "khip_get_timing")
+        return out[0]
+
+    def reset_timing(self):
+        self._check(self.lib.khip_reset_timing(self.h), "khip_reset_timing")
+
+    def fp64_selfcheck(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros((a.shape[0], 3))
+        self._check(self.lib.khip_fp64_selfcheck(self.h, a.shape[0], _ptr(a), _ptr(b), _ptr(out)),
+                    "khip_fp64_selfcheck")
+        return out
+
+
+def device_count() -> int:
+    return int(library().khip_device_count())

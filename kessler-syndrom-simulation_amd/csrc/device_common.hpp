@@ -1,0 +1,175 @@
+// Device-side building blocks of the gfx950 engine: constants, the FP64
+// dynamics, the Hermite closest-approach solve and wave64 helpers.
+//
+// All FP64 arithmetic keeps the reference's operation order (orbital.cpp:
+// 91-118, collision.cpp:77-132) and the file is compiled with
+// -ffp-contract=off: with IEEE-rounded division and square root on the
+// device, every result is bit-identical to the host oracle.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../host/collision_filter.hpp"
+
+namespace kessler {
+namespace dev {
+
+constexpr double MU_EARTH = 398600.4418;
+constexpr double MU_MOON = 4902.800066;
+constexpr double R_EARTH = 6378.137;
+constexpr double J2 = 1.08262668e-3;
+// Folded at compile time left to right, as the reference's expression is.
+constexpr double K_J2 = 1.5 * J2 * MU_EARTH * R_EARTH * R_EARTH;
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
+
+// One lunar sample: position and the indirect (frame) acceleration
+// rm * mu_M / |rm|^3, both computed on the host.
+struct MoonSample {
+    double m[3];
+    double ind[3];
+};
+
+struct D3 {
+    double x, y, z;
+};
+
+__device__ __forceinline__ D3 accel(D3 r, const MoonSample& mo) {
+    const double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const double rn = sqrt(r2);
+    const double c = -MU_EARTH / (r2 * rn);
+    D3 a{r.x * c, r.y * c, r.z * c};
+    const double k = K_J2 / (r2 * r2 * rn);
+    const double z2 = 5.0 * r.z * r.z / r2;
+    a.x = a.x + k * r.x * (z2 - 1.0);
+    a.y = a.y + k * r.y * (z2 - 1.0);
+    a.z = a.z + k * r.z * (z2 - 3.0);
+    const double dx = mo.m[0] - r.x, dy = mo.m[1] - r.y, dz = mo.m[2] - r.z;
+    const double dn = sqrt(dx * dx + dy * dy + dz * dz);
+    const double s = MU_MOON / (dn * dn * dn);
+    a.x = a.x + dx * s - mo.ind[0];
+    a.y = a.y + dy * s - mo.ind[1];
+    a.z = a.z + dz * s - mo.ind[2];
+    return a;
+}
+
+// Classical RK4 (orbital.cpp:107-118) on one object; r, v updated in place.
+__device__ __forceinline__ void rk4(D3& r, D3& v, double dt, const MoonSample& m0,
+                                    const MoonSample& m1, const MoonSample& m2) {
+    const double h = dt / 2;
+    const D3 k1v = accel(r, m0);
+    const D3 k2r{v.x + k1v.x * h, v.y + k1v.y * h, v.z + k1v.z * h};
+    const D3 k2v = accel(D3{r.x + v.x * h, r.y + v.y * h, r.z + v.z * h}, m1);
+    const D3 k3r{v.x + k2v.x * h, v.y + k2v.y * h, v.z + k2v.z * h};
+    const D3 k3v = accel(D3{r.x + k2r.x * h, r.y + k2r.y * h, r.z + k2r.z * h}, m1);
+    const D3 k4r{v.x + k3v.x * dt, v.y + k3v.y * dt, v.z + k3v.z * dt};
+    const D3 k4v = accel(D3{r.x + k3r.x * dt, r.y + k3r.y * dt, r.z + k3r.z * dt}, m2);
+    const double w = dt / 6;
+    const D3 nr{r.x + (v.x + k2r.x * 2 + k3r.x * 2 + k4r.x) * w,
+                r.y + (v.y + k2r.y * 2 + k3r.y * 2 + k4r.y) * w,
+                r.z + (v.z + k2r.z * 2 + k3r.z * 2 + k4r.z) * w};
+    const D3 nv{v.x + (k1v.x + k2v.x * 2 + k3v.x * 2 + k4v.x) * w,
+                v.y + (k1v.y + k2v.y * 2 + k3v.y * 2 + k4v.y) * w,
+                v.z + (k1v.z + k2v.z * 2 + k3v.z * 2 + k4v.z) * w};
+    r = nr;
+    v = nv;
+}
+
+// Cubic Hermite of the relative state over a step (collision.cpp:77-90).
+struct Hermite {
+    D3 p0, m0, p1, m1;
+    __device__ __forceinline__ D3 pos(double s) const {
+        const double s2 = s * s, s3 = s2 * s;
+        const double c0 = 2 * s3 - 3 * s2 + 1, c1 = s3 - 2 * s2 + s, c2 = -2 * s3 + 3 * s2,
+                     c3 = s3 - s2;
+        return D3{p0.x * c0 + m0.x * c1 + p1.x * c2 + m1.x * c3,
+                  p0.y * c0 + m0.y * c1 + p1.y * c2 + m1.y * c3,
+                  p0.z * c0 + m0.z * c1 + p1.z * c2 + m1.z * c3};
+    }
+    __device__ __forceinline__ D3 vel(double s) const {
+        const double s2 = s * s;
+        const double c0 = 6 * s2 - 6 * s, c1 = 3 * s2 - 4 * s + 1, c2 = -6 * s2 + 6 * s,
+                     c3 = 3 * s2 - 2 * s;
+        return D3{p0.x * c0 + m0.x * c1 + p1.x * c2 + m1.x * c3,
+                  p0.y * c0 + m0.y * c1 + p1.y * c2 + m1.y * c3,
+                  p0.z * c0 + m0.z * c1 + p1.z * c2 + m1.z * c3};
+    }
+};
+
+// Minimum of |p(s)|^2 on [0, 1] by 60 golden-section iterations
+// (collision.cpp:94-105).
+__device__ __forceinline__ double golden_min(const Hermite& h) {
+    const double g = 0.6180339887498949;
+    double a = 0.0, b = 1.0;
+    double c = b - g * (b - a), d = a + g * (b - a);
+    D3 p = h.pos(c);
+    double fc = p.x * p.x + p.y * p.y + p.z * p.z;
+    p = h.pos(d);
+    double fd = p.x * p.x + p.y * p.y + p.z * p.z;
+    for (int it = 0; it < 60; ++it) {
+        if (fc < fd) {
+            b = d;
+            d = c;
+            fd = fc;
+            c = b - g * (b - a);
+            p = h.pos(c);
+            fc = p.x * p.x + p.y * p.y + p.z * p.z;
+        } else {
+            a = c;
+            c = d;
+            fc = fd;
+            d = a + g * (b - a);
+            p = h.pos(d);
+            fd = p.x * p.x + p.y * p.y + p.z * p.z;
+        }
+    }
+    return 0.5 * (a + b);
+}
+
+// Monotone map of doubles onto uint64 (for atomic min / max).
+__device__ __forceinline__ unsigned long long ordered_bits(double x) {
+    const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ULL);
+}
+__host__ __device__ inline double from_ordered_bits(unsigned long long u) {
+    const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffULL) : ~u;
+    double x;
+    __builtin_memcpy(&x, &b, sizeof(x));
+    return x;
+}
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanes_below() {
+    const unsigned l = __lane_id();
+    return l == 0 ? 0ULL : (~0ULL >> (64 - l));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, WAVE);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+}  // namespace dev
+}  // namespace kessler

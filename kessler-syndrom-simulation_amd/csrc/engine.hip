@@ -1,0 +1,1365 @@
+// gfx950 propagate-and-screen engine and its C ABI (include/kessler_hip.h).
+//
+// Device-resident design. Object state lives in HBM as structure-of-arrays
+// FP64 (x, y, z, vx, vy, vz planes) in two buffers that swap every step, so
+// one step reads the start-of-step buffer and writes the end-of-step one;
+// the screening of that step then has both at hand with no host copy.
+//
+// One step is a fixed sequence of kernels on one stream:
+//
+//   k_propagate   fused four-stage RK4, one object per lane, coalesced SoA
+//                 loads/stores; the Moon samples of the step are uniform
+//                 (scalar) loads; reduces, over the objects alive at the end
+//                 of the step, max |v| and the bounding box of the
+//                 start-of-step positions (wave reductions + one atomic per
+//                 wave).
+//   k_grid_setup  cell size (collision.cpp:339-356) and a dense cell key:
+//                 per-axis cell offsets from the bounding box, packed into
+//                 at most 31 bits, so the radix sort runs 1-4 byte passes.
+//   k_keys        packed cell key per object (dead objects get a sentinel
+//                 that sorts last) + first-pass digit histogram.
+//   k_hist / k_digit_scan / k_scatter
+//                 stable LSD radix sort of (key, index): per-tile digit
+//                 histograms, per-digit scans, then a scatter whose in-tile
+//                 ranks come from wave64 ballots (match on the 8 digit bits).
+//   k_cell_count / k_cell_scan / k_cell_write
+//                 cell boundaries by a wavefront scan of the head flags;
+//                 writes the cell table and the compact FP32 copy of each
+//                 sorted object (cell-relative position, velocity, index).
+//   k_neighbours  per cell: population and the 13 forward neighbours
+//                 (binary search in the sorted cell keys).
+//   k_pairs       one lane per sorted object: balanced intra-cell pairs +
+//                 13 forward cells, FP32 prefilter_keep, FP64 filter_pair,
+//                 FP64 Hermite / golden-section TCA and radius test; events
+//                 appended with wave-aggregated atomics.
+//   k_finish      per-step statistics record, accumulators reset.
+//
+// The host only enqueues; it synchronises once per chunk of steps to drain
+// events and statistics.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kessler_hip.h"
+#include "../host/orbital.hpp"
+#include "device_common.hpp"
+
+using namespace kessler;
+using namespace kessler::dev;
+
+namespace {
+
+constexpr int ITEMS = 8;               // elements per thread per tile
+constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
+constexpr int RADIX = 256;
+constexpr int KEY_BITS_MAX = 31;       // packed key width; sentinel is 0xffffffff
+constexpr uint32_t DEAD_KEY = 0xffffffffu;
+constexpr int MAX_PASSES = 4;
+
+// Device-resident grid parameters and per-step accumulators.
+struct GridState {
+    // Accumulated by k_propagate / k_reduce over end-of-step-alive objects.
+    unsigned long long vmax_bits;   // max |v| as raw double bits (non-negative)
+    unsigned long long lo[3];       // ordered bits of min position
+    unsigned long long hi[3];       // ordered bits of max position
+    unsigned int alive;             // objects alive at the end of the step
+    unsigned int pad0;
+    // Derived by k_grid_setup.
+    double cell;                    // cell edge (km)
+    double cell_ref;                // the reference's cell size before any doubling
+    long long imin[3];              // smallest cell coordinate per axis
+    int bits[3];                    // packed bits per axis (x, y, z)
+    int npasses;                    // radix passes needed (0: nothing alive)
+    int doublings;                  // times the cell was doubled to fit the key
+    unsigned int ncells;
+    // Accumulated by k_pairs.
+    unsigned long long cand, refined, co;
+    unsigned int max_cell;
+    unsigned int pad1;
+    unsigned long long events;      // events appended since the last drain
+};
+
+struct DevEvent {
+    int32_t i, j;
+    int64_t step;
+    double t, miss_km, v_rel_km_s;
+    int32_t collision, pad;
+};
+static_assert(sizeof(DevEvent) == sizeof(khip_event), "event layout");
+
+struct DevStats {
+    long long cand, refined, co;
+    double cell_km;
+    int max_cell, doublings;
+};
+static_assert(sizeof(DevStats) == sizeof(khip_stats), "stats layout");
+
+// SoA view of one state buffer.
+struct Planes {
+    double *x, *y, *z, *vx, *vy, *vz;
+};
+
+__device__ __forceinline__ void block_reduce_grid(GridState* gs, bool on, double vn, double rx,
+                                                  double ry, double rz) {
+    // Per-wave reduction, then one atomic per wave: 8 quantities.
+    unsigned long long vb = on ? static_cast<unsigned long long>(__double_as_longlong(vn)) : 0ULL;
+    unsigned long long l0 = on ? ordered_bits(rx) : ~0ULL, l1 = on ? ordered_bits(ry) : ~0ULL,
+                       l2 = on ? ordered_bits(rz) : ~0ULL;
+    unsigned long long h0 = on ? ordered_bits(rx) : 0ULL, h1 = on ? ordered_bits(ry) : 0ULL,
+                       h2 = on ? ordered_bits(rz) : 0ULL;
+    const unsigned long long ball = __ballot(on);
+    if (ball == 0ULL) return;
+    vb = wave_max_u64(vb);
+    l0 = wave_min_u64(l0); l1 = wave_min_u64(l1); l2 = wave_min_u64(l2);
+    h0 = wave_max_u64(h0); h1 = wave_max_u64(h1); h2 = wave_max_u64(h2);
+    if (__lane_id() == 0) {
+        atomicMax(&gs->vmax_bits, vb);
+        atomicMin(&gs->lo[0], l0); atomicMin(&gs->lo[1], l1); atomicMin(&gs->lo[2], l2);
+        atomicMax(&gs->hi[0], h0); atomicMax(&gs->hi[1], h1); atomicMax(&gs->hi[2], h2);
+        atomicAdd(&gs->alive, static_cast<unsigned int>(__popcll(ball)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Propagation
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uint8_t* alive, int n,
+                                                      const MoonSample* __restrict__ moon3,
+                                                      double dt, GridState* gs) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    bool on = false;
+    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+    if (i < n) {
+        D3 r{cur.x[i], cur.y[i], cur.z[i]};
+        D3 v{cur.vx[i], cur.vy[i], cur.vz[i]};
+        rx = r.x; ry = r.y; rz = r.z;
+        vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+        bool a = alive[i] != 0;
+        if (a) {
+            const MoonSample m0 = moon3[0], m1 = moon3[1], m2 = moon3[2];
+            rk4(r, v, dt, m0, m1, m2);
+            if (sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) {
+                a = false;
+                alive[i] = 0;
+            }
+        }
+        nxt.x[i] = r.x; nxt.y[i] = r.y; nxt.z[i] = r.z;
+        nxt.vx[i] = v.x; nxt.vy[i] = v.y; nxt.vz[i] = v.z;
+        on = a;
+    }
+    block_reduce_grid(gs, on, vn, rx, ry, rz);
+}
+
+// Reductions only (screening of host-given states).
+__global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* alive, int n,
+                                                   GridState* gs) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    bool on = false;
+    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+    if (i < n && alive[i]) {
+        on = true;
+        rx = cur.x[i]; ry = cur.y[i]; rz = cur.z[i];
+        const double vx = cur.vx[i], vy = cur.vy[i], vz = cur.vz[i];
+        vn = sqrt(vx * vx + vy * vy + vz * vz);
+    }
+    block_reduce_grid(gs, on, vn, rx, ry, rz);
+}
+
+__global__ void k_reset(GridState* gs) {
+    if (threadIdx.x != 0) return;
+    gs->vmax_bits = 0ULL;
+    for (int a = 0; a < 3; ++a) {
+        gs->lo[a] = ~0ULL;
+        gs->hi[a] = 0ULL;
+    }
+    gs->alive = 0;
+    gs->cand = gs->refined = gs->co = 0ULL;
+    gs->max_cell = 0;
+    gs->ncells = 0;
+    gs->npasses = 0;
+}
+
+__global__ void k_zero_events(GridState* gs) {
+    if (threadIdx.x == 0) gs->events = 0ULL;
+}
+
+// ---------------------------------------------------------------------------
+// Grid
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int bits_for(long long range) {
+    return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
+}
+
+__global__ void k_grid_setup(GridState* gs, double dt, double thr) {
+    if (threadIdx.x != 0) return;
+    const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
+    const double vrel_max = 2.0 * vmax;
+    const double raw = vrel_max * dt + thr + GRAVITY_GRADIENT * vrel_max * dt * dt * dt;
+    double cell = raw > 1.0 ? raw : 1.0;
+    gs->cell_ref = cell;
+    gs->doublings = 0;
+    if (gs->alive == 0) {
+        gs->cell = cell;
+        gs->npasses = 0;
+        return;
+    }
+    const double lo[3] = {from_ordered_bits(gs->lo[0]), from_ordered_bits(gs->lo[1]),
+                          from_ordered_bits(gs->lo[2])};
+    const double hi[3] = {from_ordered_bits(gs->hi[0]), from_ordered_bits(gs->hi[1]),
+                          from_ordered_bits(gs->hi[2])};
+    // floor(x / cell) is monotone in x, so the extreme cells come from the
+    // extreme coordinates. A cell larger than the reference's never changes
+    // which pairs pass the filters (collision.cpp:337-338); it is doubled
+    // only if the packed key would not fit.
+    for (;;) {
+        int total = 0;
+        for (int a = 0; a < 3; ++a) {
+            const long long i0 = static_cast<long long>(floor(lo[a] / cell));
+            const long long i1 = static_cast<long long>(floor(hi[a] / cell));
+            gs->imin[a] = i0;
+            gs->bits[a] = bits_for(i1 - i0);
+            total += gs->bits[a];
+        }
+        if (total <= KEY_BITS_MAX) {
+            gs->npasses = (total + 8) / 8;
+            break;
+        }
+        cell = cell * 2.0;
+        gs->doublings += 1;
+    }
+    gs->cell = cell;
+}
+
+__device__ __forceinline__ uint32_t pack_key(const GridState& g, double x, double y, double z) {
+    const double c = g.cell;
+    const long long ix = static_cast<long long>(floor(x / c)) - g.imin[0];
+    const long long iy = static_cast<long long>(floor(y / c)) - g.imin[1];
+    const long long iz = static_cast<long long>(floor(z / c)) - g.imin[2];
+    return (static_cast<uint32_t>(ix) << (g.bits[1] + g.bits[2])) |
+           (static_cast<uint32_t>(iy) << g.bits[2]) | static_cast<uint32_t>(iz);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, const uint8_t* alive, int n,
+                                                 const GridState* gs, uint32_t* keys, int* idx,
+                                                 int* hist, int ntiles) {
+    __shared__ int s_hist[RADIX];
+    const GridState g = *gs;
+    if (g.npasses == 0) return;
+    s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * TILE;
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = base + r * BLOCK + threadIdx.x;
+        if (e < n) {
+            uint32_t k = DEAD_KEY;
+            if (alive[e]) k = pack_key(g, cur.x[e], cur.y[e], cur.z[e]);
+            keys[e] = k;
+            idx[e] = e;
+            atomicAdd(&s_hist[k & 255u], 1);
+        }
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(BLOCK) k_hist(const uint32_t* keys, int n, int pass,
+                                                 const GridState* gs, int* hist, int ntiles) {
+    __shared__ int s_hist[RADIX];
+    if (pass >= gs->npasses) return;
+    s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int shift = 8 * pass;
+    const int base = blockIdx.x * TILE;
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = base + r * BLOCK + threadIdx.x;
+        if (e < n) atomicAdd(&s_hist[(keys[e] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+}
+
+// Block-wide exclusive scan of one int per thread; returns the total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int& total, int* s_wave) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / WAVE;
+    int incl = v;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int o = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += o;
+    }
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    int before = 0, sum = 0;
+    for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+        if (w < wave) before += s_wave[w];
+        sum += s_wave[w];
+    }
+    __syncthreads();
+    total = sum;
+    return before + incl - v;
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts.
+__global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int pass,
+                                                       const GridState* gs, int* totals) {
+    __shared__ int s_wave[WAVES_PER_BLOCK];
+    if (pass >= gs->npasses) return;
+    int* row = hist + blockIdx.x * ntiles;
+    int carry = 0;
+    for (int b0 = 0; b0 < ntiles; b0 += BLOCK) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < ntiles ? row[b] : 0;
+        int tot;
+        const int ex = block_exclusive_scan(v, tot, s_wave);
+        if (b < ntiles) row[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+// Stable scatter of one radix pass. In-tile ranks: elements are taken in
+// index order, 256 per round; a wave finds the lanes holding the same digit
+// with 8 ballots, and per-wave digit counts in LDS give the cross-wave
+// offsets.
+__global__ void __launch_bounds__(BLOCK) k_scatter(const uint32_t* keys_in, const int* idx_in,
+                                                    uint32_t* keys_out, int* idx_out, int n,
+                                                    int pass, const GridState* gs,
+                                                    const int* hist, const int* totals,
+                                                    int ntiles) {
+    __shared__ int s_off[RADIX];
+    __shared__ int s_wcnt[WAVES_PER_BLOCK][RADIX];
+    __shared__ int s_wave[WAVES_PER_BLOCK];
+    if (pass >= gs->npasses) return;
+    const int t = threadIdx.x;
+    {
+        int tot;
+        const int digit_base = block_exclusive_scan(totals[t], tot, s_wave);
+        s_off[t] = digit_base + hist[t * ntiles + blockIdx.x];
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) s_wcnt[w][t] = 0;
+    }
+    __syncthreads();
+    const int shift = 8 * pass;
+    const int wave = t / WAVE;
+    const unsigned long long below = lanes_below();
+    const int base = blockIdx.x * TILE;
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = base + r * BLOCK + t;
+        const bool valid = e < n;
+        const uint32_t key = valid ? keys_in[e] : 0u;
+        const int id = valid ? idx_in[e] : 0;
+        const int d = static_cast<int>((key >> shift) & 255u);
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pre = 0;
+            for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
+            const int pos = s_off[d] + pre + rank;
+            keys_out[pos] = key;
+            idx_out[pos] = id;
+        }
+        __syncthreads();
+        int add = 0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+            add += s_wcnt[w][t];
+            s_wcnt[w][t] = 0;
+        }
+        s_off[t] += add;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ const uint32_t* sorted_keys(const GridState& g, const uint32_t* k0,
+                                                       const uint32_t* k1) {
+    return (g.npasses & 1) ? k1 : k0;
+}
+__device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* i0, const int* i1) {
+    return (g.npasses & 1) ? i1 : i0;
+}
+
+// Cell heads per tile over the first `alive` sorted elements.
+__global__ void __launch_bounds__(BLOCK) k_cell_count(const uint32_t* k0, const uint32_t* k1,
+                                                       const GridState* gs, int* bcount) {
+    __shared__ int s_wave[WAVES_PER_BLOCK];
+    const GridState& g = *gs;
+    const int A = static_cast<int>(g.alive);
+    const uint32_t* keys = sorted_keys(g, k0, k1);
+    const int base = blockIdx.x * TILE;
+    int cnt = 0;
+    if (base < A && g.npasses > 0) {
+        for (int r = 0; r < ITEMS; ++r) {
+            const int e = base + r * BLOCK + threadIdx.x;
+            if (e < A && (e == 0 || keys[e] != keys[e - 1])) ++cnt;
+        }
+    }
+    int tot;
+    block_exclusive_scan(cnt, tot, s_wave);
+    if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, GridState* gs) {
+    __shared__ int s_wave[WAVES_PER_BLOCK];
+    int carry = 0;
+    for (int b0 = 0; b0 < ntiles; b0 += BLOCK) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < ntiles ? bcount[b] : 0;
+        int tot;
+        const int ex = block_exclusive_scan(v, tot, s_wave);
+        if (b < ntiles) bcount[b] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) gs->ncells = static_cast<unsigned int>(carry);
+}
+
+// Compact per-object record for the pair kernel, in sorted order.
+//   pos: position relative to its cell's low corner (FP64 subtraction,
+//        rounded to float: the error DR_ERR_KM bounds), w = object index bits.
+//   vel: velocity rounded to float (DV_ERR_KM_S), w unused.
+__global__ void __launch_bounds__(BLOCK) k_cell_write(const uint32_t* k0, const uint32_t* k1,
+                                                       const int* i0, const int* i1,
+                                                       const GridState* gs, const int* boff,
+                                                       Planes cur, uint32_t* cell_key,
+                                                       int* cell_begin, int* cell_of,
+                                                       float4* lpos, float4* lvel) {
+    __shared__ int s_wcnt[WAVES_PER_BLOCK];
+    __shared__ int s_run;
+    const GridState& g = *gs;
+    const int A = static_cast<int>(g.alive);
+    const int base = blockIdx.x * TILE;
+    if (base >= A || g.npasses == 0) return;
+    const uint32_t* keys = sorted_keys(g, k0, k1);
+    const int* sidx = sorted_idx(g, i0, i1);
+    const int wave = threadIdx.x / WAVE;
+    const unsigned long long below = lanes_below();
+    const double cell = g.cell;
+    const int sy = g.bits[2], sx = g.bits[1] + g.bits[2];
+    const uint32_t mz = (1u << g.bits[2]) - 1u, my = (1u << g.bits[1]) - 1u;
+    if (threadIdx.x == 0) s_run = boff[blockIdx.x];
+    __syncthreads();
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = base + r * BLOCK + threadIdx.x;
+        const bool valid = e < A;
+        uint32_t key = 0;
+        bool head = false;
+        if (valid) {
+            key = keys[e];
+            head = e == 0 || key != keys[e - 1];
+        }
+        const unsigned long long hm = __ballot(head);
+        if (__lane_id() == 0) s_wcnt[wave] = __popcll(hm);
+        __syncthreads();
+        int pre = s_run;
+        for (int w = 0; w < wave; ++w) pre += s_wcnt[w];
+        if (valid) {
+            const int c = pre + __popcll(hm & below) + (head ? 1 : 0) - 1;
+            cell_of[e] = c;
+            if (head) {
+                cell_key[c] = key;
+                cell_begin[c] = e;
+            }
+            const int id = sidx[e];
+            const long long ix = static_cast<long long>(key >> sx) + g.imin[0];
+            const long long iy = static_cast<long long>((key >> sy) & my) + g.imin[1];
+            const long long iz = static_cast<long long>(key & mz) + g.imin[2];
+            lpos[e] = make_float4(static_cast<float>(cur.x[id] - static_cast<double>(ix) * cell),
+                                  static_cast<float>(cur.y[id] - static_cast<double>(iy) * cell),
+                                  static_cast<float>(cur.z[id] - static_cast<double>(iz) * cell),
+                                  __int_as_float(id));
+            lvel[e] = make_float4(static_cast<float>(cur.vx[id]), static_cast<float>(cur.vy[id]),
+                                  static_cast<float>(cur.vz[id]), 0.0f);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int add = 0;
+            for (int w = 0; w < WAVES_PER_BLOCK; ++w) add += s_wcnt[w];
+            s_run += add;
+        }
+        __syncthreads();
+    }
+}
+
+// Offsets of the 13 forward neighbours, in the order of
+// KESSLER_FORWARD_NEIGHBOURS, without a private-memory table.
+__device__ __forceinline__ void forward_offset(int o, int& dx, int& dy, int& dz) {
+    if (o < 9) {
+        dx = 1;
+        dy = o / 3 - 1;
+        dz = o % 3 - 1;
+    } else if (o < 12) {
+        dx = 0;
+        dy = 1;
+        dz = o - 10;
+    } else {
+        dx = 0;
+        dy = 0;
+        dz = 1;
+    }
+}
+
+__device__ __forceinline__ int find_cell(const uint32_t* ck, int ncells, uint32_t key) {
+    int lo = 0, hi = ncells;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ck[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < ncells && ck[lo] == key) ? lo : -1;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint32_t* cell_key,
+                                                       const int* cell_begin, int* cell_count,
+                                                       int* nbr) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int nc = static_cast<int>(gs->ncells);
+    unsigned int cnt = 0;
+    if (c < nc) {
+        const int A = static_cast<int>(gs->alive);
+        const int b = cell_begin[c];
+        cnt = static_cast<unsigned int>((c + 1 < nc ? cell_begin[c + 1] : A) - b);
+        cell_count[c] = static_cast<int>(cnt);
+        const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
+        const uint32_t key = cell_key[c];
+        const long long x = key >> (by + bz);
+        const long long y = (key >> bz) & ((1u << by) - 1u);
+        const long long z = key & ((1u << bz) - 1u);
+        for (int o = 0; o < 13; ++o) {
+            int ox, oy, oz;
+            forward_offset(o, ox, oy, oz);
+            const long long nx = x + ox, ny = y + oy, nz = z + oz;
+            int found = -1;
+            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (1LL << bx) && ny < (1LL << by) &&
+                nz < (1LL << bz)) {
+                const uint32_t nk = (static_cast<uint32_t>(nx) << (by + bz)) |
+                                    (static_cast<uint32_t>(ny) << bz) | static_cast<uint32_t>(nz);
+                found = find_cell(cell_key, nc, nk);
+            }
+            nbr[c * 13 + o] = found;
+        }
+    }
+    const unsigned long long m = wave_max_u64(cnt);
+    if (__lane_id() == 0 && m > 0) atomicMax(&gs->max_cell, static_cast<unsigned int>(m));
+}
+
+// ---------------------------------------------------------------------------
+// Pairs
+// ---------------------------------------------------------------------------
+
+struct PairParams {
+    Planes before, after;
+    const double* radius;   // collision radius (km) per object
+    const int* cell_of;
+    const int* cell_begin;
+    const int* cell_count;
+    const int* nbr;
+    const float4* lpos;
+    const float4* lvel;
+    DevEvent* events;
+    unsigned long long cap;
+    long long step;
+    double t0, dt, thr, vmin2;
+    float dt_f, thr_f, vmin2_f;
+};
+
+struct PairCounters {
+    unsigned int cand, refined, co;
+};
+
+__device__ __forceinline__ void append_event(GridState* gs, const PairParams& P, const DevEvent& ev) {
+    // Wave-aggregated append among the lanes that reach this point together.
+    const unsigned long long m = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    const int rank = __popcll(m & lanes_below());
+    unsigned long long base = 0;
+    if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(&gs->events, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader, WAVE);
+    const unsigned long long slot = base + rank;
+    if (slot < P.cap) P.events[slot] = ev;
+}
+
+__device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, int a, int b,
+                                           PairCounters& cnt) {
+    const int i = a < b ? a : b, j = a < b ? b : a;
+    const D3 ri{P.before.x[i], P.before.y[i], P.before.z[i]};
+    const D3 rj{P.before.x[j], P.before.y[j], P.before.z[j]};
+    const D3 vi{P.before.vx[i], P.before.vy[i], P.before.vz[i]};
+    const D3 vj{P.before.vx[j], P.before.vy[j], P.before.vz[j]};
+    const int res = filter_pair(rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, vj.x - vi.x, vj.y - vi.y,
+                                vj.z - vi.z, P.dt, P.thr, P.vmin2);
+    if (res == FILTER_CO_ORBITING) {
+        ++cnt.co;
+        return;
+    }
+    if (res != FILTER_REFINE) return;
+    ++cnt.refined;
+    const D3 ai{P.after.x[i], P.after.y[i], P.after.z[i]};
+    const D3 aj{P.after.x[j], P.after.y[j], P.after.z[j]};
+    const D3 wi{P.after.vx[i], P.after.vy[i], P.after.vz[i]};
+    const D3 wj{P.after.vx[j], P.after.vy[j], P.after.vz[j]};
+    const double dt = P.dt;
+    const Hermite h{{rj.x - ri.x, rj.y - ri.y, rj.z - ri.z},
+                    {(vj.x - vi.x) * dt, (vj.y - vi.y) * dt, (vj.z - vi.z) * dt},
+                    {aj.x - ai.x, aj.y - ai.y, aj.z - ai.z},
+                    {(wj.x - wi.x) * dt, (wj.y - wi.y) * dt, (wj.z - wi.z) * dt}};
+    const double s = golden_min(h);
+    if (s <= 1e-9 || s >= 1.0 - 1e-9) return;
+    const D3 p = h.pos(s);
+    const double miss = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    if (miss > P.thr) return;
+    const D3 w = h.vel(s);
+    DevEvent ev;
+    ev.i = i;
+    ev.j = j;
+    ev.step = P.step;
+    ev.t = P.t0 + s * dt;
+    ev.miss_km = miss;
+    ev.v_rel_km_s = sqrt(w.x * w.x + w.y * w.y + w.z * w.z) / dt;
+    ev.collision = miss < P.radius[i] + P.radius[j] ? 1 : 0;
+    ev.pad = 0;
+    append_event(gs, P, ev);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P) {
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int A = static_cast<int>(gs->alive);
+    PairCounters cnt{0, 0, 0};
+    if (p < A && gs->npasses > 0) {
+        const float cellf = static_cast<float>(gs->cell);
+        const int c = P.cell_of[p];
+        const int begin = P.cell_begin[c];
+        const int k = P.cell_count[c];
+        const int u = p - begin;
+        const float4 la = P.lpos[p], va = P.lvel[p];
+        const int ia = __float_as_int(la.w);
+        auto visit = [&](int q, float ox, float oy, float oz) {
+            ++cnt.cand;
+            const float4 lb = P.lpos[q], vb = P.lvel[q];
+            if (!prefilter_keep(ox * cellf + (lb.x - la.x), oy * cellf + (lb.y - la.y),
+                                oz * cellf + (lb.z - la.z), vb.x - va.x, vb.y - va.y,
+                                vb.z - va.z, P.dt_f, P.thr_f, P.vmin2_f))
+                return;
+            exact_pair(gs, P, ia, __float_as_int(lb.w), cnt);
+        };
+        // Intra-cell pairs, balanced: lane u sees the floor((k-1)/2) objects
+        // after it (cyclically), plus the opposite one when k is even and
+        // u < k/2. Every pair once, equal work per lane.
+        const int half = (k - 1) / 2;
+        for (int m = 1; m <= half; ++m) {
+            int q = u + m;
+            if (q >= k) q -= k;
+            visit(begin + q, 0.0f, 0.0f, 0.0f);
+        }
+        if ((k & 1) == 0 && u < k / 2) visit(begin + u + k / 2, 0.0f, 0.0f, 0.0f);
+        for (int o = 0; o < 13; ++o) {
+            const int nc = P.nbr[c * 13 + o];
+            if (nc < 0) continue;
+            int dx, dy, dz;
+            forward_offset(o, dx, dy, dz);
+            const float ox = static_cast<float>(dx);
+            const float oy = static_cast<float>(dy);
+            const float oz = static_cast<float>(dz);
+            const int b0 = P.cell_begin[nc], b1 = b0 + P.cell_count[nc];
+            for (int q = b0; q < b1; ++q) visit(q, ox, oy, oz);
+        }
+    }
+    const unsigned long long c0 = wave_sum<unsigned long long>(cnt.cand);
+    const unsigned long long c1 = wave_sum<unsigned long long>(cnt.refined);
+    const unsigned long long c2 = wave_sum<unsigned long long>(cnt.co);
+    if (__lane_id() == 0) {
+        if (c0) atomicAdd(&gs->cand, c0);
+        if (c1) atomicAdd(&gs->refined, c1);
+        if (c2) atomicAdd(&gs->co, c2);
+    }
+}
+
+__global__ void k_finish(GridState* gs, DevStats* out, int screened) {
+    if (threadIdx.x != 0) return;
+    if (screened) {
+        DevStats s;
+        s.cand = static_cast<long long>(gs->cand);
+        s.refined = static_cast<long long>(gs->refined);
+        s.co = static_cast<long long>(gs->co);
+        s.cell_km = gs->cell_ref;
+        s.max_cell = static_cast<int>(gs->max_cell);
+        s.doublings = gs->doublings;
+        *out = s;
+    }
+    gs->vmax_bits = 0ULL;
+    for (int a = 0; a < 3; ++a) {
+        gs->lo[a] = ~0ULL;
+        gs->hi[a] = 0ULL;
+    }
+    gs->alive = 0;
+    gs->cand = gs->refined = gs->co = 0ULL;
+    gs->max_cell = 0;
+    gs->ncells = 0;
+    gs->npasses = 0;
+}
+
+__global__ void k_fp64_check(int n, const double* a, const double* b, double* out) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const double x = a[k], y = b[k];
+    out[3 * k + 0] = sqrt(x);
+    out[3 * k + 1] = x / y;
+    out[3 * k + 2] = floor(x / y);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+struct khip_ctx {
+    int device = 0;
+    std::string name, error;
+    hipStream_t stream = nullptr;
+
+    int64_t n = 0, cap = 0;
+    double* state[2] = {nullptr, nullptr};  // SoA planes, 6 * cap doubles each
+    int cur = 0;
+    uint8_t* alive = nullptr;
+    double* radius = nullptr;
+    double* size_m = nullptr;
+    std::vector<double> host_size;
+
+    // Grid buffers.
+    uint32_t* keys[2] = {nullptr, nullptr};
+    int* idx[2] = {nullptr, nullptr};
+    int* hist = nullptr;
+    int* totals = nullptr;
+    int* bcount = nullptr;
+    uint32_t* cell_key = nullptr;
+    int* cell_begin = nullptr;
+    int* cell_count = nullptr;
+    int* cell_of = nullptr;
+    int* nbr = nullptr;
+    float4* lpos = nullptr;
+    float4* lvel = nullptr;
+    GridState* gs = nullptr;
+
+    // Snapshot for re-running a chunk after an event-buffer overflow.
+    double* snap_state = nullptr;
+    uint8_t* snap_alive = nullptr;
+
+    DevEvent* events = nullptr;
+    int64_t ev_cap = 0;
+    DevStats* stats = nullptr;
+    int64_t stats_cap = 0;
+    MoonSample* moon = nullptr;
+    int64_t moon_cap = 0;
+
+    khip_screen_config cfg{5.0, 1.0, 0.01};
+    double epoch_jd = JD_DEFAULT_EPOCH, t = 0.0, dt = 10.0;
+    int64_t step = 0;
+    int chunk = 256;
+
+    std::vector<khip_event> host_events;
+    std::vector<khip_stats> host_stats;
+
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;   // 4 per step of a chunk
+    khip_timing tm{};
+    std::vector<MoonSample> moon_host;
+
+    Planes planes(int which) const {
+        double* b = state[which];
+        const size_t c = static_cast<size_t>(cap);
+        return Planes{b, b + c, b + 2 * c, b + 3 * c, b + 4 * c, b + 5 * c};
+    }
+    int ntiles() const { return static_cast<int>((n + TILE - 1) / TILE); }
+};
+
+namespace {
+
+bool fail(khip_ctx* ctx, const std::string& what, hipError_t e) {
+    ctx->error = what + ": " + hipGetErrorString(e);
+    return false;
+}
+
+#define KHIP_TRY(ctx, call)                                         \
+    do {                                                            \
+        hipError_t err_ = (call);                                   \
+        if (err_ != hipSuccess) return fail((ctx), #call, err_);    \
+    } while (0)
+
+template <typename T>
+bool dev_alloc(khip_ctx* ctx, T** p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * count));
+    return true;
+}
+
+void free_all(khip_ctx* ctx) {
+    void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
+                    ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
+                    ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
+                    ctx->cell_of, ctx->nbr, ctx->lpos, ctx->lvel, ctx->snap_state,
+                    ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+}
+
+bool ensure_capacity(khip_ctx* ctx, int64_t n) {
+    if (n <= ctx->cap) return true;
+    // Objects are re-uploaded by the caller after a growth.
+    const int64_t cap = n + n / 2 + 1024;
+    const int64_t tiles = (cap + TILE - 1) / TILE;
+    ctx->cap = cap;
+    for (int b = 0; b < 2; ++b)
+        if (!dev_alloc(ctx, &ctx->state[b], 6 * static_cast<size_t>(cap))) return false;
+    return dev_alloc(ctx, &ctx->alive, cap) && dev_alloc(ctx, &ctx->radius, cap) &&
+           dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->keys[0], cap) &&
+           dev_alloc(ctx, &ctx->keys[1], cap) && dev_alloc(ctx, &ctx->idx[0], cap) &&
+           dev_alloc(ctx, &ctx->idx[1], cap) && dev_alloc(ctx, &ctx->hist, RADIX * tiles) &&
+           dev_alloc(ctx, &ctx->totals, RADIX) && dev_alloc(ctx, &ctx->bcount, tiles) &&
+           dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
+           dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
+           dev_alloc(ctx, &ctx->nbr, 13 * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
+           dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->snap_alive, cap);
+}
+
+bool ensure_events(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->ev_cap) return true;
+    if (!dev_alloc(ctx, &ctx->events, want)) return false;
+    ctx->ev_cap = want;
+    return true;
+}
+
+bool ensure_stats(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->stats_cap) return true;
+    if (!dev_alloc(ctx, &ctx->stats, want)) return false;
+    ctx->stats_cap = want;
+    return true;
+}
+
+// Moon samples for steps [0, nsteps) starting at time t: entry 2k is the
+// Moon at the start of step k, 2k+1 at its midpoint; entry 2k+2 (the end of
+// step k) is the start of step k+1, bit for bit, since both are evaluated at
+// the same double t + dt (orbital.cpp:287-289, 299).
+bool upload_moon(khip_ctx* ctx, double t_start, int64_t nsteps) {
+    const int64_t entries = 2 * nsteps + 1;
+    if (entries > ctx->moon_cap) {
+        if (!dev_alloc(ctx, &ctx->moon, entries)) return false;
+        ctx->moon_cap = entries;
+    }
+    std::vector<MoonSample>& tab = ctx->moon_host;
+    tab.resize(static_cast<size_t>(entries));
+    double t = t_start;
+    auto fill = [&](MoonSample& s, const Vec3& m) {
+        s.m[0] = m.x; s.m[1] = m.y; s.m[2] = m.z;
+        const double mn = norm(m);
+        const double f = kessler::MU_MOON / (mn * mn * mn);
+        s.ind[0] = m.x * f; s.ind[1] = m.y * f; s.ind[2] = m.z * f;
+    };
+    for (int64_t k = 0; k < nsteps; ++k) {
+        fill(tab[2 * k], moon_position(ctx->epoch_jd + t / 86400.0));
+        fill(tab[2 * k + 1], moon_position(ctx->epoch_jd + (t + ctx->dt / 2) / 86400.0));
+        fill(tab[2 * k + 2], moon_position(ctx->epoch_jd + (t + ctx->dt) / 86400.0));
+        t += ctx->dt;
+    }
+    KHIP_TRY(ctx, hipMemcpyAsync(ctx->moon, tab.data(), sizeof(MoonSample) * entries,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    return true;
+}
+
+inline int grid_of(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
+
+// Timing marks of step slot `slot` in the current chunk: 0 before the
+// propagation, 1 after it, 2 before the pair kernel, 3 after it.
+int g_slot = 0;
+
+bool ensure_event_pool(khip_ctx* ctx, int64_t steps) {
+    while (static_cast<int64_t>(ctx->ev_pool.size()) < 4 * steps) {
+        hipEvent_t e;
+        KHIP_TRY(ctx, hipEventCreate(&e));
+        ctx->ev_pool.push_back(e);
+    }
+    return true;
+}
+
+void mark(khip_ctx* ctx, int k) {
+    if (ctx->timing) (void)hipEventRecord(ctx->ev_pool[4 * g_slot + k], ctx->stream);
+}
+
+// Enqueues the screening kernels for the step whose start / end states are
+// `before` / `after` (stats record `stat_slot`).
+bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t step, int stat_slot) {
+    const int n = static_cast<int>(ctx->n);
+    const int tiles = ctx->ntiles();
+    hipStream_t s = ctx->stream;
+    const Planes pb = ctx->planes(before), pa = ctx->planes(after);
+    k_grid_setup<<<1, 64, 0, s>>>(ctx->gs, ctx->dt, ctx->cfg.threshold_km);
+    k_keys<<<tiles, BLOCK, 0, s>>>(pb, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
+                                   ctx->hist, tiles);
+    for (int pass = 0; pass < MAX_PASSES; ++pass) {
+        const int src = pass & 1, dst = src ^ 1;
+        if (pass > 0)
+            k_hist<<<tiles, BLOCK, 0, s>>>(ctx->keys[src], n, pass, ctx->gs, ctx->hist, tiles);
+        k_digit_scan<<<RADIX, BLOCK, 0, s>>>(ctx->hist, tiles, pass, ctx->gs, ctx->totals);
+        k_scatter<<<tiles, BLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst],
+                                          ctx->idx[dst], n, pass, ctx->gs, ctx->hist, ctx->totals,
+                                          tiles);
+    }
+    k_cell_count<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
+    k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
+    k_cell_write<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
+                                         ctx->gs, ctx->bcount, pb, ctx->cell_key, ctx->cell_begin,
+                                         ctx->cell_of, ctx->lpos, ctx->lvel);
+    k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
+                                              ctx->cell_count, ctx->nbr);
+    mark(ctx, 2);
+    PairParams P;
+    P.before = pb;
+    P.after = pa;
+    P.radius = ctx->radius;
+    P.cell_of = ctx->cell_of;
+    P.cell_begin = ctx->cell_begin;
+    P.cell_count = ctx->cell_count;
+    P.nbr = ctx->nbr;
+    P.lpos = ctx->lpos;
+    P.lvel = ctx->lvel;
+    P.events = ctx->events;
+    P.cap = static_cast<unsigned long long>(ctx->ev_cap);
+    P.step = step;
+    P.t0 = t0;
+    P.dt = ctx->dt;
+    P.thr = ctx->cfg.threshold_km;
+    P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
+    P.dt_f = static_cast<float>(P.dt);
+    P.thr_f = static_cast<float>(P.thr);
+    P.vmin2_f = static_cast<float>(P.vmin2);
+    k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P);
+    mark(ctx, 3);
+    k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1);
+    KHIP_TRY(ctx, hipGetLastError());
+    return true;
+}
+
+// Called after the chunk's stream synchronisation.
+void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
+    for (int64_t k = 0; k < steps; ++k) {
+        hipEvent_t* e = ctx->ev_pool.data() + 4 * k;
+        float a = 0, b = 0, c = 0, d = 0;
+        (void)hipEventElapsedTime(&a, e[0], e[1]);
+        ctx->tm.propagate_ms += a;
+        if (screened) {
+            (void)hipEventElapsedTime(&b, e[1], e[2]);
+            (void)hipEventElapsedTime(&c, e[2], e[3]);
+            (void)hipEventElapsedTime(&d, e[0], e[3]);
+            ctx->tm.grid_ms += b;
+            ctx->tm.pairs_ms += c;
+            ctx->tm.total_ms += d;
+        } else {
+            (void)hipEventElapsedTime(&d, e[0], e[3]);
+            ctx->tm.total_ms += d;
+        }
+        ctx->tm.steps += 1;
+    }
+}
+
+bool drain_chunk(khip_ctx* ctx, int64_t steps, int screen, bool& overflow) {
+    GridState g;
+    KHIP_TRY(ctx, hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
+    KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    overflow = static_cast<int64_t>(g.events) > ctx->ev_cap;
+    if (overflow) {
+        if (!ensure_events(ctx, 2 * static_cast<int64_t>(g.events) + 1024)) return false;
+        return true;
+    }
+    if (!screen) return true;
+    const int64_t ne = static_cast<int64_t>(g.events);
+    std::vector<khip_event> ev(static_cast<size_t>(ne));
+    if (ne > 0)
+        KHIP_TRY(ctx, hipMemcpy(ev.data(), ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
+    std::sort(ev.begin(), ev.end(), [](const khip_event& a, const khip_event& b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.t != b.t) return a.t < b.t;
+        if (a.i != b.i) return a.i < b.i;
+        return a.j < b.j;
+    });
+    ctx->host_events.insert(ctx->host_events.end(), ev.begin(), ev.end());
+    std::vector<khip_stats> st(static_cast<size_t>(steps));
+    KHIP_TRY(ctx, hipMemcpy(st.data(), ctx->stats, sizeof(khip_stats) * steps, hipMemcpyDeviceToHost));
+    ctx->host_stats.insert(ctx->host_stats.end(), st.begin(), st.end());
+    return true;
+}
+
+bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
+    const int n = static_cast<int>(ctx->n);
+    hipStream_t s = ctx->stream;
+    if (n == 0) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            if (screen) ctx->host_stats.push_back(khip_stats{0, 0, 0, 1.0, 0, 0});
+            ctx->t += ctx->dt;
+            ctx->step += 1;
+        }
+        return true;
+    }
+    int64_t done = 0;
+    while (done < nsteps) {
+        const int64_t len = std::min<int64_t>(ctx->chunk, nsteps - done);
+        if (!ensure_stats(ctx, len) || !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, 1 << 16)) ||
+            (ctx->timing && !ensure_event_pool(ctx, len)))
+            return false;
+        if (!upload_moon(ctx, ctx->t, len)) return false;
+        // Snapshot of the chunk's starting population, for a re-run.
+        const size_t c = static_cast<size_t>(ctx->cap);
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
+                                     hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_alive, ctx->alive, c, hipMemcpyDeviceToDevice, s));
+        const int cur0 = ctx->cur;
+        const double t_chunk = ctx->t;
+        for (;;) {
+            k_reset<<<1, 64, 0, s>>>(ctx->gs);
+            k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+            double t = t_chunk;
+            int cur = cur0;
+            for (int64_t k = 0; k < len; ++k) {
+                const int nxt = cur ^ 1;
+                g_slot = static_cast<int>(k);
+                mark(ctx, 0);
+                k_propagate<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(cur), ctx->planes(nxt),
+                                                         ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
+                                                         ctx->gs);
+                mark(ctx, 1);
+                if (screen) {
+                    if (!enqueue_screen(ctx, cur, nxt, t, ctx->step + done + k, static_cast<int>(k)))
+                        return false;
+                } else {
+                    k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats, 0);
+                    mark(ctx, 3);
+                }
+                t += ctx->dt;
+                cur = nxt;
+            }
+            KHIP_TRY(ctx, hipGetLastError());
+            bool overflow = false;
+            if (!drain_chunk(ctx, len, screen, overflow)) return false;
+            if (!overflow) {
+                if (ctx->timing) accumulate_timing(ctx, len, screen != 0);
+                ctx->cur = cur;
+                ctx->t = t;
+                break;
+            }
+            // Event buffer overflowed: restore the chunk's start and re-run.
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], ctx->snap_state, sizeof(double) * 6 * c,
+                                         hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, ctx->snap_alive, c, hipMemcpyDeviceToDevice, s));
+        }
+        done += len;
+    }
+    ctx->step += nsteps;
+    return true;
+}
+
+bool upload_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
+                    const double* size_m) {
+    if (!ensure_capacity(ctx, n)) return false;
+    ctx->n = n;
+    ctx->cur = 0;
+    ctx->host_size.assign(static_cast<size_t>(n), 0.8);
+    if (n == 0) return true;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    std::vector<double> planes(6 * c, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 6; ++k) planes[k * c + i] = states[6 * i + k];
+    KHIP_TRY(ctx, hipMemcpy(ctx->state[0], planes.data(), sizeof(double) * 6 * c, hipMemcpyHostToDevice));
+    KHIP_TRY(ctx, hipMemcpy(ctx->state[1], planes.data(), sizeof(double) * 6 * c, hipMemcpyHostToDevice));
+    std::vector<uint8_t> a(c, 0);
+    for (int64_t i = 0; i < n; ++i) a[i] = alive ? (alive[i] ? 1 : 0) : 1;
+    KHIP_TRY(ctx, hipMemcpy(ctx->alive, a.data(), c, hipMemcpyHostToDevice));
+    for (int64_t i = 0; i < n; ++i) ctx->host_size[i] = size_m ? size_m[i] : 0.8;
+    std::vector<double> rad(c, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        rad[i] = 0.5 * ctx->host_size[i] * 1e-3 * ctx->cfg.radius_scale;
+    KHIP_TRY(ctx, hipMemcpy(ctx->radius, rad.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    return true;
+}
+
+bool refresh_radius(khip_ctx* ctx) {
+    if (ctx->n == 0) return true;
+    std::vector<double> rad(static_cast<size_t>(ctx->n));
+    for (int64_t i = 0; i < ctx->n; ++i)
+        rad[i] = 0.5 * ctx->host_size[i] * 1e-3 * ctx->cfg.radius_scale;
+    KHIP_TRY(ctx, hipMemcpy(ctx->radius, rad.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice));
+    return true;
+}
+
+int status(khip_ctx* ctx, bool ok) { return ok ? KHIP_OK : KHIP_ERR_HIP; }
+
+}  // namespace
+
+extern "C" {
+
+int khip_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
+    auto say = [&](const std::string& s) {
+        if (msg && msg_cap > 0) std::snprintf(msg, msg_cap, "%s", s.c_str());
+    };
+    if (!out) return KHIP_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        say("no HIP device");
+        return KHIP_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= count) {
+        say("device index out of range");
+        return KHIP_ERR_ARG;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        say("hipSetDevice failed");
+        return KHIP_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        say("hipGetDeviceProperties failed");
+        return KHIP_ERR_HIP;
+    }
+    khip_ctx* ctx = new khip_ctx;
+    ctx->device = device;
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName,
+                  prop.multiProcessorCount, prop.totalGlobalMem / 1e9);
+    ctx->name = buf;
+    bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&ctx->gs), sizeof(GridState)) == hipSuccess;
+    if (ok) {
+        k_reset<<<1, 64, 0, ctx->stream>>>(ctx->gs);
+        k_zero_events<<<1, 64, 0, ctx->stream>>>(ctx->gs);
+        ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
+    }
+    if (!ok) {
+        say("initialisation failed on " + ctx->name);
+        khip_destroy(ctx);
+        return KHIP_ERR_HIP;
+    }
+    say(ctx->name);
+    *out = ctx;
+    return KHIP_OK;
+}
+
+void khip_destroy(khip_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* khip_last_error(const khip_ctx* ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
+                     const double* size_m) {
+    if (!ctx || n < 0 || (n > 0 && !states) || n >= (1LL << 31) - TILE) return KHIP_ERR_ARG;
+    return status(ctx, upload_objects(ctx, n, states, alive, size_m));
+}
+
+int64_t khip_object_count(const khip_ctx* ctx) { return ctx ? ctx->n : -1; }
+
+int khip_get_objects(khip_ctx* ctx, double* states, uint8_t* alive) {
+    if (!ctx) return KHIP_ERR_ARG;
+    if (ctx->n == 0) return KHIP_OK;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    if (states) {
+        std::vector<double> planes(6 * c);
+        if (hipMemcpy(planes.data(), ctx->state[ctx->cur], sizeof(double) * 6 * c,
+                      hipMemcpyDeviceToHost) != hipSuccess) {
+            ctx->error = "hipMemcpy of states failed";
+            return KHIP_ERR_HIP;
+        }
+        for (int64_t i = 0; i < ctx->n; ++i)
+            for (int k = 0; k < 6; ++k) states[6 * i + k] = planes[k * c + i];
+    }
+    if (alive && hipMemcpy(alive, ctx->alive, ctx->n, hipMemcpyDeviceToHost) != hipSuccess) {
+        ctx->error = "hipMemcpy of alive flags failed";
+        return KHIP_ERR_HIP;
+    }
+    return KHIP_OK;
+}
+
+int khip_set_config(khip_ctx* ctx, const khip_screen_config* cfg) {
+    if (!ctx || !cfg || !(cfg->threshold_km >= 0) || !(cfg->radius_scale >= 0) ||
+        !(cfg->min_encounter_speed_km_s >= 0))
+        return KHIP_ERR_ARG;
+    ctx->cfg = *cfg;
+    return status(ctx, refresh_radius(ctx));
+}
+
+int khip_set_time(khip_ctx* ctx, double epoch_jd, double t, double dt) {
+    if (!ctx || !(dt > 0)) return KHIP_ERR_ARG;
+    ctx->epoch_jd = epoch_jd;
+    ctx->t = t;
+    ctx->dt = dt;
+    ctx->step = 0;
+    return KHIP_OK;
+}
+
+int khip_get_time(const khip_ctx* ctx, double* t, int64_t* step) {
+    if (!ctx) return KHIP_ERR_ARG;
+    if (t) *t = ctx->t;
+    if (step) *step = ctx->step;
+    return KHIP_OK;
+}
+
+int khip_run(khip_ctx* ctx, int64_t nsteps, int screen) {
+    if (!ctx || nsteps < 0) return KHIP_ERR_ARG;
+    return status(ctx, run_steps(ctx, nsteps, screen));
+}
+
+int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* after,
+                const uint8_t* alive_after, const double* size_m, double t0, double dt,
+                khip_event* events, int64_t cap, int64_t* nevents, khip_stats* stats) {
+    if (!ctx || n < 0 || (n > 0 && (!before || !after)) || !(dt > 0)) return KHIP_ERR_ARG;
+    const double dt_saved = ctx->dt;
+    ctx->dt = dt;
+    bool ok = upload_objects(ctx, n, before, alive_after, size_m);
+    if (ok && n > 0) {
+        // Second buffer <- end-of-step states.
+        const size_t c = static_cast<size_t>(ctx->cap);
+        std::vector<double> planes(6 * c, 0.0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 6; ++k) planes[k * c + i] = after[6 * i + k];
+        ok = hipMemcpy(ctx->state[1], planes.data(), sizeof(double) * 6 * c, hipMemcpyHostToDevice) ==
+             hipSuccess;
+        if (!ok) ctx->error = "hipMemcpy of end-of-step states failed";
+    }
+    ok = ok && ensure_stats(ctx, 1) && ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, 1 << 16));
+    while (ok) {
+        hipStream_t s = ctx->stream;
+        k_reset<<<1, 64, 0, s>>>(ctx->gs);
+        k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+        g_slot = 0;
+        if (ctx->timing && !ensure_event_pool(ctx, 1)) { ok = false; break; }
+        if (n > 0) {
+            mark(ctx, 0);
+            mark(ctx, 1);
+            k_reduce<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n), ctx->gs);
+            ok = enqueue_screen(ctx, 0, 1, t0, 0, 0);
+        } else {
+            k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats, 1);
+        }
+        if (!ok) break;
+        std::vector<khip_event> saved_ev;
+        std::vector<khip_stats> saved_st;
+        saved_ev.swap(ctx->host_events);
+        saved_st.swap(ctx->host_stats);
+        bool overflow = false;
+        ok = drain_chunk(ctx, 1, 1, overflow);
+        std::vector<khip_event> got_ev;
+        std::vector<khip_stats> got_st;
+        got_ev.swap(ctx->host_events);
+        got_st.swap(ctx->host_stats);
+        ctx->host_events.swap(saved_ev);
+        ctx->host_stats.swap(saved_st);
+        if (!ok || overflow) continue;
+        if (nevents) *nevents = static_cast<int64_t>(got_ev.size());
+        for (size_t k = 0; k < got_ev.size() && events && static_cast<int64_t>(k) < cap; ++k)
+            events[k] = got_ev[k];
+        if (stats && !got_st.empty()) *stats = got_st[0];
+        break;
+    }
+    ctx->dt = dt_saved;
+    return status(ctx, ok);
+}
+
+int khip_take_events(khip_ctx* ctx, khip_event* out, int64_t cap, int64_t* n) {
+    if (!ctx || !n) return KHIP_ERR_ARG;
+    const int64_t have = static_cast<int64_t>(ctx->host_events.size());
+    if (!out) {
+        *n = have;
+        return KHIP_OK;
+    }
+    const int64_t m = std::min(have, cap);
+    std::copy(ctx->host_events.begin(), ctx->host_events.begin() + m, out);
+    ctx->host_events.erase(ctx->host_events.begin(), ctx->host_events.begin() + m);
+    *n = m;
+    return KHIP_OK;
+}
+
+int khip_take_stats(khip_ctx* ctx, khip_stats* out, int64_t cap, int64_t* n) {
+    if (!ctx || !n) return KHIP_ERR_ARG;
+    const int64_t have = static_cast<int64_t>(ctx->host_stats.size());
+    if (!out) {
+        *n = have;
+        return KHIP_OK;
+    }
+    const int64_t m = std::min(have, cap);
+    std::copy(ctx->host_stats.begin(), ctx->host_stats.begin() + m, out);
+    ctx->host_stats.erase(ctx->host_stats.begin(), ctx->host_stats.begin() + m);
+    *n = m;
+    return KHIP_OK;
+}
+
+int khip_set_timing(khip_ctx* ctx, int enabled) {
+    if (!ctx) return KHIP_ERR_ARG;
+    ctx->timing = enabled != 0;
+    return KHIP_OK;
+}
+
+int khip_get_timing(const khip_ctx* ctx, khip_timing* out) {
+    if (!ctx || !out) return KHIP_ERR_ARG;
+    *out = ctx->tm;
+    return KHIP_OK;
+}
+
+int khip_reset_timing(khip_ctx* ctx) {
+    if (!ctx) return KHIP_ERR_ARG;
+    ctx->tm = khip_timing{};
+    return KHIP_OK;
+}
+
+int khip_set_chunk(khip_ctx* ctx, int steps) {
+    if (!ctx || steps <= 0) return KHIP_ERR_ARG;
+    ctx->chunk = steps;
+    return KHIP_OK;
+}
+
+int khip_fp64_selfcheck(khip_ctx* ctx, int64_t n, const double* a, const double* b, double* out) {
+    if (!ctx || n <= 0 || n > (1 << 20) || !a || !b || !out) return KHIP_ERR_ARG;
+    double *da = nullptr, *db = nullptr, *dout = nullptr;
+    bool ok = hipMalloc(reinterpret_cast<void**>(&da), sizeof(double) * n) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&db), sizeof(double) * n) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&dout), sizeof(double) * 3 * n) == hipSuccess &&
+              hipMemcpy(da, a, sizeof(double) * n, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(db, b, sizeof(double) * n, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        k_fp64_check<<<grid_of(n), BLOCK, 0, ctx->stream>>>(static_cast<int>(n), da, db, dout);
+        ok = hipStreamSynchronize(ctx->stream) == hipSuccess &&
+             hipMemcpy(out, dout, sizeof(double) * 3 * n, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dout) (void)hipFree(dout);
+    if (!ok) ctx->error = "fp64 self-check failed";
+    return ok ? KHIP_OK : KHIP_ERR_HIP;
+}
+
+}  // extern "C"

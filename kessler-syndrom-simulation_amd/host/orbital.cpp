@@ -1,0 +1,234 @@
+// Host side of the orbital core: the lunar ephemeris (evaluated on the host
+// once per half step and broadcast to the RK4 kernel), the host RK4 used for
+// fragments born inside a step, orbital elements and catalogue I/O.
+//
+// Operation order follows the reference (orbital.cpp:53-130) so that the
+// Moon table and the host RK4 are bit-identical to it; built with
+// -ffp-contract=off.
+
+#include "orbital.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+
+namespace kessler {
+
+namespace {
+
+std::string strip(const std::string& s) {
+    const size_t b = s.find_first_not_of(" \t\r\n");
+    const size_t e = s.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? std::string() : s.substr(b, e - b + 1);
+}
+
+// Civil date (UTC) to Julian day, Meeus ch. 7 (orbital.cpp:19-25).
+double julian_day(int y, int m, double day) {
+    if (m <= 2) {
+        y -= 1;
+        m += 12;
+    }
+    const int century = y / 100;
+    const int gregorian = 2 - century + century / 4;
+    return std::floor(365.25 * (y + 4716)) + std::floor(30.6001 * (m + 1)) + day + gregorian - 1524.5;
+}
+
+double wrap_deg(double d) { return std::fmod(std::fmod(d, 360.0) + 360.0, 360.0); }
+
+}  // namespace
+
+ObjectType parse_object_type(const std::string& s) {
+    if (s == "PAYLOAD") return ObjectType::Payload;
+    if (s == "DEBRIS") return ObjectType::Debris;
+    if (s == "ROCKET BODY") return ObjectType::RocketBody;
+    return ObjectType::Unknown;
+}
+
+const char* object_type_name(ObjectType t) {
+    switch (t) {
+        case ObjectType::Payload: return "PAYLOAD";
+        case ObjectType::Debris: return "DEBRIS";
+        case ObjectType::RocketBody: return "ROCKET BODY";
+        default: return "UNKNOWN";
+    }
+}
+
+Vec3 moon_position(double jd) {
+    const double T = (jd - 2451545.0) / 36525.0;
+    const double Lp = wrap_deg(218.3164477 + 481267.88123421 * T);  // mean longitude
+    double D = wrap_deg(297.8501921 + 445267.1114034 * T);          // mean elongation
+    double M = wrap_deg(357.5291092 + 35999.0502909 * T);           // Sun mean anomaly
+    double Mp = wrap_deg(134.9633964 + 477198.8675055 * T);         // Moon mean anomaly
+    double F = wrap_deg(93.2720950 + 483202.0175233 * T);           // argument of latitude
+    D *= DEG;
+    M *= DEG;
+    Mp *= DEG;
+    F *= DEG;
+    const double lon = Lp + 6.288774 * std::sin(Mp) + 1.274027 * std::sin(2 * D - Mp)
+                     + 0.658314 * std::sin(2 * D) + 0.213618 * std::sin(2 * Mp)
+                     - 0.185116 * std::sin(M) - 0.114332 * std::sin(2 * F)
+                     + 0.058793 * std::sin(2 * D - 2 * Mp) + 0.057066 * std::sin(2 * D - M - Mp)
+                     + 0.053322 * std::sin(2 * D + Mp) + 0.045758 * std::sin(2 * D - M);
+    const double lat = 5.128122 * std::sin(F) + 0.280602 * std::sin(Mp + F)
+                     + 0.277693 * std::sin(Mp - F) + 0.173237 * std::sin(2 * D - F)
+                     + 0.055413 * std::sin(2 * D - Mp + F) + 0.046271 * std::sin(2 * D - Mp - F);
+    const double dist = 385000.56 - 20905.355 * std::cos(Mp) - 3699.111 * std::cos(2 * D - Mp)
+                      - 2955.968 * std::cos(2 * D) - 569.925 * std::cos(2 * Mp)
+                      + 48.888 * std::cos(M) - 3.149 * std::cos(2 * F)
+                      + 246.158 * std::cos(2 * D - 2 * Mp) - 152.138 * std::cos(2 * D - M - Mp)
+                      - 170.733 * std::cos(2 * D + Mp) - 204.586 * std::cos(2 * D - M);
+    const double l = lon * DEG;
+    const double b = lat * DEG;
+    const double eps = (23.439291 - 0.0130042 * T) * DEG;  // obliquity
+    const double xe = dist * std::cos(b) * std::cos(l);
+    const double ye = dist * std::cos(b) * std::sin(l);
+    const double ze = dist * std::sin(b);
+    return {xe, ye * std::cos(eps) - ze * std::sin(eps), ye * std::sin(eps) + ze * std::cos(eps)};
+}
+
+Vec3 acceleration(const Vec3& r, const Vec3& rm) {
+    const double r2 = dot(r, r);
+    const double rn = std::sqrt(r2);
+    Vec3 a = r * (-MU_EARTH / (r2 * rn));
+    const double k = 1.5 * J2 * MU_EARTH * R_EARTH * R_EARTH / (r2 * r2 * rn);
+    const double z2 = 5.0 * r.z * r.z / r2;
+    a = a + Vec3{k * r.x * (z2 - 1.0), k * r.y * (z2 - 1.0), k * r.z * (z2 - 3.0)};
+    const Vec3 d = rm - r;
+    const double dn = norm(d);
+    const double mn = norm(rm);
+    return a + d * (MU_MOON / (dn * dn * dn)) - rm * (MU_MOON / (mn * mn * mn));
+}
+
+State rk4(const State& s, double dt, const Vec3& rm0, const Vec3& rm1, const Vec3& rm2) {
+    const double h = dt / 2;
+    const Vec3 k1r = s.v, k1v = acceleration(s.r, rm0);
+    const Vec3 k2r = s.v + k1v * h, k2v = acceleration(s.r + k1r * h, rm1);
+    const Vec3 k3r = s.v + k2v * h, k3v = acceleration(s.r + k2r * h, rm1);
+    const Vec3 k4r = s.v + k3v * dt, k4v = acceleration(s.r + k3r * dt, rm2);
+    const double w = dt / 6;
+    return {s.r + (k1r + k2r * 2 + k3r * 2 + k4r) * w, s.v + (k1v + k2v * 2 + k3v * 2 + k4v) * w};
+}
+
+Elements elements(const State& s) {
+    const double r = norm(s.r), v2 = dot(s.v, s.v);
+    const double energy = v2 / 2 - MU_EARTH / r;
+    const double a = -MU_EARTH / (2 * energy);
+    const Vec3 h = cross(s.r, s.v);
+    const Vec3 ev = cross(s.v, h) * (1.0 / MU_EARTH) - s.r * (1.0 / r);
+    const double e = norm(ev);
+    const double period = a > 0 ? 2 * PI * std::sqrt(a * a * a / MU_EARTH) / 60.0 : 0.0;
+    return {a, e, std::acos(h.z / norm(h)) / DEG, a * (1 - e) - R_EARTH, a * (1 + e) - R_EARTH, period};
+}
+
+std::vector<Object> load_catalog(const std::string& path, int* dropped) {
+    std::vector<Object> objs;
+    std::ifstream in(path);
+    if (!in) {
+        std::cerr << "cannot open " << path << "\n";
+        if (dropped) *dropped = 0;
+        return objs;
+    }
+    int bad = 0;
+    std::string line;
+    std::vector<std::string> field;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        field.clear();
+        std::stringstream ss(line);
+        std::string tok;
+        while (std::getline(ss, tok, '|')) field.push_back(strip(tok));
+        if (field.size() < 17) {
+            ++bad;
+            continue;
+        }
+        try {
+            Object o;
+            o.id = std::stoi(field[0]);
+            o.name = field[1];
+            o.type = parse_object_type(field[2]);
+            o.country = field[4];
+            o.rcs = field[5];
+            // Size from the radar cross-section class: collision radii
+            // 0.1 / 0.4 / 2 m for SMALL / MEDIUM (and unknown) / LARGE.
+            o.size_m = o.rcs == "SMALL" ? 0.2 : (o.rcs == "LARGE" ? 4.0 : 0.8);
+            o.s.r = {std::stod(field[11]), std::stod(field[12]), std::stod(field[13])};
+            o.s.v = {std::stod(field[14]), std::stod(field[15]), std::stod(field[16])};
+            objs.push_back(std::move(o));
+        } catch (...) {
+            ++bad;
+        }
+    }
+    if (dropped) *dropped = bad;
+    return objs;
+}
+
+double read_epoch_jd(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) return JD_DEFAULT_EPOCH;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] != '#') break;
+        const size_t pos = line.find("epoch commun) :");
+        if (pos == std::string::npos) continue;
+        int y, mo, d, h, mi, s;
+        if (std::sscanf(line.c_str() + pos, "epoch commun) : %d-%d-%dT%d:%d:%d", &y, &mo, &d, &h, &mi,
+                        &s) == 6)
+            return julian_day(y, mo, d + (h + mi / 60.0 + s / 3600.0) / 24.0);
+    }
+    return JD_DEFAULT_EPOCH;
+}
+
+static void write_snapshot_impl(const std::string& path, const std::vector<Object>& objs, double t,
+                                int precision) {
+    std::ofstream out(path);
+    out << "# t = " << t << " s apres l'epoch\nid,x_km,y_km,z_km,vx,vy,vz,alive\n";
+    out.precision(precision);
+    for (const Object& o : objs)
+        out << o.id << ',' << o.s.r.x << ',' << o.s.r.y << ',' << o.s.r.z << ',' << o.s.v.x << ','
+            << o.s.v.y << ',' << o.s.v.z << ',' << o.alive << '\n';
+}
+
+void write_snapshot(const std::string& path, const std::vector<Object>& objs, double t) {
+    write_snapshot_impl(path, objs, t, 9);
+}
+
+void write_snapshot_exact(const std::string& path, const std::vector<Object>& objs, double t) {
+    write_snapshot_impl(path, objs, t, 17);
+}
+
+std::string jd_to_utc_string(double jd) {
+    const double z = std::floor(jd + 0.5);
+    const double frac = jd + 0.5 - z;
+    double a = z;
+    if (z >= 2299161) {
+        const double alpha = std::floor((z - 1867216.25) / 36524.25);
+        a = z + 1 + alpha - std::floor(alpha / 4);
+    }
+    const double b = a + 1524;
+    const double c = std::floor((b - 122.1) / 365.25);
+    const double d = std::floor(365.25 * c);
+    const double e = std::floor((b - d) / 30.6001);
+    const int day = static_cast<int>(b - d - std::floor(30.6001 * e));
+    const int month = static_cast<int>(e < 14 ? e - 1 : e - 13);
+    const int year = static_cast<int>(month > 2 ? c - 4716 : c - 4715);
+    const double hours = frac * 24.0;
+    int hh = static_cast<int>(hours);
+    int mm = static_cast<int>((hours - hh) * 60.0);
+    int sec = static_cast<int>(((hours - hh) * 60.0 - mm) * 60.0 + 0.5);
+    if (sec >= 60) { sec -= 60; ++mm; }
+    if (mm >= 60) { mm -= 60; ++hh; }
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%04d-%02d-%02d %02d:%02d:%02d UTC", year, month, day, hh, mm, sec);
+    return buf;
+}
+
+State Simulation::propagate_state(const State& s, double t_from, double duration) const {
+    if (duration <= 0) return s;
+    const double jd0 = epoch_jd + t_from / 86400.0;
+    return rk4(s, duration, moon_position(jd0), moon_position(jd0 + duration / 2 / 86400.0),
+               moon_position(jd0 + duration / 86400.0));
+}
+
+}  // namespace kessler
