@@ -1,0 +1,196 @@
+"""Benchmark of the MI355X propagate-and-screen engine (one GPU).
+
+Workload: the cascade-scale plain-FP64 configuration, a 28,340-object
+synthetic catalogue (orbit mix of the reference's 2026-08-01 catalogue,
+seed 0) plus 40 seeded catastrophic NASA-breakup collisions at t = 0
+(target and projectile 100-2000 kg, 8-12 km/s, L_c = 0.045 m), 255,220
+objects in all; FP64 RK4 (Earth + J2 + Moon) at dt = 10 s with screening
+at 5 km (radius scale 1, minimum encounter speed 10 m/s). A step is one
+simulation step of the whole population: propagation plus screening.
+
+value = object-steps/s = (sum over the timed steps of the objects propagated)
+/ wall time of the timed steps, with the population already resident in HBM.
+
+The JSON line also carries the roofline of the dominant kernel (HIP-event
+timing on the engine's stream during the timed steps) and the CPU baseline:
+the reference's own OpenMP CPU path (oracle/_ref, compiled from the
+reference sources) timed on this host over a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from __graft_entry__ import load_package  # noqa: E402
+
+METRIC = "object-steps/s incl. screening, learned/mixed vs plain FP64 at <=10 m/24h"
+EPOCH = 2461253.916666667
+DT = 10.0
+THRESHOLD_KM = 5.0
+# Algorithmic work per unit: FP32 flops per candidate pair of
+# the pre-filter (prefilter_keep + building dr, dv), HBM bytes per propagated
+# object of the RK4 step (FP64 state in and out, alive flag).
+FLOP_PER_CANDIDATE = 60.0
+BYTES_PER_OBJECT_STEP = 97.0
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+
+def cascade_population(kamd):
+    return kamd.cascade(28340, 0, 40, 0.045)
+
+
+def run_gpu(kamd, pop, steps, warmup, timing=True):
+    eng = kamd.Engine(0)
+    alive = pop["alive"].astype(np.uint8)
+
+    def reset():
+        eng.set_objects(pop["state"], alive, pop["size_m"])
+        eng.set_config(THRESHOLD_KM, 1.0, 0.01)
+        eng.set_time(EPOCH, 0.0, DT)
+
+    if warmup > 0:
+        reset()
+        eng.run(warmup, screen=True)
+        eng.take_events()
+        eng.take_stats()
+    reset()
+    eng.set_timing(timing)
+    eng.reset_timing()
+    t0 = time.perf_counter()
+    eng.run(steps, screen=True)
+    wall = time.perf_counter() - t0
+    tm = eng.timing()
+    ev = eng.take_events()
+    st = eng.take_stats()
+    return eng, wall, tm, ev, st
+
+
+def cpu_baseline(kamd, eng, pop, windows_h=(0.0, 6.0, 12.0, 18.0), budget_s=24.0):
+    """The compiled reference CPU path over a bounded sample: a few steps from
+    each of several start times of the same run. The start states come from
+    the engine, which is bit-identical to the CPU path."""
+    from oracle.bind import oracle, reference
+
+    ref = reference()
+    kind = "reference"
+    if ref is None:
+        ref, kind = oracle(), "port"
+    cores = ref.threads(0)
+    alive0 = pop["alive"].astype(np.uint8)
+    size_m = pop["size_m"]
+    per_window = budget_s / len(windows_h)
+    obj_steps, cpu_s, steps_done, sample = 0, 0.0, 0, []
+    for h in windows_h:
+        nstart = int(round(h * 3600.0 / DT))
+        eng.set_objects(pop["state"], alive0, size_m)
+        eng.set_config(THRESHOLD_KM, 1.0, 0.01)
+        eng.set_time(EPOCH, 0.0, DT)
+        if nstart:
+            eng.run(nstart, screen=False)
+        states, alive = eng.get_objects()
+        t_start = nstart * DT
+        spent, k = 0.0, 0
+        while spent < per_window and k < 50:
+            n_alive = int(alive.sum())
+            states, alive, _, _, loop_s = ref.run(states, alive, size_m, EPOCH, DT, 1, screen=True,
+                                                  thr=THRESHOLD_KM, t_start=t_start + k * DT,
+                                                  per_step=False)
+            spent += loop_s
+            obj_steps += n_alive
+            k += 1
+        cpu_s += spent
+        steps_done += k
+        sample.append(f"{k} steps from t={h:g} h")
+    return {
+        "value": obj_steps / cpu_s,
+        "unit": "object-steps/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": (f"{steps_done} steps of the same workload ({'; '.join(sample)}), start states "
+                   f"from the engine (bit-identical to the CPU path), {cpu_s:.1f} s of CPU loop time"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8640)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only: --gpus must be 1 "
+                                   "(multi-GPU sharding is a design in DESIGN.md, not built)"}))
+        sys.exit(2)
+
+    kamd = load_package()
+    pop = cascade_population(kamd)
+    eng, wall, tm, ev, st = run_gpu(kamd, pop, args.steps, args.warmup)
+    obj_steps = int(st["propagated"].sum())
+    cand = int(st["candidate_pairs"].sum())
+    value = obj_steps / wall
+
+    kernels = {"pairs": float(tm["pairs_ms"]), "grid": float(tm["grid_ms"]),
+               "propagate": float(tm["propagate_ms"])}
+    dominant = max(kernels, key=kernels.get)
+    if dominant == "propagate":
+        achieved = BYTES_PER_OBJECT_STEP * obj_steps / (tm["propagate_ms"] * 1e-3) / 1e9
+        roof = {"kernel": "k_propagate", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                "traffic": None}
+    else:
+        ms = tm["pairs_ms"] if dominant == "pairs" else tm["grid_ms"]
+        achieved = FLOP_PER_CANDIDATE * cand / (ms * 1e-3) / 1e12
+        roof = {"kernel": "k_pairs" if dominant == "pairs" else "grid kernels",
+                "bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
+    roof["kernel_ms_per_step"] = {k: v / max(1, int(tm["steps"])) for k, v in kernels.items()}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "object-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"cascade: 28,340-object synthetic catalogue (seed 0) + 40 catastrophic "
+                         f"breakups at t=0 (L_c 0.045 m) = {len(pop):,} objects; plain FP64 RK4 "
+                         f"Earth+J2+Moon, dt 10 s, screening 5 km; steps 0..{args.steps}"),
+            "objects": int(len(pop)),
+            "dt_s": DT,
+            "threshold_km": THRESHOLD_KM,
+            "simulated_h": args.steps * DT / 3600.0,
+        },
+        "object_seconds_per_s": value * DT,
+        "events": int(len(ev)),
+        "collisions": int(ev["collision"].sum()),
+        "candidate_pairs": cand,
+        "roofline": roof,
+        "device": eng.device_name,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(kamd, eng, pop)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    eng.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

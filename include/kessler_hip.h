@@ -78,6 +78,7 @@ typedef struct {
     double cell_km;
     int32_t max_cell_objects;
     int32_t cell_doublings; /* > 0 only where the grid extent needed a coarser cell (see khip_run) */
+    int64_t propagated;     /* objects alive at the start of the step (0 for khip_screen) */
 } khip_stats;
 
 /* Device time per kernel family, summed over the steps since the last

@@ -24,7 +24,11 @@ EVENT_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("step", "<i8"), ("t", "<f8"
                         ("pad", "<i4")])
 STATS_DTYPE = np.dtype([("candidate_pairs", "<i8"), ("refined_pairs", "<i8"),
                         ("co_orbiting", "<i8"), ("cell_km", "<f8"),
-                        ("max_cell_objects", "<i4"), ("cell_doublings", "<i4")])
+                        ("max_cell_objects", "<i4"), ("cell_doublings", "<i4"),
+                        ("propagated", "<i8")])
+OBJECT_DTYPE = np.dtype([("id", "<i4"), ("event", "<i4"), ("alive", "<i4"), ("type", "<i4"),
+                         ("state", "<f8", (6,)), ("size_m", "<f8"), ("mass_kg", "<f8"),
+                         ("area_to_mass", "<f8")])
 TIMING_DTYPE = np.dtype([("propagate_ms", "<f8"), ("grid_ms", "<f8"), ("pairs_ms", "<f8"),
                          ("total_ms", "<f8"), ("steps", "<i8")])
 
@@ -35,6 +39,11 @@ ABI_SYMBOLS = (
     "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
     "khip_set_chunk", "khip_fp64_selfcheck",
+)
+# Every symbol include/kessler_host.h declares.
+HOST_SYMBOLS = (
+    "khost_generate", "khost_impacts", "khost_cascade", "khost_write_catalog",
+    "khost_load_catalog", "khost_moon_position",
 )
 
 _P = C.c_void_p
@@ -83,6 +92,12 @@ def library() -> C.CDLL:
         "khip_reset_timing": (_I, [_P]),
         "khip_set_chunk": (_I, [_P, _I]),
         "khip_fp64_selfcheck": (_I, [_P, _L, _P, _P, _P]),
+        "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
+        "khost_impacts": (_L, [_L, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P]),
+        "khost_cascade": (_L, [_L, C.c_uint64, _I, _D, _D, _D, _D, _D, _L, _P]),
+        "khost_write_catalog": (_I, [C.c_char_p, _L, _P, C.c_char_p]),
+        "khost_load_catalog": (_L, [C.c_char_p, _L, _P, C.POINTER(_D), C.POINTER(_I)]),
+        "khost_moon_position": (None, [_D, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -223,3 +238,73 @@ class Engine:
 
 def device_count() -> int:
     return int(library().khip_device_count())
+
+
+class Mix(C.Structure):
+    _fields_ = [("frac_leo", _D), ("frac_heo", _D), ("frac_meo", _D), ("frac_geo", _D),
+                ("leo_shell_boost", _D)]
+
+
+def generate(n: int, seed: int = 0, mix=None) -> np.ndarray:
+    """Seeded synthetic catalogue (khost_generate) as OBJECT_DTYPE records."""
+    out = np.zeros(n, dtype=OBJECT_DTYPE)
+    m = None if mix is None else C.byref(Mix(*mix))
+    rc = library().khost_generate(int(n), int(seed), m, _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"khost_generate failed ({rc})")
+    return out
+
+
+def impacts(objs, epoch_jd, t, targets, projectile_mass_kg, v_rel_km_s, seed, lc_m):
+    """Forced impacts in sequence (khost_impacts): returns (objects, events[fragments, escaped])."""
+    objs = np.ascontiguousarray(objs, dtype=OBJECT_DTYPE)
+    targets = np.ascontiguousarray(targets, dtype=np.int32)
+    pm = np.ascontiguousarray(projectile_mass_kg, dtype=np.float64)
+    vr = np.ascontiguousarray(v_rel_km_s, dtype=np.float64)
+    lib = library()
+    args = (len(objs), _ptr(objs), float(epoch_jd), float(t), len(targets), _ptr(targets), _ptr(pm),
+            _ptr(vr), int(seed), float(lc_m))
+    total = lib.khost_impacts(*args, 0, None, None)
+    if total < 0:
+        raise RuntimeError(f"khost_impacts failed ({total})")
+    out = np.zeros(total, dtype=OBJECT_DTYPE)
+    ev = np.zeros((len(targets), 2), dtype=np.int32)
+    lib.khost_impacts(*args, total, _ptr(out), _ptr(ev))
+    return out, ev
+
+
+def cascade(n_catalog: int, seed: int, n_events: int, lc_m: float, mass=(100.0, 2000.0),
+            vrel=(8.0, 12.0)) -> np.ndarray:
+    """Seeded cascade catalogue (khost_cascade): catalogue + forced impacts at t = 0."""
+    lib = library()
+    args = (int(n_catalog), int(seed), int(n_events), float(lc_m), float(mass[0]), float(mass[1]),
+            float(vrel[0]), float(vrel[1]))
+    total = lib.khost_cascade(*args, 0, None)
+    if total < 0:
+        raise RuntimeError(f"khost_cascade failed ({total})")
+    out = np.zeros(total, dtype=OBJECT_DTYPE)
+    lib.khost_cascade(*args, total, _ptr(out))
+    return out
+
+
+def write_catalog(path: str, objs, epoch_iso: str = "2026-08-01T10:00:00+00:00"):
+    objs = np.ascontiguousarray(objs, dtype=OBJECT_DTYPE)
+    rc = library().khost_write_catalog(path.encode(), len(objs), _ptr(objs), epoch_iso.encode())
+    if rc != 0:
+        raise RuntimeError(f"khost_write_catalog failed ({rc})")
+
+
+def load_catalog(path: str):
+    lib = library()
+    ep = _D()
+    dr = _I()
+    n = lib.khost_load_catalog(path.encode(), 0, None, C.byref(ep), C.byref(dr))
+    out = np.zeros(max(n, 0), dtype=OBJECT_DTYPE)
+    lib.khost_load_catalog(path.encode(), n, _ptr(out), C.byref(ep), C.byref(dr))
+    return out, ep.value, dr.value
+
+
+def moon_position(jd: float) -> np.ndarray:
+    out = np.zeros(3)
+    library().khost_moon_position(float(jd), _ptr(out))
+    return out

@@ -58,9 +58,9 @@ namespace {
 constexpr int ITEMS = 8;               // elements per thread per tile
 constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
 constexpr int RADIX = 256;
-constexpr int KEY_BITS_MAX = 31;       // packed key width; sentinel is 0xffffffff
-constexpr uint32_t DEAD_KEY = 0xffffffffu;
-constexpr int MAX_PASSES = 4;
+constexpr int KEY_BITS_MAX = 63;       // packed key width; the dead sentinel is all ones
+constexpr uint64_t DEAD_KEY = ~0ULL;
+constexpr int MAX_PASSES = 8;
 
 // Device-resident grid parameters and per-step accumulators.
 struct GridState {
@@ -69,7 +69,7 @@ struct GridState {
     unsigned long long lo[3];       // ordered bits of min position
     unsigned long long hi[3];       // ordered bits of max position
     unsigned int alive;             // objects alive at the end of the step
-    unsigned int pad0;
+    unsigned int propagated;        // objects alive at the start of the step
     // Derived by k_grid_setup.
     double cell;                    // cell edge (km)
     double cell_ref;                // the reference's cell size before any doubling
@@ -78,6 +78,8 @@ struct GridState {
     int npasses;                    // radix passes needed (0: nothing alive)
     int doublings;                  // times the cell was doubled to fit the key
     unsigned int ncells;
+    int passes_max;                 // max npasses over the chunk (reset per chunk)
+    int passes_short;               // a step needed more passes than were launched
     // Accumulated by k_pairs.
     unsigned long long cand, refined, co;
     unsigned int max_cell;
@@ -97,6 +99,7 @@ struct DevStats {
     long long cand, refined, co;
     double cell_km;
     int max_cell, doublings;
+    long long propagated;
 };
 static_assert(sizeof(DevStats) == sizeof(khip_stats), "stats layout");
 
@@ -134,7 +137,7 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
                                                       const MoonSample* __restrict__ moon3,
                                                       double dt, GridState* gs) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
-    bool on = false;
+    bool on = false, was = false;
     double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
     if (i < n) {
         D3 r{cur.x[i], cur.y[i], cur.z[i]};
@@ -142,6 +145,7 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
         rx = r.x; ry = r.y; rz = r.z;
         vn = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
         bool a = alive[i] != 0;
+        was = a;
         if (a) {
             const MoonSample m0 = moon3[0], m1 = moon3[1], m2 = moon3[2];
             rk4(r, v, dt, m0, m1, m2);
@@ -154,6 +158,8 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
         nxt.vx[i] = v.x; nxt.vy[i] = v.y; nxt.vz[i] = v.z;
         on = a;
     }
+    const unsigned long long wm = __ballot(was);
+    if (__lane_id() == 0 && wm) atomicAdd(&gs->propagated, static_cast<unsigned int>(__popcll(wm)));
     block_reduce_grid(gs, on, vn, rx, ry, rz);
 }
 
@@ -174,12 +180,15 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* ali
 
 __global__ void k_reset(GridState* gs) {
     if (threadIdx.x != 0) return;
+    gs->passes_max = 0;
+    gs->passes_short = 0;
     gs->vmax_bits = 0ULL;
     for (int a = 0; a < 3; ++a) {
         gs->lo[a] = ~0ULL;
         gs->hi[a] = 0ULL;
     }
     gs->alive = 0;
+    gs->propagated = 0;
     gs->cand = gs->refined = gs->co = 0ULL;
     gs->max_cell = 0;
     gs->ncells = 0;
@@ -198,7 +207,7 @@ __device__ __forceinline__ int bits_for(long long range) {
     return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
 }
 
-__global__ void k_grid_setup(GridState* gs, double dt, double thr) {
+__global__ void k_grid_setup(GridState* gs, double dt, double thr, int launched_passes) {
     if (threadIdx.x != 0) return;
     const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
     const double vrel_max = 2.0 * vmax;
@@ -236,19 +245,21 @@ __global__ void k_grid_setup(GridState* gs, double dt, double thr) {
         gs->doublings += 1;
     }
     gs->cell = cell;
+    if (gs->npasses > gs->passes_max) gs->passes_max = gs->npasses;
+    if (gs->npasses > launched_passes) gs->passes_short = 1;
 }
 
-__device__ __forceinline__ uint32_t pack_key(const GridState& g, double x, double y, double z) {
+__device__ __forceinline__ uint64_t pack_key(const GridState& g, double x, double y, double z) {
     const double c = g.cell;
     const long long ix = static_cast<long long>(floor(x / c)) - g.imin[0];
     const long long iy = static_cast<long long>(floor(y / c)) - g.imin[1];
     const long long iz = static_cast<long long>(floor(z / c)) - g.imin[2];
-    return (static_cast<uint32_t>(ix) << (g.bits[1] + g.bits[2])) |
-           (static_cast<uint32_t>(iy) << g.bits[2]) | static_cast<uint32_t>(iz);
+    return (static_cast<uint64_t>(ix) << (g.bits[1] + g.bits[2])) |
+           (static_cast<uint64_t>(iy) << g.bits[2]) | static_cast<uint64_t>(iz);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, const uint8_t* alive, int n,
-                                                 const GridState* gs, uint32_t* keys, int* idx,
+                                                 const GridState* gs, uint64_t* keys, int* idx,
                                                  int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
@@ -259,18 +270,18 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, const uint8_t* alive
     for (int r = 0; r < ITEMS; ++r) {
         const int e = base + r * BLOCK + threadIdx.x;
         if (e < n) {
-            uint32_t k = DEAD_KEY;
+            uint64_t k = DEAD_KEY;
             if (alive[e]) k = pack_key(g, cur.x[e], cur.y[e], cur.z[e]);
             keys[e] = k;
             idx[e] = e;
-            atomicAdd(&s_hist[k & 255u], 1);
+            atomicAdd(&s_hist[static_cast<int>(k & 255u)], 1);
         }
     }
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_hist(const uint32_t* keys, int n, int pass,
+__global__ void __launch_bounds__(BLOCK) k_hist(const uint64_t* keys, int n, int pass,
                                                  const GridState* gs, int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
     if (pass >= gs->npasses) return;
@@ -280,7 +291,7 @@ __global__ void __launch_bounds__(BLOCK) k_hist(const uint32_t* keys, int n, int
     const int base = blockIdx.x * TILE;
     for (int r = 0; r < ITEMS; ++r) {
         const int e = base + r * BLOCK + threadIdx.x;
-        if (e < n) atomicAdd(&s_hist[(keys[e] >> shift) & 255u], 1);
+        if (e < n) atomicAdd(&s_hist[static_cast<int>((keys[e] >> shift) & 255u)], 1);
     }
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
@@ -329,8 +340,8 @@ __global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int
 // index order, 256 per round; a wave finds the lanes holding the same digit
 // with 8 ballots, and per-wave digit counts in LDS give the cross-wave
 // offsets.
-__global__ void __launch_bounds__(BLOCK) k_scatter(const uint32_t* keys_in, const int* idx_in,
-                                                    uint32_t* keys_out, int* idx_out, int n,
+__global__ void __launch_bounds__(BLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
+                                                    uint64_t* keys_out, int* idx_out, int n,
                                                     int pass, const GridState* gs,
                                                     const int* hist, const int* totals,
                                                     int ntiles) {
@@ -353,7 +364,7 @@ __global__ void __launch_bounds__(BLOCK) k_scatter(const uint32_t* keys_in, cons
     for (int r = 0; r < ITEMS; ++r) {
         const int e = base + r * BLOCK + t;
         const bool valid = e < n;
-        const uint32_t key = valid ? keys_in[e] : 0u;
+        const uint64_t key = valid ? keys_in[e] : 0ULL;
         const int id = valid ? idx_in[e] : 0;
         const int d = static_cast<int>((key >> shift) & 255u);
         unsigned long long peers = __ballot(valid);
@@ -383,8 +394,8 @@ __global__ void __launch_bounds__(BLOCK) k_scatter(const uint32_t* keys_in, cons
     }
 }
 
-__device__ __forceinline__ const uint32_t* sorted_keys(const GridState& g, const uint32_t* k0,
-                                                       const uint32_t* k1) {
+__device__ __forceinline__ const uint64_t* sorted_keys(const GridState& g, const uint64_t* k0,
+                                                       const uint64_t* k1) {
     return (g.npasses & 1) ? k1 : k0;
 }
 __device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* i0, const int* i1) {
@@ -392,12 +403,12 @@ __device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* 
 }
 
 // Cell heads per tile over the first `alive` sorted elements.
-__global__ void __launch_bounds__(BLOCK) k_cell_count(const uint32_t* k0, const uint32_t* k1,
+__global__ void __launch_bounds__(BLOCK) k_cell_count(const uint64_t* k0, const uint64_t* k1,
                                                        const GridState* gs, int* bcount) {
     __shared__ int s_wave[WAVES_PER_BLOCK];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
-    const uint32_t* keys = sorted_keys(g, k0, k1);
+    const uint64_t* keys = sorted_keys(g, k0, k1);
     const int base = blockIdx.x * TILE;
     int cnt = 0;
     if (base < A && g.npasses > 0) {
@@ -429,10 +440,10 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
 //   pos: position relative to its cell's low corner (FP64 subtraction,
 //        rounded to float: the error DR_ERR_KM bounds), w = object index bits.
 //   vel: velocity rounded to float (DV_ERR_KM_S), w unused.
-__global__ void __launch_bounds__(BLOCK) k_cell_write(const uint32_t* k0, const uint32_t* k1,
+__global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
-                                                       Planes cur, uint32_t* cell_key,
+                                                       Planes cur, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lpos, float4* lvel) {
     __shared__ int s_wcnt[WAVES_PER_BLOCK];
@@ -441,19 +452,19 @@ __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint32_t* k0, const 
     const int A = static_cast<int>(g.alive);
     const int base = blockIdx.x * TILE;
     if (base >= A || g.npasses == 0) return;
-    const uint32_t* keys = sorted_keys(g, k0, k1);
+    const uint64_t* keys = sorted_keys(g, k0, k1);
     const int* sidx = sorted_idx(g, i0, i1);
     const int wave = threadIdx.x / WAVE;
     const unsigned long long below = lanes_below();
     const double cell = g.cell;
     const int sy = g.bits[2], sx = g.bits[1] + g.bits[2];
-    const uint32_t mz = (1u << g.bits[2]) - 1u, my = (1u << g.bits[1]) - 1u;
+    const uint64_t mz = (1ULL << g.bits[2]) - 1ULL, my = (1ULL << g.bits[1]) - 1ULL;
     if (threadIdx.x == 0) s_run = boff[blockIdx.x];
     __syncthreads();
     for (int r = 0; r < ITEMS; ++r) {
         const int e = base + r * BLOCK + threadIdx.x;
         const bool valid = e < A;
-        uint32_t key = 0;
+        uint64_t key = 0;
         bool head = false;
         if (valid) {
             key = keys[e];
@@ -510,7 +521,11 @@ __device__ __forceinline__ void forward_offset(int o, int& dx, int& dy, int& dz)
     }
 }
 
-__device__ __forceinline__ int find_cell(const uint32_t* ck, int ncells, uint32_t key) {
+__device__ __forceinline__ bool axis_fits(long long v, int bits) {
+    return v >= 0 && (bits >= 62 || v < (1LL << bits));
+}
+
+__device__ __forceinline__ int find_cell(const uint64_t* ck, int ncells, uint64_t key) {
     int lo = 0, hi = ncells;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -520,7 +535,7 @@ __device__ __forceinline__ int find_cell(const uint32_t* ck, int ncells, uint32_
     return (lo < ncells && ck[lo] == key) ? lo : -1;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint32_t* cell_key,
+__global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint64_t* cell_key,
                                                        const int* cell_begin, int* cell_count,
                                                        int* nbr) {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
@@ -532,19 +547,18 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint3
         cnt = static_cast<unsigned int>((c + 1 < nc ? cell_begin[c + 1] : A) - b);
         cell_count[c] = static_cast<int>(cnt);
         const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
-        const uint32_t key = cell_key[c];
-        const long long x = key >> (by + bz);
-        const long long y = (key >> bz) & ((1u << by) - 1u);
-        const long long z = key & ((1u << bz) - 1u);
+        const uint64_t key = cell_key[c];
+        const long long x = static_cast<long long>(key >> (by + bz));
+        const long long y = static_cast<long long>((key >> bz) & ((1ULL << by) - 1ULL));
+        const long long z = static_cast<long long>(key & ((1ULL << bz) - 1ULL));
         for (int o = 0; o < 13; ++o) {
             int ox, oy, oz;
             forward_offset(o, ox, oy, oz);
             const long long nx = x + ox, ny = y + oy, nz = z + oz;
             int found = -1;
-            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < (1LL << bx) && ny < (1LL << by) &&
-                nz < (1LL << bz)) {
-                const uint32_t nk = (static_cast<uint32_t>(nx) << (by + bz)) |
-                                    (static_cast<uint32_t>(ny) << bz) | static_cast<uint32_t>(nz);
+            if (axis_fits(nx, bx) && axis_fits(ny, by) && axis_fits(nz, bz)) {
+                const uint64_t nk = (static_cast<uint64_t>(nx) << (by + bz)) |
+                                    (static_cast<uint64_t>(ny) << bz) | static_cast<uint64_t>(nz);
                 found = find_cell(cell_key, nc, nk);
             }
             nbr[c * 13 + o] = found;
@@ -695,8 +709,10 @@ __global__ void k_finish(GridState* gs, DevStats* out, int screened) {
         s.cell_km = gs->cell_ref;
         s.max_cell = static_cast<int>(gs->max_cell);
         s.doublings = gs->doublings;
+        s.propagated = static_cast<long long>(gs->propagated);
         *out = s;
     }
+    gs->propagated = 0;
     gs->vmax_bits = 0ULL;
     for (int a = 0; a < 3; ++a) {
         gs->lo[a] = ~0ULL;
@@ -738,12 +754,12 @@ struct khip_ctx {
     std::vector<double> host_size;
 
     // Grid buffers.
-    uint32_t* keys[2] = {nullptr, nullptr};
+    uint64_t* keys[2] = {nullptr, nullptr};
     int* idx[2] = {nullptr, nullptr};
     int* hist = nullptr;
     int* totals = nullptr;
     int* bcount = nullptr;
-    uint32_t* cell_key = nullptr;
+    uint64_t* cell_key = nullptr;
     int* cell_begin = nullptr;
     int* cell_count = nullptr;
     int* cell_of = nullptr;
@@ -767,6 +783,7 @@ struct khip_ctx {
     double epoch_jd = JD_DEFAULT_EPOCH, t = 0.0, dt = 10.0;
     int64_t step = 0;
     int chunk = 256;
+    int passes = 4;   // radix passes launched per step (adapted per chunk)
 
     std::vector<khip_event> host_events;
     std::vector<khip_stats> host_stats;
@@ -909,10 +926,10 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const Planes pb = ctx->planes(before), pa = ctx->planes(after);
-    k_grid_setup<<<1, 64, 0, s>>>(ctx->gs, ctx->dt, ctx->cfg.threshold_km);
+    k_grid_setup<<<1, 64, 0, s>>>(ctx->gs, ctx->dt, ctx->cfg.threshold_km, ctx->passes);
     k_keys<<<tiles, BLOCK, 0, s>>>(pb, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
                                    ctx->hist, tiles);
-    for (int pass = 0; pass < MAX_PASSES; ++pass) {
+    for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         if (pass > 0)
             k_hist<<<tiles, BLOCK, 0, s>>>(ctx->keys[src], n, pass, ctx->gs, ctx->hist, tiles);
@@ -978,15 +995,26 @@ void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
     }
 }
 
-bool drain_chunk(khip_ctx* ctx, int64_t steps, int screen, bool& overflow) {
+// Waits for the chunk and moves its events and statistics to the host.
+// `rerun` is set when the chunk must run again: its events overflowed the
+// device buffer (grown here), or a step needed more radix passes than were
+// launched (raised here). The pass count also shrinks to what the chunk
+// needed, so the next chunk launches no idle passes.
+bool drain_chunk(khip_ctx* ctx, int64_t steps, int screen, bool& rerun) {
     GridState g;
     KHIP_TRY(ctx, hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    overflow = static_cast<int64_t>(g.events) > ctx->ev_cap;
-    if (overflow) {
-        if (!ensure_events(ctx, 2 * static_cast<int64_t>(g.events) + 1024)) return false;
-        return true;
+    rerun = false;
+    if (screen && g.passes_short) {
+        ctx->passes = std::min(MAX_PASSES, std::max(g.passes_max, ctx->passes + 1));
+        rerun = true;
     }
+    if (static_cast<int64_t>(g.events) > ctx->ev_cap) {
+        if (!ensure_events(ctx, 2 * static_cast<int64_t>(g.events) + 1024)) return false;
+        rerun = true;
+    }
+    if (rerun) return true;
+    if (screen && g.passes_max > 0 && g.passes_max < ctx->passes) ctx->passes = g.passes_max;
     if (!screen) return true;
     const int64_t ne = static_cast<int64_t>(g.events);
     std::vector<khip_event> ev(static_cast<size_t>(ne));
@@ -1010,7 +1038,7 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
     hipStream_t s = ctx->stream;
     if (n == 0) {
         for (int64_t k = 0; k < nsteps; ++k) {
-            if (screen) ctx->host_stats.push_back(khip_stats{0, 0, 0, 1.0, 0, 0});
+            if (screen) ctx->host_stats.push_back(khip_stats{0, 0, 0, 1.0, 0, 0, 0});
             ctx->t += ctx->dt;
             ctx->step += 1;
         }

@@ -155,6 +155,10 @@ std::vector<Object> load_catalog(const std::string& path, int* dropped) {
             o.size_m = o.rcs == "SMALL" ? 0.2 : (o.rcs == "LARGE" ? 4.0 : 0.8);
             o.s.r = {std::stod(field[11]), std::stod(field[12]), std::stod(field[13])};
             o.s.v = {std::stod(field[14]), std::stod(field[15]), std::stod(field[16])};
+            // Extension: an 18th field "size_m=<L>" (written for fragments,
+            // whose size is not an RCS class) overrides the class size.
+            if (field.size() > 17 && field[17].rfind("size_m=", 0) == 0)
+                o.size_m = std::stod(field[17].substr(7));
             objs.push_back(std::move(o));
         } catch (...) {
             ++bad;

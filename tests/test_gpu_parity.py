@@ -138,7 +138,7 @@ def test_reentries_dead_objects_and_fresh_fragments(engine, orc):
     assert np.array_equal(s1, s2) and np.array_equal(a1, a2)
     _same_events(e1, e2)
     _same_stats(t1, t2)
-    assert t1["max_cell_objects"].max() >= 800
+    assert t1["max_cell_objects"].max() >= 750  # the cloud, less the objects marked dead
 
 
 def test_empty_and_single_object(engine, orc):
