@@ -130,6 +130,41 @@ __device__ __forceinline__ double golden_min(const Hermite& h) {
     return 0.5 * (a + b);
 }
 
+// prefilter_keep (collision_filter.hpp) with hardware reciprocal and square
+// root (v_rcp_f32 / v_sqrt_f32, about 1 ulp) and a squared distance test.
+// Still conservative with respect to the exact FP64 filter_pair:
+//  * an error e in ts moves the computed distance by at most |v| |e|, i.e.
+//    ~2^-22 |v| dt for a 1-ulp reciprocal, against the 1e-5 (|r| + |v| dt)
+//    relative margin;
+//  * squaring both sides of d > gate + margin is monotone on non-negative
+//    values and adds ~1 ulp;
+//  * |v| and |r| only scale the gate and the margins.
+// Returns false only where prefilter_keep's own bound says filter_pair
+// rejects.
+__device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz, float vx, float vy,
+                                                    float vz, float dt, float threshold_km,
+                                                    float vmin2) {
+    // FP32 only: fused multiply-adds round once instead of twice, which can
+    // only shrink the error the margins cover.
+#pragma clang fp contract(fast)
+    const float dv2 = vx * vx + vy * vy + vz * vz;
+    const float rv = rx * vx + ry * vy + rz * vz;
+    const float r2 = rx * rx + ry * ry + rz * rz;
+    float ts = dv2 > 0.0f ? -rv * __builtin_amdgcn_rcpf(dv2) : 0.0f;
+    ts = ts < 0.0f ? 0.0f : ts;
+    ts = ts > dt ? dt : ts;
+    const float cx = rx + vx * ts, cy = ry + vy * ts, cz = rz + vz * ts;
+    const float d2 = cx * cx + cy * cy + cz * cz;
+    const float dvn = __builtin_amdgcn_sqrtf(dv2);
+    const float rn = __builtin_amdgcn_sqrtf(r2);
+    const float gate = threshold_km + static_cast<float>(GRAVITY_GRADIENT) * dvn * dt * dt * dt + 1e-3f;
+    const float lim = gate + 1e-5f * (rn + dvn * dt) + DR_ERR_KM + DV_ERR_KM_S * dt;
+    if (d2 > lim * lim) return false;
+    const float rv_tol = 1e-5f * rn * dvn + DR_ERR_KM * dvn + DV_ERR_KM_S * rn;
+    return !(dv2 > 1.1f * vmin2 && rv > rv_tol &&
+             dv2 > 1.01f * static_cast<float>(GRAVITY_GRADIENT) * r2);
+}
+
 // Monotone map of doubles onto uint64 (for atomic min / max).
 __device__ __forceinline__ unsigned long long ordered_bits(double x) {
     const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(x));

@@ -61,6 +61,14 @@ constexpr int RADIX = 256;
 constexpr int KEY_BITS_MAX = 63;       // packed key width; the dead sentinel is all ones
 constexpr uint64_t DEAD_KEY = ~0ULL;
 constexpr int MAX_PASSES = 8;
+// Cells with at least DENSE_MIN objects (fresh fragment clouds, crowded
+// shells) are screened by work items: CHUNK own objects (one per lane)
+// against up to SEG partners read by the whole wave at once.
+constexpr int DENSE_MIN = 32;
+constexpr int CHUNK = 64;
+constexpr int SEG = 256;
+constexpr int KIND_INTRA = 13;
+constexpr int DENSE_BLOCKS = 1024;
 
 // Device-resident grid parameters and per-step accumulators.
 struct GridState {
@@ -80,6 +88,10 @@ struct GridState {
     unsigned int ncells;
     int passes_max;                 // max npasses over the chunk (reset per chunk)
     int passes_short;               // a step needed more passes than were launched
+    unsigned int n_items;           // dense work items of this step
+    unsigned int n_dense;           // dense cells of this step
+    unsigned int items_max;         // max n_items over the chunk
+    int items_short;                // a step had more items than the buffer holds
     // Accumulated by k_pairs.
     unsigned long long cand, refined, co;
     unsigned int max_cell;
@@ -103,30 +115,95 @@ struct DevStats {
 };
 static_assert(sizeof(DevStats) == sizeof(khip_stats), "stats layout");
 
+// Per-block partial reductions, combined by the next single-block kernel
+// (no same-address atomics from thousands of waves).
+struct StepPartial {
+    unsigned long long vmax_bits;  // max |v| (raw bits of a non-negative double)
+    unsigned long long lo[3];      // ordered bits of min start-of-step position
+    unsigned long long hi[3];      // ordered bits of max start-of-step position
+    unsigned int alive;            // alive at the end of the step
+    unsigned int propagated;       // alive at the start of the step
+};
+
+struct PairPartial {
+    unsigned long long cand, refined, co;
+    unsigned int max_cell, pad;
+};
+
+// A dense work item: lanes take own objects [own_begin, own_begin +
+// own_count) of one cell; the wave walks partners [part_begin, part_begin +
+// part_count) of the same cell (kind KIND_INTRA: only partners after the
+// lane's own object) or of forward neighbour `kind` (0..12).
+struct PairItem {
+    int own_begin, own_count, part_begin, part_count;
+    int kind, pad0, pad1, pad2;
+};
+
+struct DenseCell {
+    int cell, item_base;
+};
+
 // SoA view of one state buffer.
 struct Planes {
     double *x, *y, *z, *vx, *vy, *vz;
 };
 
-__device__ __forceinline__ void block_reduce_grid(GridState* gs, bool on, double vn, double rx,
-                                                  double ry, double rz) {
-    // Per-wave reduction, then one atomic per wave: 8 quantities.
-    unsigned long long vb = on ? static_cast<unsigned long long>(__double_as_longlong(vn)) : 0ULL;
-    unsigned long long l0 = on ? ordered_bits(rx) : ~0ULL, l1 = on ? ordered_bits(ry) : ~0ULL,
-                       l2 = on ? ordered_bits(rz) : ~0ULL;
-    unsigned long long h0 = on ? ordered_bits(rx) : 0ULL, h1 = on ? ordered_bits(ry) : 0ULL,
-                       h2 = on ? ordered_bits(rz) : 0ULL;
-    const unsigned long long ball = __ballot(on);
-    if (ball == 0ULL) return;
-    vb = wave_max_u64(vb);
-    l0 = wave_min_u64(l0); l1 = wave_min_u64(l1); l2 = wave_min_u64(l2);
-    h0 = wave_max_u64(h0); h1 = wave_max_u64(h1); h2 = wave_max_u64(h2);
-    if (__lane_id() == 0) {
-        atomicMax(&gs->vmax_bits, vb);
-        atomicMin(&gs->lo[0], l0); atomicMin(&gs->lo[1], l1); atomicMin(&gs->lo[2], l2);
-        atomicMax(&gs->hi[0], h0); atomicMax(&gs->hi[1], h1); atomicMax(&gs->hi[2], h2);
-        atomicAdd(&gs->alive, static_cast<unsigned int>(__popcll(ball)));
+__device__ __forceinline__ StepPartial empty_step_partial() {
+    StepPartial p;
+    p.vmax_bits = 0ULL;
+    for (int a = 0; a < 3; ++a) {
+        p.lo[a] = ~0ULL;
+        p.hi[a] = 0ULL;
     }
+    p.alive = 0;
+    p.propagated = 0;
+    return p;
+}
+
+__device__ __forceinline__ void combine(StepPartial& a, const StepPartial& b) {
+    a.vmax_bits = b.vmax_bits > a.vmax_bits ? b.vmax_bits : a.vmax_bits;
+    for (int k = 0; k < 3; ++k) {
+        a.lo[k] = b.lo[k] < a.lo[k] ? b.lo[k] : a.lo[k];
+        a.hi[k] = b.hi[k] > a.hi[k] ? b.hi[k] : a.hi[k];
+    }
+    a.alive += b.alive;
+    a.propagated += b.propagated;
+}
+
+__device__ __forceinline__ StepPartial wave_reduce(StepPartial p) {
+    p.vmax_bits = wave_max_u64(p.vmax_bits);
+    for (int k = 0; k < 3; ++k) {
+        p.lo[k] = wave_min_u64(p.lo[k]);
+        p.hi[k] = wave_max_u64(p.hi[k]);
+    }
+    p.alive = wave_sum<unsigned int>(p.alive);
+    p.propagated = wave_sum<unsigned int>(p.propagated);
+    return p;
+}
+
+// Block-wide combine; thread 0 returns the block's total.
+__device__ __forceinline__ StepPartial block_reduce(StepPartial p, StepPartial* s_rows) {
+    p = wave_reduce(p);
+    const int wave = threadIdx.x / WAVE;
+    if (__lane_id() == 0) s_rows[wave] = p;
+    __syncthreads();
+    StepPartial t = s_rows[0];
+    for (int w = 1; w < WAVES_PER_BLOCK; ++w) combine(t, s_rows[w]);
+    return t;
+}
+
+__device__ __forceinline__ StepPartial object_partial(bool on, bool was, double vn, double rx,
+                                                      double ry, double rz) {
+    StepPartial p = empty_step_partial();
+    if (on) {
+        p.vmax_bits = static_cast<unsigned long long>(__double_as_longlong(vn));
+        p.lo[0] = p.hi[0] = ordered_bits(rx);
+        p.lo[1] = p.hi[1] = ordered_bits(ry);
+        p.lo[2] = p.hi[2] = ordered_bits(rz);
+        p.alive = 1;
+    }
+    p.propagated = was ? 1 : 0;
+    return p;
 }
 
 // ---------------------------------------------------------------------------
@@ -135,7 +212,8 @@ __device__ __forceinline__ void block_reduce_grid(GridState* gs, bool on, double
 
 __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uint8_t* alive, int n,
                                                       const MoonSample* __restrict__ moon3,
-                                                      double dt, GridState* gs) {
+                                                      double dt, StepPartial* partials) {
+    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     bool on = false, was = false;
     double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
@@ -158,14 +236,14 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
         nxt.vx[i] = v.x; nxt.vy[i] = v.y; nxt.vz[i] = v.z;
         on = a;
     }
-    const unsigned long long wm = __ballot(was);
-    if (__lane_id() == 0 && wm) atomicAdd(&gs->propagated, static_cast<unsigned int>(__popcll(wm)));
-    block_reduce_grid(gs, on, vn, rx, ry, rz);
+    const StepPartial t = block_reduce(object_partial(on, was, vn, rx, ry, rz), s_rows);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 // Reductions only (screening of host-given states).
 __global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* alive, int n,
-                                                   GridState* gs) {
+                                                   StepPartial* partials) {
+    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     bool on = false;
     double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
@@ -175,13 +253,18 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* ali
         const double vx = cur.vx[i], vy = cur.vy[i], vz = cur.vz[i];
         vn = sqrt(vx * vx + vy * vy + vz * vz);
     }
-    block_reduce_grid(gs, on, vn, rx, ry, rz);
+    const StepPartial t = block_reduce(object_partial(on, false, vn, rx, ry, rz), s_rows);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
 __global__ void k_reset(GridState* gs) {
     if (threadIdx.x != 0) return;
     gs->passes_max = 0;
     gs->passes_short = 0;
+    gs->items_max = 0;
+    gs->items_short = 0;
+    gs->n_items = 0;
+    gs->n_dense = 0;
     gs->vmax_bits = 0ULL;
     for (int a = 0; a < 3; ++a) {
         gs->lo[a] = ~0ULL;
@@ -207,8 +290,21 @@ __device__ __forceinline__ int bits_for(long long range) {
     return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
 }
 
-__global__ void k_grid_setup(GridState* gs, double dt, double thr, int launched_passes) {
+__global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
+                                                       int nparts, double dt, double thr,
+                                                       int launched_passes) {
+    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
+    StepPartial acc = empty_step_partial();
+    for (int b = threadIdx.x; b < nparts; b += BLOCK) combine(acc, partials[b]);
+    const StepPartial tot = block_reduce(acc, s_rows);
     if (threadIdx.x != 0) return;
+    gs->vmax_bits = tot.vmax_bits;
+    for (int a = 0; a < 3; ++a) {
+        gs->lo[a] = tot.lo[a];
+        gs->hi[a] = tot.hi[a];
+    }
+    gs->alive = tot.alive;
+    gs->propagated = tot.propagated;
     const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
     const double vrel_max = 2.0 * vmax;
     const double raw = vrel_max * dt + thr + GRAVITY_GRADIENT * vrel_max * dt * dt * dt;
@@ -535,14 +631,30 @@ __device__ __forceinline__ int find_cell(const uint64_t* ck, int ncells, uint64_
     return (lo < ncells && ck[lo] == key) ? lo : -1;
 }
 
+__device__ __forceinline__ int segments(int n) { return (n + SEG - 1) / SEG; }
+
+// Work items of dense cell c with k objects whose forward neighbours hold
+// `fwd_segs` partner segments in all: per chunk of CHUNK own objects, the
+// intra-cell tail in SEG-long segments plus every forward segment.
+__device__ __forceinline__ int dense_items(int k, int fwd_segs) {
+    int items = 0;
+    for (int i = 0; i * CHUNK < k; ++i) items += segments(k - i * CHUNK) + fwd_segs;
+    return items;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint64_t* cell_key,
                                                        const int* cell_begin, int* cell_count,
-                                                       int* nbr) {
+                                                       int* nbr, PairPartial* parts,
+                                                       DenseCell* dense, unsigned int item_cap) {
+    __shared__ unsigned int s_max[WAVES_PER_BLOCK];
+    __shared__ int s_wave[WAVES_PER_BLOCK];
+    __shared__ int s_base[2];
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     const int nc = static_cast<int>(gs->ncells);
+    const int A = static_cast<int>(gs->alive);
     unsigned int cnt = 0;
+    int items = 0;
     if (c < nc) {
-        const int A = static_cast<int>(gs->alive);
         const int b = cell_begin[c];
         cnt = static_cast<unsigned int>((c + 1 < nc ? cell_begin[c + 1] : A) - b);
         cell_count[c] = static_cast<int>(cnt);
@@ -551,6 +663,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         const long long x = static_cast<long long>(key >> (by + bz));
         const long long y = static_cast<long long>((key >> bz) & ((1ULL << by) - 1ULL));
         const long long z = static_cast<long long>(key & ((1ULL << bz) - 1ULL));
+        int fwd_segs = 0;
         for (int o = 0; o < 13; ++o) {
             int ox, oy, oz;
             forward_offset(o, ox, oy, oz);
@@ -562,10 +675,79 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
                 found = find_cell(cell_key, nc, nk);
             }
             nbr[c * 13 + o] = found;
+            if (found >= 0) {
+                const int kn = (found + 1 < nc ? cell_begin[found + 1] : A) - cell_begin[found];
+                fwd_segs += segments(kn);
+            }
         }
+        if (static_cast<int>(cnt) >= DENSE_MIN) items = dense_items(static_cast<int>(cnt), fwd_segs);
+    }
+    // Block-aggregated registration of dense cells: two atomics per block.
+    int tot_items, tot_dense;
+    const int ex_items = block_exclusive_scan(items, tot_items, s_wave);
+    const int ex_dense = block_exclusive_scan(items > 0 ? 1 : 0, tot_dense, s_wave);
+    if (threadIdx.x == 0) {
+        s_base[0] = tot_items ? static_cast<int>(atomicAdd(&gs->n_items, static_cast<unsigned int>(tot_items))) : 0;
+        s_base[1] = tot_dense ? static_cast<int>(atomicAdd(&gs->n_dense, static_cast<unsigned int>(tot_dense))) : 0;
+    }
+    __syncthreads();
+    if (items > 0) {
+        const int base = s_base[0] + ex_items;
+        dense[s_base[1] + ex_dense] = DenseCell{c, base};
+        if (static_cast<unsigned int>(base + items) > item_cap) gs->items_short = 1;
     }
     const unsigned long long m = wave_max_u64(cnt);
-    if (__lane_id() == 0 && m > 0) atomicMax(&gs->max_cell, static_cast<unsigned int>(m));
+    if (__lane_id() == 0) s_max[threadIdx.x / WAVE] = static_cast<unsigned int>(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int mm = 0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) mm = s_max[w] > mm ? s_max[w] : mm;
+        parts[blockIdx.x].max_cell = mm;
+    }
+}
+
+// Writes the work items of every dense cell: one block per dense cell
+// (grid-stride), one thread per chunk.
+__global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell* dense,
+                                                  const int* cell_begin, const int* cell_count,
+                                                  const int* nbr, PairItem* items,
+                                                  unsigned int item_cap) {
+    const int nd = static_cast<int>(gs->n_dense);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && gs->n_items > gs->items_max) gs->items_max = gs->n_items;
+    for (int d = blockIdx.x; d < nd; d += gridDim.x) {
+        const DenseCell dc = dense[d];
+        const int c = dc.cell;
+        const int b = cell_begin[c];
+        const int k = cell_count[c];
+        int fwd_segs = 0;
+        for (int o = 0; o < 13; ++o) {
+            const int n2 = nbr[c * 13 + o];
+            if (n2 >= 0) fwd_segs += segments(cell_count[n2]);
+        }
+        const int nch = (k + CHUNK - 1) / CHUNK;
+        for (int i = threadIdx.x; i < nch; i += BLOCK) {
+            int off = dc.item_base + i * fwd_segs;
+            for (int j = 0; j < i; ++j) off += segments(k - j * CHUNK);
+            const int own_begin = b + i * CHUNK;
+            const int own_count = min(CHUNK, k - i * CHUNK);
+            const int tail = k - i * CHUNK;
+            for (int sgi = 0; sgi * SEG < tail; ++sgi, ++off) {
+                if (static_cast<unsigned int>(off) >= item_cap) continue;
+                items[off] = PairItem{own_begin, own_count, own_begin + sgi * SEG,
+                                      min(SEG, tail - sgi * SEG), KIND_INTRA, 0, 0, 0};
+            }
+            for (int o = 0; o < 13; ++o) {
+                const int n2 = nbr[c * 13 + o];
+                if (n2 < 0) continue;
+                const int kb = cell_count[n2], pb = cell_begin[n2];
+                for (int sgi = 0; sgi * SEG < kb; ++sgi, ++off) {
+                    if (static_cast<unsigned int>(off) >= item_cap) continue;
+                    items[off] = PairItem{own_begin, own_count, pb + sgi * SEG, min(SEG, kb - sgi * SEG),
+                                          o, 0, 0, 0};
+                }
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -646,7 +828,8 @@ __device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, i
     append_event(gs, P, ev);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P) {
+__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
+    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     const int p = blockIdx.x * BLOCK + threadIdx.x;
     const int A = static_cast<int>(gs->alive);
     PairCounters cnt{0, 0, 0};
@@ -656,14 +839,15 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P) {
         const int begin = P.cell_begin[c];
         const int k = P.cell_count[c];
         const int u = p - begin;
+        if (k >= DENSE_MIN) goto done;  // screened by k_pairs_dense
         const float4 la = P.lpos[p], va = P.lvel[p];
         const int ia = __float_as_int(la.w);
         auto visit = [&](int q, float ox, float oy, float oz) {
             ++cnt.cand;
             const float4 lb = P.lpos[q], vb = P.lvel[q];
-            if (!prefilter_keep(ox * cellf + (lb.x - la.x), oy * cellf + (lb.y - la.y),
-                                oz * cellf + (lb.z - la.z), vb.x - va.x, vb.y - va.y,
-                                vb.z - va.z, P.dt_f, P.thr_f, P.vmin2_f))
+            if (!prefilter_keep_fast(ox * cellf + (lb.x - la.x), oy * cellf + (lb.y - la.y),
+                                     oz * cellf + (lb.z - la.z), vb.x - va.x, vb.y - va.y,
+                                     vb.z - va.z, P.dt_f, P.thr_f, P.vmin2_f))
                 return;
             exact_pair(gs, P, ia, __float_as_int(lb.w), cnt);
         };
@@ -689,23 +873,144 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P) {
             for (int q = b0; q < b1; ++q) visit(q, ox, oy, oz);
         }
     }
+done:
+    const unsigned long long c0 = wave_sum<unsigned long long>(cnt.cand);
+    const unsigned long long c1 = wave_sum<unsigned long long>(cnt.refined);
+    const unsigned long long c2 = wave_sum<unsigned long long>(cnt.co);
+    const int wave = threadIdx.x / WAVE;
+    if (__lane_id() == 0) {
+        s_cnt[wave][0] = c0;
+        s_cnt[wave][1] = c1;
+        s_cnt[wave][2] = c2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, c = 0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+            a += s_cnt[w][0];
+            b += s_cnt[w][1];
+            c += s_cnt[w][2];
+        }
+        parts[blockIdx.x].cand = a;
+        parts[blockIdx.x].refined = b;
+        parts[blockIdx.x].co = c;
+    }
+}
+
+
+// Dense cells: one wave per work item (grid-stride over the items). The
+// partner index is wave-uniform, so each partner record is one scalar load
+// broadcast to all 64 lanes; each lane tests it against its own object.
+__global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
+                                                        const PairItem* __restrict__ items,
+                                                        const float4* __restrict__ lpos,
+                                                        const float4* __restrict__ lvel,
+                                                        unsigned int item_cap, PairPartial* parts) {
+    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
+    const unsigned int total_waves = gridDim.x * WAVES_PER_BLOCK;
+    const unsigned int n_items = min(gs->n_items, item_cap);
+    const float cellf = static_cast<float>(gs->cell);
+    PairCounters cnt{0, 0, 0};
+    for (unsigned int it = blockIdx.x * WAVES_PER_BLOCK + wave; it < n_items; it += total_waves) {
+        const PairItem item = items[it];
+        const int p = item.own_begin + lane;
+        const bool active = lane < item.own_count;
+        float4 la = make_float4(0.f, 0.f, 0.f, 0.f), va = la;
+        if (active) {
+            la = lpos[p];
+            va = lvel[p];
+        }
+        const int ia = __float_as_int(la.w);
+        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+        if (item.kind != KIND_INTRA) {
+            int dx, dy, dz;
+            forward_offset(item.kind, dx, dy, dz);
+            ox = static_cast<float>(dx) * cellf;
+            oy = static_cast<float>(dy) * cellf;
+            oz = static_cast<float>(dz) * cellf;
+        }
+        const bool intra = item.kind == KIND_INTRA;
+        const int q1 = item.part_begin + item.part_count;
+        for (int q = item.part_begin; q < q1; ++q) {
+            const float4 lb = lpos[q];
+            const float4 vb = lvel[q];
+            if (!active || (intra && q <= p)) continue;
+            ++cnt.cand;
+            if (!prefilter_keep_fast(ox + (lb.x - la.x), oy + (lb.y - la.y), oz + (lb.z - la.z),
+                                     vb.x - va.x, vb.y - va.y, vb.z - va.z, P.dt_f, P.thr_f,
+                                     P.vmin2_f))
+                continue;
+            exact_pair(gs, P, ia, __float_as_int(lb.w), cnt);
+        }
+    }
     const unsigned long long c0 = wave_sum<unsigned long long>(cnt.cand);
     const unsigned long long c1 = wave_sum<unsigned long long>(cnt.refined);
     const unsigned long long c2 = wave_sum<unsigned long long>(cnt.co);
     if (__lane_id() == 0) {
-        if (c0) atomicAdd(&gs->cand, c0);
-        if (c1) atomicAdd(&gs->refined, c1);
-        if (c2) atomicAdd(&gs->co, c2);
+        s_cnt[threadIdx.x / WAVE][0] = c0;
+        s_cnt[threadIdx.x / WAVE][1] = c1;
+        s_cnt[threadIdx.x / WAVE][2] = c2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0, c = 0;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+            a += s_cnt[w][0];
+            b += s_cnt[w][1];
+            c += s_cnt[w][2];
+        }
+        parts[blockIdx.x] = PairPartial{a, b, c, 0u, 0u};
     }
 }
 
-__global__ void k_finish(GridState* gs, DevStats* out, int screened) {
+__global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
+                                                   const PairPartial* parts, int nparts,
+                                                   const StepPartial* sparts, int nsparts,
+                                                   const PairPartial* dparts, int ndparts) {
+    __shared__ unsigned long long s_red[WAVES_PER_BLOCK][4];
+    (void)sparts;
+    (void)nsparts;
+    unsigned long long a = 0, b = 0, c = 0, m = 0;
+    if (screened) {
+        for (int k = threadIdx.x; k < nparts; k += BLOCK) {
+            a += parts[k].cand;
+            b += parts[k].refined;
+            c += parts[k].co;
+            m = parts[k].max_cell > m ? parts[k].max_cell : m;
+        }
+        for (int k = threadIdx.x; k < ndparts; k += BLOCK) {
+            a += dparts[k].cand;
+            b += dparts[k].refined;
+            c += dparts[k].co;
+        }
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    m = wave_max_u64(m);
+    const int wave = threadIdx.x / WAVE;
+    if (__lane_id() == 0) {
+        s_red[wave][0] = a;
+        s_red[wave][1] = b;
+        s_red[wave][2] = c;
+        s_red[wave][3] = m;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    for (int w = 1; w < WAVES_PER_BLOCK; ++w) {
+        s_red[0][0] += s_red[w][0];
+        s_red[0][1] += s_red[w][1];
+        s_red[0][2] += s_red[w][2];
+        s_red[0][3] = s_red[w][3] > s_red[0][3] ? s_red[w][3] : s_red[0][3];
+    }
     if (screened) {
         DevStats s;
-        s.cand = static_cast<long long>(gs->cand);
-        s.refined = static_cast<long long>(gs->refined);
-        s.co = static_cast<long long>(gs->co);
+        s.cand = static_cast<long long>(s_red[0][0]);
+        s.refined = static_cast<long long>(s_red[0][1]);
+        s.co = static_cast<long long>(s_red[0][2]);
+        gs->max_cell = static_cast<unsigned int>(s_red[0][3]);
         s.cell_km = gs->cell_ref;
         s.max_cell = static_cast<int>(gs->max_cell);
         s.doublings = gs->doublings;
@@ -723,6 +1028,8 @@ __global__ void k_finish(GridState* gs, DevStats* out, int screened) {
     gs->max_cell = 0;
     gs->ncells = 0;
     gs->npasses = 0;
+    gs->n_items = 0;
+    gs->n_dense = 0;
 }
 
 __global__ void k_fp64_check(int n, const double* a, const double* b, double* out) {
@@ -767,6 +1074,12 @@ struct khip_ctx {
     float4* lpos = nullptr;
     float4* lvel = nullptr;
     GridState* gs = nullptr;
+    StepPartial* step_parts = nullptr;
+    PairPartial* pair_parts = nullptr;
+    PairPartial* dense_parts = nullptr;
+    DenseCell* dense = nullptr;
+    PairItem* items = nullptr;
+    int64_t item_cap = 0;
 
     // Snapshot for re-running a chunk after an event-buffer overflow.
     double* snap_state = nullptr;
@@ -830,9 +1143,17 @@ void free_all(khip_ctx* ctx) {
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
                     ctx->cell_of, ctx->nbr, ctx->lpos, ctx->lvel, ctx->snap_state,
-                    ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs};
+                    ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
+                    ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+}
+
+bool ensure_items(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->item_cap && ctx->items) return true;
+    if (!dev_alloc(ctx, &ctx->items, want)) return false;
+    ctx->item_cap = want;
+    return true;
 }
 
 bool ensure_capacity(khip_ctx* ctx, int64_t n) {
@@ -853,7 +1174,11 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->nbr, 13 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
-           dev_alloc(ctx, &ctx->snap_alive, cap);
+           dev_alloc(ctx, &ctx->snap_alive, cap) &&
+           dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
+           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) &&
+           dev_alloc(ctx, &ctx->dense_parts, DENSE_BLOCKS) &&
+           dev_alloc(ctx, &ctx->dense, cap) && ensure_items(ctx, std::max<int64_t>(ctx->item_cap, 1 << 20));
 }
 
 bool ensure_events(khip_ctx* ctx, int64_t want) {
@@ -900,6 +1225,11 @@ bool upload_moon(khip_ctx* ctx, double t_start, int64_t nsteps) {
     return true;
 }
 
+// Initial device event capacity (48 B each). Cascades with fresh fragment
+// clouds report ~1,000 sibling conjunctions per step; a chunk of 256 steps
+// fits without a re-run.
+constexpr int64_t kInitialEvents = 1 << 22;
+
 inline int grid_of(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
 
 // Timing marks of step slot `slot` in the current chunk: 0 before the
@@ -926,7 +1256,8 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const Planes pb = ctx->planes(before), pa = ctx->planes(after);
-    k_grid_setup<<<1, 64, 0, s>>>(ctx->gs, ctx->dt, ctx->cfg.threshold_km, ctx->passes);
+    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), ctx->dt,
+                                     ctx->cfg.threshold_km, ctx->passes);
     k_keys<<<tiles, BLOCK, 0, s>>>(pb, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
                                    ctx->hist, tiles);
     for (int pass = 0; pass < ctx->passes; ++pass) {
@@ -943,8 +1274,12 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     k_cell_write<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
                                          ctx->gs, ctx->bcount, pb, ctx->cell_key, ctx->cell_begin,
                                          ctx->cell_of, ctx->lpos, ctx->lvel);
+    const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
-                                              ctx->cell_count, ctx->nbr);
+                                              ctx->cell_count, ctx->nbr, ctx->pair_parts, ctx->dense,
+                                              icap);
+    k_items<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->dense, ctx->cell_begin, ctx->cell_count,
+                                           ctx->nbr, ctx->items, icap);
     mark(ctx, 2);
     PairParams P;
     P.before = pb;
@@ -966,9 +1301,12 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     P.dt_f = static_cast<float>(P.dt);
     P.thr_f = static_cast<float>(P.thr);
     P.vmin2_f = static_cast<float>(P.vmin2);
-    k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P);
+    k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    k_pairs_dense<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
+                                                 ctx->dense_parts);
     mark(ctx, 3);
-    k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1);
+    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n),
+                                 ctx->step_parts, grid_of(n), ctx->dense_parts, DENSE_BLOCKS);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
 }
@@ -1013,6 +1351,10 @@ bool drain_chunk(khip_ctx* ctx, int64_t steps, int screen, bool& rerun) {
         if (!ensure_events(ctx, 2 * static_cast<int64_t>(g.events) + 1024)) return false;
         rerun = true;
     }
+    if (screen && g.items_short) {
+        if (!ensure_items(ctx, 2 * static_cast<int64_t>(g.items_max) + 4096)) return false;
+        rerun = true;
+    }
     if (rerun) return true;
     if (screen && g.passes_max > 0 && g.passes_max < ctx->passes) ctx->passes = g.passes_max;
     if (!screen) return true;
@@ -1038,7 +1380,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
     hipStream_t s = ctx->stream;
     if (n == 0) {
         for (int64_t k = 0; k < nsteps; ++k) {
-            if (screen) ctx->host_stats.push_back(khip_stats{0, 0, 0, 1.0, 0, 0, 0});
+            if (screen)
+                ctx->host_stats.push_back(khip_stats{0, 0, 0, std::max(1.0, ctx->cfg.threshold_km), 0, 0, 0});
             ctx->t += ctx->dt;
             ctx->step += 1;
         }
@@ -1047,7 +1390,7 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
     int64_t done = 0;
     while (done < nsteps) {
         const int64_t len = std::min<int64_t>(ctx->chunk, nsteps - done);
-        if (!ensure_stats(ctx, len) || !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, 1 << 16)) ||
+        if (!ensure_stats(ctx, len) || !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents)) ||
             (ctx->timing && !ensure_event_pool(ctx, len)))
             return false;
         if (!upload_moon(ctx, ctx->t, len)) return false;
@@ -1069,13 +1412,14 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 mark(ctx, 0);
                 k_propagate<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(cur), ctx->planes(nxt),
                                                          ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
-                                                         ctx->gs);
+                                                         ctx->step_parts);
                 mark(ctx, 1);
                 if (screen) {
                     if (!enqueue_screen(ctx, cur, nxt, t, ctx->step + done + k, static_cast<int>(k)))
                         return false;
                 } else {
-                    k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats, 0);
+                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0,
+                                                 ctx->step_parts, 0, ctx->dense_parts, 0);
                     mark(ctx, 3);
                 }
                 t += ctx->dt;
@@ -1279,7 +1623,7 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
              hipSuccess;
         if (!ok) ctx->error = "hipMemcpy of end-of-step states failed";
     }
-    ok = ok && ensure_stats(ctx, 1) && ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, 1 << 16));
+    ok = ok && ensure_stats(ctx, 1) && ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents));
     while (ok) {
         hipStream_t s = ctx->stream;
         k_reset<<<1, 64, 0, s>>>(ctx->gs);
@@ -1289,10 +1633,12 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
         if (n > 0) {
             mark(ctx, 0);
             mark(ctx, 1);
-            k_reduce<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n), ctx->gs);
+            k_reduce<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n),
+                                                  ctx->step_parts);
             ok = enqueue_screen(ctx, 0, 1, t0, 0, 0);
         } else {
-            k_finish<<<1, 64, 0, s>>>(ctx->gs, ctx->stats, 1);
+            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, ctx->step_parts, 0,
+                                         ctx->dense_parts, 0);
         }
         if (!ok) break;
         std::vector<khip_event> saved_ev;
