@@ -140,21 +140,39 @@ def main():
     cand = int(st["candidate_pairs"].sum())
     value = obj_steps / wall
 
-    kernels = {"pairs": float(tm["pairs_ms"]), "grid": float(tm["grid_ms"]),
-               "propagate": float(tm["propagate_ms"])}
-    dominant = max(kernels, key=kernels.get)
-    if dominant == "propagate":
-        achieved = BYTES_PER_OBJECT_STEP * obj_steps / (tm["propagate_ms"] * 1e-3) / 1e9
-        roof = {"kernel": "k_propagate", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                "traffic": None}
+    dense_cand = int(st["dense_candidates"].sum())
+    kernels = {"k_pairs": float(tm["pairs_sparse_ms"]), "k_pairs_dense": float(tm["pairs_dense_ms"]),
+               "k_exact": float(tm["exact_ms"]), "grid": float(tm["grid_ms"]),
+               "k_propagate": float(tm["propagate_ms"])}
+    single = {k: v for k, v in kernels.items() if k != "grid"}
+    dominant = max(single, key=single.get)
+    launches = max(1, int(tm["steps"]))
+
+    def valu_roof(name, ms, cand_count):
+        achieved = FLOP_PER_CANDIDATE * cand_count / (ms * 1e-3) / 1e12
+        return {"kernel": name, "bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                "avg_launch_us": 1e3 * ms / launches, "units_per_launch": cand_count / launches,
+                "unit_of_work": "candidate pair (60 FP32 flops)"}
+
+    prop_gbs = BYTES_PER_OBJECT_STEP * obj_steps / (tm["propagate_ms"] * 1e-3) / 1e9
+    roof_prop = {"kernel": "k_propagate", "bound": "hbm", "achieved": prop_gbs, "peak": PEAK_HBM_GBS,
+                 "unit": "GB/s", "frac": prop_gbs / PEAK_HBM_GBS, "traffic": None,
+                 "avg_launch_us": 1e3 * tm["propagate_ms"] / launches,
+                 "units_per_launch": obj_steps / launches,
+                 "unit_of_work": "propagated object (97 B: FP64 state in + out, alive flag)"}
+    if dominant == "k_pairs_dense":
+        roof = valu_roof("k_pairs_dense", tm["pairs_dense_ms"], dense_cand)
+    elif dominant == "k_pairs":
+        roof = valu_roof("k_pairs", tm["pairs_sparse_ms"], cand - dense_cand)
+    elif dominant == "k_propagate":
+        roof = roof_prop
     else:
-        ms = tm["pairs_ms"] if dominant == "pairs" else tm["grid_ms"]
-        achieved = FLOP_PER_CANDIDATE * cand / (ms * 1e-3) / 1e12
-        roof = {"kernel": "k_pairs" if dominant == "pairs" else "grid kernels",
-                "bound": "valu-fp32", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
-    roof["kernel_ms_per_step"] = {k: v / max(1, int(tm["steps"])) for k, v in kernels.items()}
+        roof = {"kernel": "k_exact", "bound": "valu-fp64", "achieved": None, "peak": 78.6,
+                "unit": "TFLOP/s", "frac": None, "traffic": None,
+                "avg_launch_us": 1e3 * tm["exact_ms"] / launches}
+    roof["kernel_ms_per_step"] = {k: v / launches for k, v in kernels.items()}
+    roof["reruns"] = int(tm["reruns"])
 
     out = {
         "metric": METRIC,
@@ -183,6 +201,7 @@ def main():
         "collisions": int(ev["collision"].sum()),
         "candidate_pairs": cand,
         "roofline": roof,
+        "roofline_propagate": roof_prop,
         "device": eng.device_name,
     }
     if not args.no_cpu:

@@ -79,18 +79,29 @@ typedef struct {
     int32_t max_cell_objects;
     int32_t cell_doublings; /* > 0 only where the grid extent needed a coarser cell (see khip_run) */
     int64_t propagated;     /* objects alive at the start of the step (0 for khip_screen) */
+    int64_t dense_candidates; /* part of candidate_pairs screened by dense-cell work items */
 } khip_stats;
 
 /* Device time per kernel family, summed over the steps since the last
  * khip_reset_timing, measured with HIP events on the engine's stream when
  * timing is enabled (khip_set_timing). */
 typedef struct {
-    double propagate_ms; /* fused RK4 step */
-    double grid_ms;      /* keys, radix sort, cell table, neighbours */
-    double pairs_ms;     /* fused enumeration + filters + TCA + append */
-    double total_ms;     /* whole steps, launch gaps included */
+    double propagate_ms;    /* k_propagate: fused RK4 step */
+    double grid_ms;         /* keys, radix sort, cell table, neighbour ranges, work items */
+    double pairs_ms;        /* the three kernels below */
+    double pairs_sparse_ms; /* k_pairs: objects of sparse cells, FP32 pre-filter */
+    double pairs_dense_ms;  /* k_pairs_dense: dense-cell work items, FP32 pre-filter */
+    double exact_ms;        /* k_exact: FP64 filter, TCA, radius test, event append */
+    double total_ms;        /* whole steps, launch gaps included */
     int64_t steps;
+    int64_t reruns;         /* chunks run again after a device buffer grew */
+    int64_t rerun_reasons;  /* OR of KHIP_RERUN_* over those re-runs */
 } khip_timing;
+
+#define KHIP_RERUN_PASSES 1    /* the cell-key extent needed more radix passes */
+#define KHIP_RERUN_EVENTS 2    /* event buffer */
+#define KHIP_RERUN_ITEMS 4     /* dense-cell work items */
+#define KHIP_RERUN_SURVIVORS 8 /* pre-filter survivor queue */
 
 /* Number of visible HIP devices (0 when none or the runtime fails). */
 int khip_device_count(void);

@@ -25,12 +25,14 @@ EVENT_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("step", "<i8"), ("t", "<f8"
 STATS_DTYPE = np.dtype([("candidate_pairs", "<i8"), ("refined_pairs", "<i8"),
                         ("co_orbiting", "<i8"), ("cell_km", "<f8"),
                         ("max_cell_objects", "<i4"), ("cell_doublings", "<i4"),
-                        ("propagated", "<i8")])
+                        ("propagated", "<i8"), ("dense_candidates", "<i8")])
 OBJECT_DTYPE = np.dtype([("id", "<i4"), ("event", "<i4"), ("alive", "<i4"), ("type", "<i4"),
                          ("state", "<f8", (6,)), ("size_m", "<f8"), ("mass_kg", "<f8"),
                          ("area_to_mass", "<f8")])
 TIMING_DTYPE = np.dtype([("propagate_ms", "<f8"), ("grid_ms", "<f8"), ("pairs_ms", "<f8"),
-                         ("total_ms", "<f8"), ("steps", "<i8")])
+                         ("pairs_sparse_ms", "<f8"), ("pairs_dense_ms", "<f8"), ("exact_ms", "<f8"),
+                         ("total_ms", "<f8"), ("steps", "<i8"), ("reruns", "<i8"),
+                         ("rerun_reasons", "<i8")])
 
 # Every symbol include/kessler_hip.h declares.
 ABI_SYMBOLS = (

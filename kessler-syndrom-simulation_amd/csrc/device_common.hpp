@@ -130,19 +130,20 @@ __device__ __forceinline__ double golden_min(const Hermite& h) {
     return 0.5 * (a + b);
 }
 
-// prefilter_keep (collision_filter.hpp) with hardware reciprocal and square
-// root (v_rcp_f32 / v_sqrt_f32, about 1 ulp) and a squared distance test.
-// Still conservative with respect to the exact FP64 filter_pair:
-//  * an error e in ts moves the computed distance by at most |v| |e|, i.e.
-//    ~2^-22 |v| dt for a 1-ulp reciprocal, against the 1e-5 (|r| + |v| dt)
-//    relative margin;
-//  * squaring both sides of d > gate + margin is monotone on non-negative
-//    values and adds ~1 ulp;
-//  * |v| and |r| only scale the gate and the margins.
+// prefilter_keep (collision_filter.hpp) rearranged for throughput, still
+// conservative with respect to the exact FP64 filter_pair:
+//  * ts uses the hardware reciprocal (v_rcp_f32, about 1 ulp): an error e in
+//    ts moves the computed distance by at most |v| |e| ~ 2^-22 |v| dt,
+//    against the 1e-5 (|r| + |v| dt) relative margin;
+//  * |r| and |v| enter only the gate and the margins, where the L1 norms
+//    (>= the Euclidean ones) make the gate and every margin larger and the
+//    receding tolerance larger: fewer rejections, never more;
+//  * the distance test compares squares (monotone on non-negative values).
 // Returns false only where prefilter_keep's own bound says filter_pair
-// rejects.
+// rejects. gdt3 = GRAVITY_GRADIENT * dt^3, base = threshold + 1e-3 +
+// DR_ERR_KM + DV_ERR_KM_S * dt, folded by the caller.
 __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz, float vx, float vy,
-                                                    float vz, float dt, float threshold_km,
+                                                    float vz, float dt, float gdt3, float base,
                                                     float vmin2) {
     // FP32 only: fused multiply-adds round once instead of twice, which can
     // only shrink the error the margins cover.
@@ -155,10 +156,9 @@ __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz
     ts = ts > dt ? dt : ts;
     const float cx = rx + vx * ts, cy = ry + vy * ts, cz = rz + vz * ts;
     const float d2 = cx * cx + cy * cy + cz * cz;
-    const float dvn = __builtin_amdgcn_sqrtf(dv2);
-    const float rn = __builtin_amdgcn_sqrtf(r2);
-    const float gate = threshold_km + static_cast<float>(GRAVITY_GRADIENT) * dvn * dt * dt * dt + 1e-3f;
-    const float lim = gate + 1e-5f * (rn + dvn * dt) + DR_ERR_KM + DV_ERR_KM_S * dt;
+    const float dvn = fabsf(vx) + fabsf(vy) + fabsf(vz);
+    const float rn = fabsf(rx) + fabsf(ry) + fabsf(rz);
+    const float lim = base + gdt3 * dvn + 1e-5f * (rn + dvn * dt);
     if (d2 > lim * lim) return false;
     const float rv_tol = 1e-5f * rn * dvn + DR_ERR_KM * dvn + DV_ERR_KM_S * rn;
     return !(dv2 > 1.1f * vmin2 && rv > rv_tol &&

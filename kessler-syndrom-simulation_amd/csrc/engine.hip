@@ -65,10 +65,13 @@ constexpr int MAX_PASSES = 8;
 // shells) are screened by work items: CHUNK own objects (one per lane)
 // against up to SEG partners read by the whole wave at once.
 constexpr int DENSE_MIN = 32;
+// A sparse cell's range longer than LONG_RANGE (a neighbouring cloud) is
+// also handed to the work items instead of being walked lane by lane.
+constexpr int LONG_RANGE = 64;
 constexpr int CHUNK = 64;
 constexpr int SEG = 256;
-constexpr int KIND_INTRA = 13;
 constexpr int DENSE_BLOCKS = 1024;
+constexpr int EXACT_BLOCKS = 512;
 
 // Device-resident grid parameters and per-step accumulators.
 struct GridState {
@@ -92,6 +95,10 @@ struct GridState {
     unsigned int n_dense;           // dense cells of this step
     unsigned int items_max;         // max n_items over the chunk
     int items_short;                // a step had more items than the buffer holds
+    unsigned long long n_surv;      // pre-filter survivors of this step
+    unsigned long long surv_max;    // max n_surv over the chunk
+    int surv_short;                 // a step had more survivors than the queue holds
+    int pad2;
     // Accumulated by k_pairs.
     unsigned long long cand, refined, co;
     unsigned int max_cell;
@@ -112,6 +119,7 @@ struct DevStats {
     double cell_km;
     int max_cell, doublings;
     long long propagated;
+    long long dense_cand;
 };
 static_assert(sizeof(DevStats) == sizeof(khip_stats), "stats layout");
 
@@ -132,8 +140,8 @@ struct PairPartial {
 
 // A dense work item: lanes take own objects [own_begin, own_begin +
 // own_count) of one cell; the wave walks partners [part_begin, part_begin +
-// part_count) of the same cell (kind KIND_INTRA: only partners after the
-// lane's own object) or of forward neighbour `kind` (0..12).
+// part_count) of neighbour range `kind` (0-4, see k_neighbours); in range 0
+// a lane pairs only with positions after its own.
 struct PairItem {
     int own_begin, own_count, part_begin, part_count;
     int kind, pad0, pad1, pad2;
@@ -265,6 +273,9 @@ __global__ void k_reset(GridState* gs) {
     gs->items_short = 0;
     gs->n_items = 0;
     gs->n_dense = 0;
+    gs->n_surv = 0;
+    gs->surv_max = 0;
+    gs->surv_short = 0;
     gs->vmax_bits = 0ULL;
     for (int a = 0; a < 3; ++a) {
         gs->lo[a] = ~0ULL;
@@ -535,7 +546,8 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
 // Compact per-object record for the pair kernel, in sorted order.
 //   pos: position relative to its cell's low corner (FP64 subtraction,
 //        rounded to float: the error DR_ERR_KM bounds), w = object index bits.
-//   vel: velocity rounded to float (DV_ERR_KM_S), w unused.
+//   vel: velocity rounded to float (DV_ERR_KM_S), w = the cell's packed z
+//        coordinate (int bits), from which pairs get their z cell offset.
 __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
@@ -587,7 +599,8 @@ __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const 
                                   static_cast<float>(cur.z[id] - static_cast<double>(iz) * cell),
                                   __int_as_float(id));
             lvel[e] = make_float4(static_cast<float>(cur.vx[id]), static_cast<float>(cur.vy[id]),
-                                  static_cast<float>(cur.vz[id]), 0.0f);
+                                  static_cast<float>(cur.vz[id]),
+                                  __int_as_float(static_cast<int>(key & mz)));
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -599,52 +612,73 @@ __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const 
     }
 }
 
-// Offsets of the 13 forward neighbours, in the order of
-// KESSLER_FORWARD_NEIGHBOURS, without a private-memory table.
-__device__ __forceinline__ void forward_offset(int o, int& dx, int& dy, int& dz) {
-    if (o < 9) {
-        dx = 1;
-        dy = o / 3 - 1;
-        dz = o % 3 - 1;
-    } else if (o < 12) {
-        dx = 0;
-        dy = 1;
-        dz = o - 10;
-    } else {
-        dx = 0;
-        dy = 0;
-        dz = 1;
-    }
+// ---------------------------------------------------------------------------
+// Neighbour ranges
+// ---------------------------------------------------------------------------
+//
+// Packed keys hold z in their lowest bits, so the cells (x', y', z-1),
+// (x', y', z) and (x', y', z+1) that exist are consecutive in key order and
+// their objects form one contiguous run of the sorted array. The 13 forward
+// neighbours of cell (x, y, z) (collision_filter.hpp:137-140) are therefore
+// four runs, for (dx, dy) = (1, -1), (1, 0), (1, 1), (0, 1), plus the single
+// cell (0, 0, 1), which directly follows cell (x, y, z) in key order and
+// merges with the cell's own tail. Each cell keeps NRANGES (begin, end)
+// runs of sorted positions:
+//   range 0   the cell itself and, if present, cell (x, y, z+1): a lane at
+//             sorted position p pairs with positions q > p only;
+//   range 1-4 the three-cell runs above; every object pairs.
+// The candidate set is the reference's (intra-cell pairs + 13 forward
+// cells), each pair once.
+
+constexpr int NRANGES = 5;
+
+__device__ __forceinline__ void range_offset(int r, int& dx, int& dy) {
+    dx = r == 4 ? 0 : (r == 0 ? 0 : 1);
+    dy = r == 0 ? 0 : (r == 4 ? 1 : r - 2);
 }
 
 __device__ __forceinline__ bool axis_fits(long long v, int bits) {
     return v >= 0 && (bits >= 62 || v < (1LL << bits));
 }
 
-__device__ __forceinline__ int find_cell(const uint64_t* ck, int ncells, uint64_t key) {
-    int lo = 0, hi = ncells;
+// First index in [0, n) whose key is >= key.
+__device__ __forceinline__ int lower_bound_u64(const uint64_t* ck, int n, uint64_t key) {
+    int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
         if (ck[mid] < key) lo = mid + 1;
         else hi = mid;
     }
-    return (lo < ncells && ck[lo] == key) ? lo : -1;
+    return lo;
 }
 
 __device__ __forceinline__ int segments(int n) { return (n + SEG - 1) / SEG; }
 
-// Work items of dense cell c with k objects whose forward neighbours hold
-// `fwd_segs` partner segments in all: per chunk of CHUNK own objects, the
-// intra-cell tail in SEG-long segments plus every forward segment.
-__device__ __forceinline__ int dense_items(int k, int fwd_segs) {
+// Dense cell with k objects at sorted positions [b, b + k): per chunk of
+// CHUNK own objects, the range-0 tail from the chunk start plus ranges 1-4,
+// each cut into SEG-long segments.
+// Items of a sparse cell (one chunk, k < DENSE_MIN): its ranges longer than
+// LONG_RANGE objects (range 0 counted from the cell start).
+__device__ __forceinline__ int long_range_items(const int2* rng) {
     int items = 0;
-    for (int i = 0; i * CHUNK < k; ++i) items += segments(k - i * CHUNK) + fwd_segs;
+    for (int r = 0; r < NRANGES; ++r) {
+        const int len = rng[r].y - rng[r].x;
+        if (len > LONG_RANGE) items += segments(len);
+    }
+    return items;
+}
+
+__device__ __forceinline__ int dense_items(int b, int k, const int2* rng) {
+    int fixed = 0;
+    for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x);
+    int items = 0;
+    for (int i = 0; i * CHUNK < k; ++i) items += segments(rng[0].y - (b + i * CHUNK)) + fixed;
     return items;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint64_t* cell_key,
                                                        const int* cell_begin, int* cell_count,
-                                                       int* nbr, PairPartial* parts,
+                                                       int2* ranges, PairPartial* parts,
                                                        DenseCell* dense, unsigned int item_cap) {
     __shared__ unsigned int s_max[WAVES_PER_BLOCK];
     __shared__ int s_wave[WAVES_PER_BLOCK];
@@ -656,31 +690,40 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
     int items = 0;
     if (c < nc) {
         const int b = cell_begin[c];
-        cnt = static_cast<unsigned int>((c + 1 < nc ? cell_begin[c + 1] : A) - b);
+        const int e = c + 1 < nc ? cell_begin[c + 1] : A;
+        cnt = static_cast<unsigned int>(e - b);
         cell_count[c] = static_cast<int>(cnt);
         const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
         const uint64_t key = cell_key[c];
+        const uint64_t mz = (1ULL << bz) - 1ULL;
         const long long x = static_cast<long long>(key >> (by + bz));
         const long long y = static_cast<long long>((key >> bz) & ((1ULL << by) - 1ULL));
-        const long long z = static_cast<long long>(key & ((1ULL << bz) - 1ULL));
-        int fwd_segs = 0;
-        for (int o = 0; o < 13; ++o) {
-            int ox, oy, oz;
-            forward_offset(o, ox, oy, oz);
-            const long long nx = x + ox, ny = y + oy, nz = z + oz;
-            int found = -1;
-            if (axis_fits(nx, bx) && axis_fits(ny, by) && axis_fits(nz, bz)) {
-                const uint64_t nk = (static_cast<uint64_t>(nx) << (by + bz)) |
-                                    (static_cast<uint64_t>(ny) << bz) | static_cast<uint64_t>(nz);
-                found = find_cell(cell_key, nc, nk);
+        const long long z = static_cast<long long>(key & mz);
+        int2 rng[NRANGES];
+        // Range 0: own cell, then (x, y, z+1) if it exists (the next key).
+        int e0 = e;
+        if (c + 1 < nc && cell_key[c + 1] == key + 1ULL && (key & mz) != mz)
+            e0 = c + 2 < nc ? cell_begin[c + 2] : A;
+        rng[0] = make_int2(b, e0);
+        const long long zlo = z > 0 ? z - 1 : 0;
+        const long long zhi = axis_fits(z + 1, bz) ? z + 1 : z;
+        for (int r = 1; r < NRANGES; ++r) {
+            int dx, dy;
+            range_offset(r, dx, dy);
+            const long long nx = x + dx, ny = y + dy;
+            int2 run = make_int2(0, 0);
+            if (axis_fits(nx, bx) && axis_fits(ny, by)) {
+                const uint64_t row = (static_cast<uint64_t>(nx) << (by + bz)) |
+                                     (static_cast<uint64_t>(ny) << bz);
+                const int i0 = lower_bound_u64(cell_key, nc, row | static_cast<uint64_t>(zlo));
+                const int i1 = lower_bound_u64(cell_key, nc, (row | static_cast<uint64_t>(zhi)) + 1ULL);
+                if (i0 < i1) run = make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A);
             }
-            nbr[c * 13 + o] = found;
-            if (found >= 0) {
-                const int kn = (found + 1 < nc ? cell_begin[found + 1] : A) - cell_begin[found];
-                fwd_segs += segments(kn);
-            }
+            rng[r] = run;
         }
-        if (static_cast<int>(cnt) >= DENSE_MIN) items = dense_items(static_cast<int>(cnt), fwd_segs);
+        for (int r = 0; r < NRANGES; ++r) ranges[c * NRANGES + r] = rng[r];
+        items = static_cast<int>(cnt) >= DENSE_MIN ? dense_items(b, static_cast<int>(cnt), rng)
+                                                   : long_range_items(rng);
     }
     // Block-aggregated registration of dense cells: two atomics per block.
     int tot_items, tot_dense;
@@ -710,7 +753,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
 // (grid-stride), one thread per chunk.
 __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell* dense,
                                                   const int* cell_begin, const int* cell_count,
-                                                  const int* nbr, PairItem* items,
+                                                  const int2* ranges, PairItem* items,
                                                   unsigned int item_cap) {
     const int nd = static_cast<int>(gs->n_dense);
     if (blockIdx.x == 0 && threadIdx.x == 0 && gs->n_items > gs->items_max) gs->items_max = gs->n_items;
@@ -719,31 +762,37 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
         const int c = dc.cell;
         const int b = cell_begin[c];
         const int k = cell_count[c];
-        int fwd_segs = 0;
-        for (int o = 0; o < 13; ++o) {
-            const int n2 = nbr[c * 13 + o];
-            if (n2 >= 0) fwd_segs += segments(cell_count[n2]);
+        int2 rng[NRANGES];
+        for (int r = 0; r < NRANGES; ++r) rng[r] = ranges[c * NRANGES + r];
+        if (k < DENSE_MIN) {
+            // Sparse cell: only its long ranges, own objects = the whole cell.
+            if (threadIdx.x != 0) continue;
+            int off = dc.item_base;
+            for (int r = 0; r < NRANGES; ++r) {
+                const int len = rng[r].y - rng[r].x;
+                if (len <= LONG_RANGE) continue;
+                for (int sgi = 0; sgi * SEG < len; ++sgi, ++off) {
+                    if (static_cast<unsigned int>(off) >= item_cap) continue;
+                    items[off] = PairItem{b, k, rng[r].x + sgi * SEG, min(SEG, len - sgi * SEG), r, 0, 0, 0};
+                }
+            }
+            continue;
         }
+        int fixed = 0;
+        for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x);
         const int nch = (k + CHUNK - 1) / CHUNK;
         for (int i = threadIdx.x; i < nch; i += BLOCK) {
-            int off = dc.item_base + i * fwd_segs;
-            for (int j = 0; j < i; ++j) off += segments(k - j * CHUNK);
+            int off = dc.item_base + i * fixed;
+            for (int j = 0; j < i; ++j) off += segments(rng[0].y - (b + j * CHUNK));
             const int own_begin = b + i * CHUNK;
             const int own_count = min(CHUNK, k - i * CHUNK);
-            const int tail = k - i * CHUNK;
-            for (int sgi = 0; sgi * SEG < tail; ++sgi, ++off) {
-                if (static_cast<unsigned int>(off) >= item_cap) continue;
-                items[off] = PairItem{own_begin, own_count, own_begin + sgi * SEG,
-                                      min(SEG, tail - sgi * SEG), KIND_INTRA, 0, 0, 0};
-            }
-            for (int o = 0; o < 13; ++o) {
-                const int n2 = nbr[c * 13 + o];
-                if (n2 < 0) continue;
-                const int kb = cell_count[n2], pb = cell_begin[n2];
-                for (int sgi = 0; sgi * SEG < kb; ++sgi, ++off) {
+            for (int r = 0; r < NRANGES; ++r) {
+                const int r0 = r == 0 ? own_begin : rng[r].x;
+                const int len = rng[r].y - r0;
+                for (int sgi = 0; sgi * SEG < len; ++sgi, ++off) {
                     if (static_cast<unsigned int>(off) >= item_cap) continue;
-                    items[off] = PairItem{own_begin, own_count, pb + sgi * SEG, min(SEG, kb - sgi * SEG),
-                                          o, 0, 0, 0};
+                    items[off] = PairItem{own_begin, own_count, r0 + sgi * SEG, min(SEG, len - sgi * SEG),
+                                          r, 0, 0, 0};
                 }
             }
         }
@@ -760,14 +809,18 @@ struct PairParams {
     const int* cell_of;
     const int* cell_begin;
     const int* cell_count;
-    const int* nbr;
+    const int2* ranges;
     const float4* lpos;
     const float4* lvel;
     DevEvent* events;
     unsigned long long cap;
+    uint2* surv;                    // pre-filter survivors (object index pairs)
+    unsigned long long surv_cap;
     long long step;
     double t0, dt, thr, vmin2;
     float dt_f, thr_f, vmin2_f;
+    float gdt3_f;   // GRAVITY_GRADIENT * dt^3
+    float base_f;   // threshold + 1e-3 + DR_ERR_KM + DV_ERR_KM_S * dt
 };
 
 struct PairCounters {
@@ -828,52 +881,37 @@ __device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, i
     append_event(gs, P, ev);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
-    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
-    const int A = static_cast<int>(gs->alive);
-    PairCounters cnt{0, 0, 0};
-    if (p < A && gs->npasses > 0) {
-        const float cellf = static_cast<float>(gs->cell);
-        const int c = P.cell_of[p];
-        const int begin = P.cell_begin[c];
-        const int k = P.cell_count[c];
-        const int u = p - begin;
-        if (k >= DENSE_MIN) goto done;  // screened by k_pairs_dense
-        const float4 la = P.lpos[p], va = P.lvel[p];
-        const int ia = __float_as_int(la.w);
-        auto visit = [&](int q, float ox, float oy, float oz) {
-            ++cnt.cand;
-            const float4 lb = P.lpos[q], vb = P.lvel[q];
-            if (!prefilter_keep_fast(ox * cellf + (lb.x - la.x), oy * cellf + (lb.y - la.y),
-                                     oz * cellf + (lb.z - la.z), vb.x - va.x, vb.y - va.y,
-                                     vb.z - va.z, P.dt_f, P.thr_f, P.vmin2_f))
-                return;
-            exact_pair(gs, P, ia, __float_as_int(lb.w), cnt);
-        };
-        // Intra-cell pairs, balanced: lane u sees the floor((k-1)/2) objects
-        // after it (cyclically), plus the opposite one when k is even and
-        // u < k/2. Every pair once, equal work per lane.
-        const int half = (k - 1) / 2;
-        for (int m = 1; m <= half; ++m) {
-            int q = u + m;
-            if (q >= k) q -= k;
-            visit(begin + q, 0.0f, 0.0f, 0.0f);
-        }
-        if ((k & 1) == 0 && u < k / 2) visit(begin + u + k / 2, 0.0f, 0.0f, 0.0f);
-        for (int o = 0; o < 13; ++o) {
-            const int nc = P.nbr[c * 13 + o];
-            if (nc < 0) continue;
-            int dx, dy, dz;
-            forward_offset(o, dx, dy, dz);
-            const float ox = static_cast<float>(dx);
-            const float oy = static_cast<float>(dy);
-            const float oz = static_cast<float>(dz);
-            const int b0 = P.cell_begin[nc], b1 = b0 + P.cell_count[nc];
-            for (int q = b0; q < b1; ++q) visit(q, ox, oy, oz);
-        }
-    }
-done:
+// Appends a pre-filter survivor, wave-aggregated among the lanes that
+// reach this point together.
+__device__ __forceinline__ void push_survivor(GridState* gs, const PairParams& P, int a, int b) {
+    const unsigned long long m = __ballot(1);
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    const int rank = __popcll(m & lanes_below());
+    unsigned long long base = 0;
+    if (static_cast<int>(__lane_id()) == leader)
+        base = atomicAdd(&gs->n_surv, static_cast<unsigned long long>(__popcll(m)));
+    base = __shfl(base, leader, WAVE);
+    const unsigned long long slot = base + rank;
+    if (slot < P.surv_cap) P.surv[slot] = make_uint2(static_cast<unsigned int>(a), static_cast<unsigned int>(b));
+}
+
+// One candidate: FP32 relative state from the compact records (positions
+// relative to the cell corner, cell offset (dx, dy, dz) in whole cells,
+// built as in the reference's GPU path so that DR_ERR_KM bounds it). Pairs
+// the pre-filter keeps go to the survivor queue for the FP64 kernel.
+__device__ __forceinline__ void candidate(GridState* gs, const PairParams& P, float4 la, float4 va,
+                                          float4 lb, float4 vb, float ox, float oy, float cellf,
+                                          unsigned int& cand) {
+    ++cand;
+    const float oz = static_cast<float>(__float_as_int(vb.w) - __float_as_int(va.w)) * cellf;
+    if (prefilter_keep_fast(ox + (lb.x - la.x), oy + (lb.y - la.y), oz + (lb.z - la.z),
+                            vb.x - va.x, vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
+                            P.vmin2_f))
+        push_survivor(gs, P, __float_as_int(la.w), __float_as_int(lb.w));
+}
+
+__device__ __forceinline__ void store_counters(PairPartial* parts, const PairCounters& cnt,
+                                               unsigned long long (*s_cnt)[3]) {
     const unsigned long long c0 = wave_sum<unsigned long long>(cnt.cand);
     const unsigned long long c1 = wave_sum<unsigned long long>(cnt.refined);
     const unsigned long long c2 = wave_sum<unsigned long long>(cnt.co);
@@ -897,10 +935,57 @@ done:
     }
 }
 
+// Sparse cells: one lane per sorted object of a cell with fewer than
+// DENSE_MIN objects, walking its cell's five ranges. The next partner's
+// record is loaded before the current one is tested.
+__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
+    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
+    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    const int A = static_cast<int>(gs->alive);
+    unsigned int cand = 0;
+    if (p < A && gs->npasses > 0) {
+        const int c = P.cell_of[p];
+        if (P.cell_count[c] < DENSE_MIN) {
+            const float cellf = static_cast<float>(gs->cell);
+            const float4 la = P.lpos[p], va = P.lvel[p];
+            for (int r = 0; r < NRANGES; ++r) {
+                const int2 run = P.ranges[c * NRANGES + r];
+                if (run.y - run.x > LONG_RANGE) continue;  // a work item covers it
+                int dx, dy;
+                range_offset(r, dx, dy);
+                const float ox = static_cast<float>(dx) * cellf, oy = static_cast<float>(dy) * cellf;
+                int q = r == 0 ? p + 1 : run.x;
+                if (q >= run.y) continue;
+                float4 lb = P.lpos[q], vb = P.lvel[q];
+                for (; q < run.y; ++q) {
+                    const int nq = q + 1 < run.y ? q + 1 : q;
+                    const float4 nlb = P.lpos[nq], nvb = P.lvel[nq];
+                    candidate(gs, P, la, va, lb, vb, ox, oy, cellf, cand);
+                    lb = nlb;
+                    vb = nvb;
+                }
+            }
+        }
+    }
+    store_counters(parts, PairCounters{cand, 0, 0}, s_cnt);
+}
+
+__device__ __forceinline__ float4 lane_broadcast(float4 v, int j) {
+    return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), j)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), j)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), j)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), j)));
+}
+
+constexpr unsigned int SURV_INVALID = 0xffffffffu;
 
 // Dense cells: one wave per work item (grid-stride over the items). The
-// partner index is wave-uniform, so each partner record is one scalar load
-// broadcast to all 64 lanes; each lane tests it against its own object.
+// wave loads 64 partner records at a time (one per lane, coalesced) and
+// broadcasts them one by one with v_readlane, so the inner loop touches no
+// memory. Survivor slots come from per-wave reservations that grow
+// geometrically (64 .. 4096), so a fresh cloud whose every pair survives
+// costs one counter atomic per thousands of survivors; unused reserved slots
+// are marked SURV_INVALID.
 __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
                                                         const PairItem* __restrict__ items,
                                                         const float4* __restrict__ lpos,
@@ -912,7 +997,10 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
     const unsigned int total_waves = gridDim.x * WAVES_PER_BLOCK;
     const unsigned int n_items = min(gs->n_items, item_cap);
     const float cellf = static_cast<float>(gs->cell);
-    PairCounters cnt{0, 0, 0};
+    const unsigned long long below = lanes_below();
+    unsigned int cand = 0;
+    unsigned long long rbase = 0;  // wave-uniform survivor reservation
+    unsigned int rleft = 0, rsize = 32;
     for (unsigned int it = blockIdx.x * WAVES_PER_BLOCK + wave; it < n_items; it += total_waves) {
         const PairItem item = items[it];
         const int p = item.own_begin + lane;
@@ -922,80 +1010,110 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
             la = lpos[p];
             va = lvel[p];
         }
-        const int ia = __float_as_int(la.w);
-        float ox = 0.0f, oy = 0.0f, oz = 0.0f;
-        if (item.kind != KIND_INTRA) {
-            int dx, dy, dz;
-            forward_offset(item.kind, dx, dy, dz);
-            ox = static_cast<float>(dx) * cellf;
-            oy = static_cast<float>(dy) * cellf;
-            oz = static_cast<float>(dz) * cellf;
-        }
-        const bool intra = item.kind == KIND_INTRA;
+        int dx, dy;
+        range_offset(item.kind, dx, dy);
+        // Cell-corner offsets folded into the own position: dr = lb - (la - o).
+        const float ax = la.x - static_cast<float>(dx) * cellf;
+        const float ay = la.y - static_cast<float>(dy) * cellf;
+        const int za = __float_as_int(va.w);
+        // Range 0 pairs only with later positions.
+        const int first = item.kind == 0 ? p + 1 : -1;
         const int q1 = item.part_begin + item.part_count;
-        for (int q = item.part_begin; q < q1; ++q) {
-            const float4 lb = lpos[q];
-            const float4 vb = lvel[q];
-            if (!active || (intra && q <= p)) continue;
-            ++cnt.cand;
-            if (!prefilter_keep_fast(ox + (lb.x - la.x), oy + (lb.y - la.y), oz + (lb.z - la.z),
-                                     vb.x - va.x, vb.y - va.y, vb.z - va.z, P.dt_f, P.thr_f,
-                                     P.vmin2_f))
-                continue;
-            exact_pair(gs, P, ia, __float_as_int(lb.w), cnt);
+        for (int q0 = item.part_begin; q0 < q1; q0 += WAVE) {
+            const int nb = min(WAVE, q1 - q0);
+            float4 tp = make_float4(0.f, 0.f, 0.f, 0.f), tv = tp;
+            if (lane < nb) {
+                tp = lpos[q0 + lane];
+                tv = lvel[q0 + lane];
+            }
+            for (int j = 0; j < nb; ++j) {
+                const float4 lb = lane_broadcast(tp, j);
+                const float4 vb = lane_broadcast(tv, j);
+                const bool test = active && q0 + j >= first;
+                bool keep = false;
+                if (test) {
+                    ++cand;
+                    const float oz = static_cast<float>(__float_as_int(vb.w) - za) * cellf;
+                    // (ox + (lb.x - la.x)) regrouped as lb.x - (la.x - ox): the
+                    // same operands, one rounding each, inside DR_ERR_KM.
+                    keep = prefilter_keep_fast(lb.x - ax, lb.y - ay, oz + (lb.z - la.z), vb.x - va.x,
+                                               vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
+                                               P.vmin2_f);
+                }
+                const unsigned long long m = __ballot(keep);
+                if (m == 0ULL) continue;
+                const unsigned int cnt = static_cast<unsigned int>(__popcll(m));
+                if (rleft < cnt) {
+                    if (static_cast<unsigned int>(lane) < rleft && rbase + lane < P.surv_cap)
+                        P.surv[rbase + lane] = make_uint2(SURV_INVALID, SURV_INVALID);
+                    rsize = rsize < 4096u ? 2u * rsize : 4096u;
+                    const unsigned int R = rsize > cnt ? rsize : cnt;
+                    unsigned long long b = 0;
+                    if (lane == 0) b = atomicAdd(&gs->n_surv, static_cast<unsigned long long>(R));
+                    rbase = __shfl(b, 0, WAVE);
+                    rleft = R;
+                }
+                if (keep) {
+                    const unsigned long long slot = rbase + __popcll(m & below);
+                    if (slot < P.surv_cap)
+                        P.surv[slot] = make_uint2(static_cast<unsigned int>(__float_as_int(la.w)),
+                                                  static_cast<unsigned int>(__float_as_int(lb.w)));
+                }
+                rbase += cnt;
+                rleft -= cnt;
+            }
         }
     }
-    const unsigned long long c0 = wave_sum<unsigned long long>(cnt.cand);
-    const unsigned long long c1 = wave_sum<unsigned long long>(cnt.refined);
-    const unsigned long long c2 = wave_sum<unsigned long long>(cnt.co);
-    if (__lane_id() == 0) {
-        s_cnt[threadIdx.x / WAVE][0] = c0;
-        s_cnt[threadIdx.x / WAVE][1] = c1;
-        s_cnt[threadIdx.x / WAVE][2] = c2;
+    for (unsigned int k = static_cast<unsigned int>(lane); k < rleft; k += WAVE)
+        if (rbase + k < P.surv_cap) P.surv[rbase + k] = make_uint2(SURV_INVALID, SURV_INVALID);
+    store_counters(parts, PairCounters{cand, 0, 0}, s_cnt);
+}
+
+// FP64 exact filter, TCA and radius test of the pre-filter survivors: one
+// lane per survivor (grid-stride), coherent FP64 work.
+__global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, PairPartial* parts) {
+    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
+    const unsigned long long n = gs->n_surv < P.surv_cap ? gs->n_surv : P.surv_cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && gs->n_surv > P.surv_cap) gs->surv_short = 1;
+    PairCounters cnt{0, 0, 0};
+    for (unsigned long long k = static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x; k < n;
+         k += static_cast<unsigned long long>(gridDim.x) * BLOCK) {
+        const uint2 pr = P.surv[k];
+        if (pr.x == SURV_INVALID) continue;  // unused reserved slot
+        exact_pair(gs, P, static_cast<int>(pr.x), static_cast<int>(pr.y), cnt);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long a = 0, b = 0, c = 0;
-        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
-            a += s_cnt[w][0];
-            b += s_cnt[w][1];
-            c += s_cnt[w][2];
-        }
-        parts[blockIdx.x] = PairPartial{a, b, c, 0u, 0u};
-    }
+    store_counters(parts, cnt, s_cnt);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
                                                    const PairPartial* parts, int nparts,
-                                                   const StepPartial* sparts, int nsparts,
-                                                   const PairPartial* dparts, int ndparts) {
-    __shared__ unsigned long long s_red[WAVES_PER_BLOCK][4];
-    (void)sparts;
-    (void)nsparts;
-    unsigned long long a = 0, b = 0, c = 0, m = 0;
+                                                   const PairPartial* dparts, int ndparts,
+                                                   const PairPartial* eparts, int neparts) {
+    __shared__ unsigned long long s_red[WAVES_PER_BLOCK][5];
+    unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0;
     if (screened) {
         for (int k = threadIdx.x; k < nparts; k += BLOCK) {
             a += parts[k].cand;
-            b += parts[k].refined;
-            c += parts[k].co;
             m = parts[k].max_cell > m ? parts[k].max_cell : m;
         }
-        for (int k = threadIdx.x; k < ndparts; k += BLOCK) {
-            a += dparts[k].cand;
-            b += dparts[k].refined;
-            c += dparts[k].co;
+        for (int k = threadIdx.x; k < ndparts; k += BLOCK) dn += dparts[k].cand;
+        for (int k = threadIdx.x; k < neparts; k += BLOCK) {
+            b += eparts[k].refined;
+            c += eparts[k].co;
         }
     }
     a = wave_sum(a);
     b = wave_sum(b);
     c = wave_sum(c);
     m = wave_max_u64(m);
+    dn = wave_sum(dn);
     const int wave = threadIdx.x / WAVE;
     if (__lane_id() == 0) {
         s_red[wave][0] = a;
         s_red[wave][1] = b;
         s_red[wave][2] = c;
         s_red[wave][3] = m;
+        s_red[wave][4] = dn;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -1004,10 +1122,12 @@ __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, 
         s_red[0][1] += s_red[w][1];
         s_red[0][2] += s_red[w][2];
         s_red[0][3] = s_red[w][3] > s_red[0][3] ? s_red[w][3] : s_red[0][3];
+        s_red[0][4] += s_red[w][4];
     }
     if (screened) {
         DevStats s;
-        s.cand = static_cast<long long>(s_red[0][0]);
+        s.cand = static_cast<long long>(s_red[0][0] + s_red[0][4]);
+        s.dense_cand = static_cast<long long>(s_red[0][4]);
         s.refined = static_cast<long long>(s_red[0][1]);
         s.co = static_cast<long long>(s_red[0][2]);
         gs->max_cell = static_cast<unsigned int>(s_red[0][3]);
@@ -1030,6 +1150,8 @@ __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, 
     gs->npasses = 0;
     gs->n_items = 0;
     gs->n_dense = 0;
+    if (gs->n_surv > gs->surv_max) gs->surv_max = gs->n_surv;
+    gs->n_surv = 0;
 }
 
 __global__ void k_fp64_check(int n, const double* a, const double* b, double* out) {
@@ -1070,13 +1192,16 @@ struct khip_ctx {
     int* cell_begin = nullptr;
     int* cell_count = nullptr;
     int* cell_of = nullptr;
-    int* nbr = nullptr;
+    int2* ranges = nullptr;
     float4* lpos = nullptr;
     float4* lvel = nullptr;
     GridState* gs = nullptr;
     StepPartial* step_parts = nullptr;
     PairPartial* pair_parts = nullptr;
     PairPartial* dense_parts = nullptr;
+    PairPartial* exact_parts = nullptr;
+    uint2* surv = nullptr;
+    int64_t surv_cap = 0;
     DenseCell* dense = nullptr;
     PairItem* items = nullptr;
     int64_t item_cap = 0;
@@ -1097,8 +1222,11 @@ struct khip_ctx {
     int64_t step = 0;
     int chunk = 256;
     int passes = 4;   // radix passes launched per step (adapted per chunk)
+    int shrink_votes = 0;  // consecutive chunks that needed fewer passes
+    int64_t last_rerun = 0;  // reasons of the last re-run (KHIP_RERUN_* bits)
 
     std::vector<khip_event> host_events;
+    std::vector<khip_event> raw_events;  // one chunk's events as the device wrote them
     std::vector<khip_stats> host_stats;
 
     bool timing = false;
@@ -1142,9 +1270,10 @@ void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->nbr, ctx->lpos, ctx->lvel, ctx->snap_state,
+                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
-                    ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items};
+                    ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
+                    ctx->exact_parts, ctx->surv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1153,6 +1282,13 @@ bool ensure_items(khip_ctx* ctx, int64_t want) {
     if (want <= ctx->item_cap && ctx->items) return true;
     if (!dev_alloc(ctx, &ctx->items, want)) return false;
     ctx->item_cap = want;
+    return true;
+}
+
+bool ensure_surv(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->surv_cap && ctx->surv) return true;
+    if (!dev_alloc(ctx, &ctx->surv, want)) return false;
+    ctx->surv_cap = want;
     return true;
 }
 
@@ -1171,13 +1307,15 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->totals, RADIX) && dev_alloc(ctx, &ctx->bcount, tiles) &&
            dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
-           dev_alloc(ctx, &ctx->nbr, 13 * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
            dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
            dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) &&
            dev_alloc(ctx, &ctx->dense_parts, DENSE_BLOCKS) &&
+           dev_alloc(ctx, &ctx->exact_parts, EXACT_BLOCKS) &&
+           ensure_surv(ctx, std::max<int64_t>(ctx->surv_cap, 1 << 24)) &&
            dev_alloc(ctx, &ctx->dense, cap) && ensure_items(ctx, std::max<int64_t>(ctx->item_cap, 1 << 20));
 }
 
@@ -1233,11 +1371,13 @@ constexpr int64_t kInitialEvents = 1 << 22;
 inline int grid_of(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
 
 // Timing marks of step slot `slot` in the current chunk: 0 before the
-// propagation, 1 after it, 2 before the pair kernel, 3 after it.
+// propagation, 1 after it, 2 before the pair kernels, 3 after k_pairs,
+// 4 after k_pairs_dense, 5 after k_exact.
+constexpr int MARKS = 6;
 int g_slot = 0;
 
 bool ensure_event_pool(khip_ctx* ctx, int64_t steps) {
-    while (static_cast<int64_t>(ctx->ev_pool.size()) < 4 * steps) {
+    while (static_cast<int64_t>(ctx->ev_pool.size()) < MARKS * steps) {
         hipEvent_t e;
         KHIP_TRY(ctx, hipEventCreate(&e));
         ctx->ev_pool.push_back(e);
@@ -1246,7 +1386,7 @@ bool ensure_event_pool(khip_ctx* ctx, int64_t steps) {
 }
 
 void mark(khip_ctx* ctx, int k) {
-    if (ctx->timing) (void)hipEventRecord(ctx->ev_pool[4 * g_slot + k], ctx->stream);
+    if (ctx->timing) (void)hipEventRecord(ctx->ev_pool[MARKS * g_slot + k], ctx->stream);
 }
 
 // Enqueues the screening kernels for the step whose start / end states are
@@ -1276,10 +1416,10 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
                                          ctx->cell_of, ctx->lpos, ctx->lvel);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
-                                              ctx->cell_count, ctx->nbr, ctx->pair_parts, ctx->dense,
+                                              ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
                                               icap);
     k_items<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->dense, ctx->cell_begin, ctx->cell_count,
-                                           ctx->nbr, ctx->items, icap);
+                                           ctx->ranges, ctx->items, icap);
     mark(ctx, 2);
     PairParams P;
     P.before = pb;
@@ -1288,11 +1428,13 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     P.cell_of = ctx->cell_of;
     P.cell_begin = ctx->cell_begin;
     P.cell_count = ctx->cell_count;
-    P.nbr = ctx->nbr;
+    P.ranges = ctx->ranges;
     P.lpos = ctx->lpos;
     P.lvel = ctx->lvel;
     P.events = ctx->events;
     P.cap = static_cast<unsigned long long>(ctx->ev_cap);
+    P.surv = ctx->surv;
+    P.surv_cap = static_cast<unsigned long long>(ctx->surv_cap);
     P.step = step;
     P.t0 = t0;
     P.dt = ctx->dt;
@@ -1301,12 +1443,17 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     P.dt_f = static_cast<float>(P.dt);
     P.thr_f = static_cast<float>(P.thr);
     P.vmin2_f = static_cast<float>(P.vmin2);
+    P.gdt3_f = static_cast<float>(GRAVITY_GRADIENT) * P.dt_f * P.dt_f * P.dt_f;
+    P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
     k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    mark(ctx, 3);
     k_pairs_dense<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
                                                  ctx->dense_parts);
-    mark(ctx, 3);
+    mark(ctx, 4);
+    k_exact<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
+    mark(ctx, 5);
     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n),
-                                 ctx->step_parts, grid_of(n), ctx->dense_parts, DENSE_BLOCKS);
+                                 ctx->dense_parts, DENSE_BLOCKS, ctx->exact_parts, EXACT_BLOCKS);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
 }
@@ -1314,19 +1461,24 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
 // Called after the chunk's stream synchronisation.
 void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
     for (int64_t k = 0; k < steps; ++k) {
-        hipEvent_t* e = ctx->ev_pool.data() + 4 * k;
-        float a = 0, b = 0, c = 0, d = 0;
+        hipEvent_t* e = ctx->ev_pool.data() + MARKS * k;
+        float a = 0, b = 0, c = 0, d = 0, f = 0, g = 0;
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         ctx->tm.propagate_ms += a;
         if (screened) {
             (void)hipEventElapsedTime(&b, e[1], e[2]);
             (void)hipEventElapsedTime(&c, e[2], e[3]);
-            (void)hipEventElapsedTime(&d, e[0], e[3]);
+            (void)hipEventElapsedTime(&f, e[3], e[4]);
+            (void)hipEventElapsedTime(&g, e[4], e[5]);
+            (void)hipEventElapsedTime(&d, e[0], e[5]);
             ctx->tm.grid_ms += b;
-            ctx->tm.pairs_ms += c;
+            ctx->tm.pairs_sparse_ms += c;
+            ctx->tm.pairs_dense_ms += f;
+            ctx->tm.exact_ms += g;
+            ctx->tm.pairs_ms += c + f + g;
             ctx->tm.total_ms += d;
         } else {
-            (void)hipEventElapsedTime(&d, e[0], e[3]);
+            (void)hipEventElapsedTime(&d, e[0], e[5]);
             ctx->tm.total_ms += d;
         }
         ctx->tm.steps += 1;
@@ -1338,37 +1490,64 @@ void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
 // device buffer (grown here), or a step needed more radix passes than were
 // launched (raised here). The pass count also shrinks to what the chunk
 // needed, so the next chunk launches no idle passes.
-bool drain_chunk(khip_ctx* ctx, int64_t steps, int screen, bool& rerun) {
+bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& rerun) {
     GridState g;
     KHIP_TRY(ctx, hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     rerun = false;
+    ctx->last_rerun = 0;
     if (screen && g.passes_short) {
         ctx->passes = std::min(MAX_PASSES, std::max(g.passes_max, ctx->passes + 1));
-        rerun = true;
+        ctx->last_rerun |= KHIP_RERUN_PASSES;
     }
     if (static_cast<int64_t>(g.events) > ctx->ev_cap) {
         if (!ensure_events(ctx, 2 * static_cast<int64_t>(g.events) + 1024)) return false;
-        rerun = true;
+        ctx->last_rerun |= KHIP_RERUN_EVENTS;
     }
     if (screen && g.items_short) {
         if (!ensure_items(ctx, 2 * static_cast<int64_t>(g.items_max) + 4096)) return false;
-        rerun = true;
+        ctx->last_rerun |= KHIP_RERUN_ITEMS;
     }
+    if (screen && g.surv_short) {
+        if (!ensure_surv(ctx, 2 * static_cast<int64_t>(g.surv_max) + 4096)) return false;
+        ctx->last_rerun |= KHIP_RERUN_SURVIVORS;
+    }
+    rerun = ctx->last_rerun != 0;
     if (rerun) return true;
-    if (screen && g.passes_max > 0 && g.passes_max < ctx->passes) ctx->passes = g.passes_max;
+    // Fewer passes only after four chunks in a row needed fewer: a bounding
+    // box that hovers at a bit boundary must not cause repeated re-runs.
+    if (screen && g.passes_max > 0 && g.passes_max < ctx->passes) {
+        if (++ctx->shrink_votes >= 4) {
+            ctx->passes = g.passes_max;
+            ctx->shrink_votes = 0;
+        }
+    } else {
+        ctx->shrink_votes = 0;
+    }
     if (!screen) return true;
     const int64_t ne = static_cast<int64_t>(g.events);
-    std::vector<khip_event> ev(static_cast<size_t>(ne));
+    std::vector<khip_event>& raw = ctx->raw_events;
+    raw.resize(static_cast<size_t>(ne));
     if (ne > 0)
-        KHIP_TRY(ctx, hipMemcpy(ev.data(), ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
-    std::sort(ev.begin(), ev.end(), [](const khip_event& a, const khip_event& b) {
-        if (a.step != b.step) return a.step < b.step;
-        if (a.t != b.t) return a.t < b.t;
-        if (a.i != b.i) return a.i < b.i;
-        return a.j < b.j;
-    });
-    ctx->host_events.insert(ctx->host_events.end(), ev.begin(), ev.end());
+        KHIP_TRY(ctx, hipMemcpy(raw.data(), ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
+    // (step, t, i, j) order, as Screener::finish returns each step's events
+    // (collision.cpp:739-747): counting sort on the chunk's step index, then
+    // each step's events sorted on their own, steps in parallel.
+    std::vector<int64_t> start(static_cast<size_t>(steps) + 1, 0);
+    for (const khip_event& e : raw) ++start[static_cast<size_t>(e.step - step0) + 1];
+    for (int64_t k = 0; k < steps; ++k) start[k + 1] += start[k];
+    const size_t out0 = ctx->host_events.size();
+    ctx->host_events.resize(out0 + static_cast<size_t>(ne));
+    khip_event* out = ctx->host_events.data() + out0;
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (const khip_event& e : raw) out[fill[static_cast<size_t>(e.step - step0)]++] = e;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t k = 0; k < steps; ++k)
+        std::sort(out + start[k], out + start[k + 1], [](const khip_event& a, const khip_event& b) {
+            if (a.t != b.t) return a.t < b.t;
+            if (a.i != b.i) return a.i < b.i;
+            return a.j < b.j;
+        });
     std::vector<khip_stats> st(static_cast<size_t>(steps));
     KHIP_TRY(ctx, hipMemcpy(st.data(), ctx->stats, sizeof(khip_stats) * steps, hipMemcpyDeviceToHost));
     ctx->host_stats.insert(ctx->host_stats.end(), st.begin(), st.end());
@@ -1381,15 +1560,20 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
     if (n == 0) {
         for (int64_t k = 0; k < nsteps; ++k) {
             if (screen)
-                ctx->host_stats.push_back(khip_stats{0, 0, 0, std::max(1.0, ctx->cfg.threshold_km), 0, 0, 0});
+                ctx->host_stats.push_back(khip_stats{0, 0, 0, std::max(1.0, ctx->cfg.threshold_km), 0, 0, 0, 0});
             ctx->t += ctx->dt;
             ctx->step += 1;
         }
         return true;
     }
     int64_t done = 0;
+    // Chunks start at one step and double up to ctx->chunk: a buffer that
+    // must grow (events, work items, survivors, radix passes) costs the re-run
+    // of a short chunk only, usually right at the start.
+    int64_t next_len = 1;
     while (done < nsteps) {
-        const int64_t len = std::min<int64_t>(ctx->chunk, nsteps - done);
+        const int64_t len = std::min<int64_t>(std::min<int64_t>(next_len, ctx->chunk), nsteps - done);
+        next_len = std::min<int64_t>(2 * next_len, ctx->chunk);
         if (!ensure_stats(ctx, len) || !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents)) ||
             (ctx->timing && !ensure_event_pool(ctx, len)))
             return false;
@@ -1419,22 +1603,24 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                         return false;
                 } else {
                     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0,
-                                                 ctx->step_parts, 0, ctx->dense_parts, 0);
-                    mark(ctx, 3);
+                                                 ctx->dense_parts, 0, ctx->exact_parts, 0);
+                    mark(ctx, 5);
                 }
                 t += ctx->dt;
                 cur = nxt;
             }
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
-            if (!drain_chunk(ctx, len, screen, overflow)) return false;
+            if (!drain_chunk(ctx, ctx->step + done, len, screen, overflow)) return false;
             if (!overflow) {
                 if (ctx->timing) accumulate_timing(ctx, len, screen != 0);
                 ctx->cur = cur;
                 ctx->t = t;
                 break;
             }
-            // Event buffer overflowed: restore the chunk's start and re-run.
+            // A device buffer was too small: restore the chunk's start and re-run.
+            ctx->tm.reruns += 1;
+            ctx->tm.rerun_reasons |= ctx->last_rerun;
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], ctx->snap_state, sizeof(double) * 6 * c,
                                          hipMemcpyDeviceToDevice, s));
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, ctx->snap_alive, c, hipMemcpyDeviceToDevice, s));
@@ -1637,8 +1823,8 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
                                                   ctx->step_parts);
             ok = enqueue_screen(ctx, 0, 1, t0, 0, 0);
         } else {
-            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, ctx->step_parts, 0,
-                                         ctx->dense_parts, 0);
+            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, ctx->dense_parts, 0,
+                                         ctx->exact_parts, 0);
         }
         if (!ok) break;
         std::vector<khip_event> saved_ev;
@@ -1646,7 +1832,7 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
         saved_ev.swap(ctx->host_events);
         saved_st.swap(ctx->host_stats);
         bool overflow = false;
-        ok = drain_chunk(ctx, 1, 1, overflow);
+        ok = drain_chunk(ctx, 0, 1, 1, overflow);
         std::vector<khip_event> got_ev;
         std::vector<khip_stats> got_st;
         got_ev.swap(ctx->host_events);

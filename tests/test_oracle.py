@@ -250,3 +250,61 @@ def test_oracle_equals_compiled_reference_on_catalog(orc, ref, catalog):
     a = orc.run(*args)
     b = ref.run(*args)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+
+
+def _prefilter_fast_emulated(r, v, dt, thr, vmin2, rcp_err):
+    """float32 emulation of the device's prefilter_keep_fast, with the
+    reciprocal perturbed by rcp_err (relative) to cover v_rcp_f32's error."""
+    f = np.float32
+    r = r.astype(f)
+    v = v.astype(f)
+    dt, thr, vmin2 = f(dt), f(thr), f(vmin2)
+    G = f(2e-6)
+    dv2 = (v * v).sum(1, dtype=f)
+    rv = (r * v).sum(1, dtype=f)
+    r2 = (r * r).sum(1, dtype=f)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = (f(1) / dv2) * f(1 + rcp_err)
+        ts = np.where(dv2 > 0, -rv * inv, f(0)).astype(f)
+    ts = np.clip(ts, f(0), dt)
+    c = r + v * ts[:, None]
+    d2 = (c * c).sum(1, dtype=f)
+    dvn = np.abs(v).sum(1, dtype=f)
+    rn = np.abs(r).sum(1, dtype=f)
+    base = thr + f(1e-3) + f(5e-4) + f(5e-6) * dt
+    lim = base + G * dt * dt * dt * dvn + f(1e-5) * (rn + dvn * dt)
+    keep = d2 <= lim * lim
+    rv_tol = f(1e-5) * rn * dvn + f(5e-4) * dvn + f(5e-6) * rn
+    rej = (dv2 > f(1.1) * vmin2) & (rv > rv_tol) & (dv2 > f(1.01) * G * r2)
+    return keep & ~rej
+
+
+@pytest.mark.parametrize("rcp_err", [-2.0 ** -21, 0.0, 2.0 ** -21])
+def test_fast_prefilter_is_conservative(orc, rcp_err):
+    """The device's pre-filter (hardware reciprocal, L1-norm margins, squared
+    test) never drops a pair the exact FP64 filter keeps: 40k random and
+    boundary pairs, with the reciprocal error pushed to +-4 ulp."""
+    rng = np.random.default_rng(6)
+    n = 40000
+    r = rng.normal(size=(n, 3))
+    r *= rng.uniform(0, 9, n)[:, None] / np.linalg.norm(r, axis=1, keepdims=True)
+    v = rng.normal(size=(n, 3)) * rng.uniform(0, 12, n)[:, None]
+    # boundary cases: closest approach right at the 5 km gate, slow pairs,
+    # receding pairs at the convexity limit
+    k = n // 4
+    u = rng.normal(size=(k, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = rng.normal(size=(k, 3))
+    w -= (w * u).sum(1, keepdims=True) * u
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    speed = rng.uniform(0.001, 12, k)
+    t_star = rng.uniform(0, 10, k)
+    miss = 5.0 + 1e-3 + rng.uniform(-1e-3, 1e-3, k)
+    r[:k] = u * miss[:, None] - w * (speed * t_star)[:, None]
+    v[:k] = w * speed[:, None]
+    v[k:2 * k] = r[k:2 * k] * np.sqrt(2e-6) * rng.uniform(0.99, 1.01, k)[:, None]
+    keep = _prefilter_fast_emulated(r, v, 10.0, 5.0, 1e-4, rcp_err)
+    exact = np.array([orc.filter_pair(a, b, 10.0, 5.0, 1e-4) for a, b in zip(r, v)])
+    lost = (exact != 0) & ~keep
+    assert not lost.any(), np.flatnonzero(lost)[:10]
+    assert (exact != 0).sum() > 1000  # the sample exercises the boundary
