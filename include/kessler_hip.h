@@ -121,6 +121,11 @@ int64_t khip_object_count(const khip_ctx* ctx);
 /* Reads back states (n x 6) and alive flags; either pointer may be NULL. */
 int khip_get_objects(khip_ctx* ctx, double* states, uint8_t* alive);
 
+/* Start-of-step states (n x 6) of the last step run, for the objects listed
+ * in idx: the `before` states that Fragmentation::apply interpolates
+ * (breakup.cpp:326-331, main.cpp:352-358), without a per-step host copy. */
+int khip_get_previous(khip_ctx* ctx, int64_t n, const int32_t* idx, double* states);
+
 int khip_set_config(khip_ctx* ctx, const khip_screen_config* cfg);
 /* Epoch (Julian day), elapsed time t (s since epoch) and step dt (s).
  * Resets the global step counter to 0. */

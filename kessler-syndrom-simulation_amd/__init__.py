@@ -40,7 +40,7 @@ ABI_SYMBOLS = (
     "khip_set_objects", "khip_object_count", "khip_get_objects", "khip_set_config",
     "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
-    "khip_set_chunk", "khip_fp64_selfcheck",
+    "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous",
 )
 # Every symbol include/kessler_host.h declares.
 HOST_SYMBOLS = (
@@ -94,6 +94,7 @@ def library() -> C.CDLL:
         "khip_reset_timing": (_I, [_P]),
         "khip_set_chunk": (_I, [_P, _I]),
         "khip_fp64_selfcheck": (_I, [_P, _L, _P, _P, _P]),
+        "khip_get_previous": (_I, [_P, _L, _P, _P]),
         "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
         "khost_impacts": (_L, [_L, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P]),
         "khost_cascade": (_L, [_L, C.c_uint64, _I, _D, _D, _D, _D, _D, _L, _P]),
@@ -217,6 +218,14 @@ class Engine:
                                          _ptr(out), cap, C.byref(ne), _ptr(st)), "khip_screen")
         assert ne.value <= cap, "event buffer too small"
         return out[: ne.value].copy(), st[0]
+
+    def get_previous(self, idx):
+        """Start-of-step states of the last step run, for objects `idx`."""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((len(idx), 6))
+        self._check(self.lib.khip_get_previous(self.h, len(idx), _ptr(idx), _ptr(out)),
+                    "khip_get_previous")
+        return out
 
     def set_timing(self, on: bool):
         self._check(self.lib.khip_set_timing(self.h, 1 if on else 0), "khip_set_timing")

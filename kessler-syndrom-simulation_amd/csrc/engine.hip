@@ -1763,6 +1763,22 @@ int khip_get_objects(khip_ctx* ctx, double* states, uint8_t* alive) {
     return KHIP_OK;
 }
 
+int khip_get_previous(khip_ctx* ctx, int64_t n, const int32_t* idx, double* states) {
+    if (!ctx || n < 0 || (n > 0 && (!idx || !states))) return KHIP_ERR_ARG;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    const double* prev = ctx->state[ctx->cur ^ 1];
+    for (int64_t k = 0; k < n; ++k) {
+        if (idx[k] < 0 || idx[k] >= ctx->n) return KHIP_ERR_ARG;
+        for (int a = 0; a < 6; ++a)
+            if (hipMemcpy(states + 6 * k + a, prev + a * c + idx[k], sizeof(double), hipMemcpyDeviceToHost) !=
+                hipSuccess) {
+                ctx->error = "hipMemcpy of previous states failed";
+                return KHIP_ERR_HIP;
+            }
+    }
+    return KHIP_OK;
+}
+
 int khip_set_config(khip_ctx* ctx, const khip_screen_config* cfg) {
     if (!ctx || !cfg || !(cfg->threshold_km >= 0) || !(cfg->radius_scale >= 0) ||
         !(cfg->min_encounter_speed_km_s >= 0))
