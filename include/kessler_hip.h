@@ -158,6 +158,15 @@ int khip_set_timing(khip_ctx* ctx, int enabled);
 int khip_get_timing(const khip_ctx* ctx, khip_timing* out);
 int khip_reset_timing(khip_ctx* ctx);
 
+/* Propagation arithmetic. KHIP_PRECISION_FP64 (default) is the reference's
+ * FP64 RK4, bit for bit. KHIP_PRECISION_MIXED keeps the FP64 state and
+ * central term and evaluates J2 and the Moon in FP32 (the precision form);
+ * screening is unchanged (FP32 pre-filter, FP64 filter, TCA and radius
+ * test). No reference counterpart: the reference has one arithmetic. */
+#define KHIP_PRECISION_FP64 0
+#define KHIP_PRECISION_MIXED 1
+int khip_set_precision(khip_ctx* ctx, int mode);
+
 /* Steps per host synchronisation in khip_run (default 256). */
 int khip_set_chunk(khip_ctx* ctx, int steps);
 

@@ -40,8 +40,10 @@ ABI_SYMBOLS = (
     "khip_set_objects", "khip_object_count", "khip_get_objects", "khip_set_config",
     "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
-    "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous",
+    "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous", "khip_set_precision",
 )
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
 # Every symbol include/kessler_host.h declares.
 HOST_SYMBOLS = (
     "khost_generate", "khost_impacts", "khost_cascade", "khost_write_catalog",
@@ -95,6 +97,7 @@ def library() -> C.CDLL:
         "khip_set_chunk": (_I, [_P, _I]),
         "khip_fp64_selfcheck": (_I, [_P, _L, _P, _P, _P]),
         "khip_get_previous": (_I, [_P, _L, _P, _P]),
+        "khip_set_precision": (_I, [_P, _I]),
         "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
         "khost_impacts": (_L, [_L, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P]),
         "khost_cascade": (_L, [_L, C.c_uint64, _I, _D, _D, _D, _D, _D, _L, _P]),
@@ -182,6 +185,9 @@ class Engine:
         s = _L()
         self._check(self.lib.khip_get_time(self.h, C.byref(t), C.byref(s)), "khip_get_time")
         return t.value, s.value
+
+    def set_precision(self, mode: int):
+        self._check(self.lib.khip_set_precision(self.h, int(mode)), "khip_set_precision")
 
     def set_chunk(self, steps: int):
         self._check(self.lib.khip_set_chunk(self.h, int(steps)), "khip_set_chunk")

@@ -57,9 +57,55 @@ __device__ __forceinline__ D3 accel(D3 r, const MoonSample& mo) {
     return a;
 }
 
+// Mixed-precision acceleration (precision form): the central term -mu r/|r|^3
+// in FP64 (in FP32 it alone would cost km over a day), J2 and the Moon's
+// direct minus indirect term in FP32 with hardware reciprocal / rsqrt. The
+// perturbations are 1e-3 (J2) and 1e-7 (Moon) of the central term, so their
+// FP32 rounding is ~1e-10 of the acceleration.
+struct F3 {
+    float x, y, z;
+};
+
+// J2 + Moon (direct minus indirect) in FP32, from the FP64 position and |r|^2.
+__device__ __forceinline__ F3 perturbation_fp32(D3 r, double r2, const MoonSample& mo) {
+#pragma clang fp contract(fast)
+    const float xf = static_cast<float>(r.x), yf = static_cast<float>(r.y), zf = static_cast<float>(r.z);
+    const float r2f = static_cast<float>(r2);
+    const float inv_r2 = __builtin_amdgcn_rcpf(r2f);
+    const float inv_rn = __builtin_amdgcn_rsqf(r2f);
+    const float k = static_cast<float>(K_J2) * inv_r2 * inv_r2 * inv_rn;
+    const float z2 = 5.0f * zf * zf * inv_r2;
+    const float dx = static_cast<float>(mo.m[0]) - xf;
+    const float dy = static_cast<float>(mo.m[1]) - yf;
+    const float dz = static_cast<float>(mo.m[2]) - zf;
+    const float inv_d = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+    const float sm = static_cast<float>(MU_MOON) * inv_d * inv_d * inv_d;
+    return F3{k * xf * (z2 - 1.0f) + (dx * sm - static_cast<float>(mo.ind[0])),
+              k * yf * (z2 - 1.0f) + (dy * sm - static_cast<float>(mo.ind[1])),
+              k * zf * (z2 - 3.0f) + (dz * sm - static_cast<float>(mo.ind[2]))};
+}
+
+__device__ __forceinline__ D3 accel_mixed(D3 r, const MoonSample& mo) {
+    const double r2 = r.x * r.x + r.y * r.y + r.z * r.z;
+    const double rn = sqrt(r2);
+    const double c = -MU_EARTH / (r2 * rn);
+    const F3 p = perturbation_fp32(r, r2, mo);
+    return D3{r.x * c + static_cast<double>(p.x), r.y * c + static_cast<double>(p.y),
+              r.z * c + static_cast<double>(p.z)};
+}
+
+struct AccelFp64 {
+    __device__ __forceinline__ D3 operator()(D3 r, const MoonSample& m) const { return accel(r, m); }
+};
+struct AccelMixed {
+    __device__ __forceinline__ D3 operator()(D3 r, const MoonSample& m) const { return accel_mixed(r, m); }
+};
+
 // Classical RK4 (orbital.cpp:107-118) on one object; r, v updated in place.
+// Acc is the acceleration model (AccelFp64: the reference's, bit for bit).
+template <class Acc = AccelFp64>
 __device__ __forceinline__ void rk4(D3& r, D3& v, double dt, const MoonSample& m0,
-                                    const MoonSample& m1, const MoonSample& m2) {
+                                    const MoonSample& m1, const MoonSample& m2, Acc accel = Acc()) {
     const double h = dt / 2;
     const D3 k1v = accel(r, m0);
     const D3 k2r{v.x + k1v.x * h, v.y + k1v.y * h, v.z + k1v.z * h};

@@ -218,6 +218,7 @@ __device__ __forceinline__ StepPartial object_partial(bool on, bool was, double 
 // Propagation
 // ---------------------------------------------------------------------------
 
+template <class Acc>
 __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uint8_t* alive, int n,
                                                       const MoonSample* __restrict__ moon3,
                                                       double dt, StepPartial* partials) {
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
         was = a;
         if (a) {
             const MoonSample m0 = moon3[0], m1 = moon3[1], m2 = moon3[2];
-            rk4(r, v, dt, m0, m1, m2);
+            rk4(r, v, dt, m0, m1, m2, Acc());
             if (sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) {
                 a = false;
                 alive[i] = 0;
@@ -1222,6 +1223,7 @@ struct khip_ctx {
     int64_t step = 0;
     int chunk = 256;
     int passes = 4;   // radix passes launched per step (adapted per chunk)
+    int precision = KHIP_PRECISION_FP64;
     int shrink_votes = 0;  // consecutive chunks that needed fewer passes
     int64_t last_rerun = 0;  // reasons of the last re-run (KHIP_RERUN_* bits)
 
@@ -1594,9 +1596,14 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 const int nxt = cur ^ 1;
                 g_slot = static_cast<int>(k);
                 mark(ctx, 0);
-                k_propagate<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(cur), ctx->planes(nxt),
-                                                         ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
-                                                         ctx->step_parts);
+                if (ctx->precision == KHIP_PRECISION_MIXED)
+                    k_propagate<AccelMixed><<<grid_of(n), BLOCK, 0, s>>>(
+                        ctx->planes(cur), ctx->planes(nxt), ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
+                        ctx->step_parts);
+                else
+                    k_propagate<AccelFp64><<<grid_of(n), BLOCK, 0, s>>>(
+                        ctx->planes(cur), ctx->planes(nxt), ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
+                        ctx->step_parts);
                 mark(ctx, 1);
                 if (screen) {
                     if (!enqueue_screen(ctx, cur, nxt, t, ctx->step + done + k, static_cast<int>(k)))
@@ -1909,6 +1916,12 @@ int khip_get_timing(const khip_ctx* ctx, khip_timing* out) {
 int khip_reset_timing(khip_ctx* ctx) {
     if (!ctx) return KHIP_ERR_ARG;
     ctx->tm = khip_timing{};
+    return KHIP_OK;
+}
+
+int khip_set_precision(khip_ctx* ctx, int mode) {
+    if (!ctx || (mode != KHIP_PRECISION_FP64 && mode != KHIP_PRECISION_MIXED)) return KHIP_ERR_ARG;
+    ctx->precision = mode;
     return KHIP_OK;
 }
 
