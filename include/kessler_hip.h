@@ -165,7 +165,102 @@ int khip_reset_timing(khip_ctx* ctx);
  * test). No reference counterpart: the reference has one arithmetic. */
 #define KHIP_PRECISION_FP64 0
 #define KHIP_PRECISION_MIXED 1
+/* KHIP_PRECISION_LEARNED: steps of tau (khip_set_time's dt, which must equal
+ * the corrector's tau). Each object steps by one of three methods, chosen
+ * at check steps: one mixed RK4 step of tau plus the corrector (near-circular
+ * orbits below a_max_km), one mixed RK4 step of tau alone (near-circular
+ * orbits above a_max_km), or `substeps` FP64 RK4 steps of tau / substeps
+ * (the plain method; everything else). A check step (every check_every
+ * steps, and the first step after khip_set_objects / khip_set_precision /
+ * khip_set_learned) runs the plain method for every object and keeps the
+ * cheaper method for an object only if it stayed within tol_km of the plain
+ * result. Screening covers each sub-step of tau / substeps on the cubic
+ * Hermite dense output of the step, with a grid on midpoints and the
+ * threshold raised by margin_km; every event found is then solved again on
+ * plain FP64 trajectories from the step's start (filter_pair, Hermite +
+ * golden section, radius test) and kept with the plain values. Event step
+ * indices count sub-steps, as a plain run at dt = tau / substeps does. */
+#define KHIP_PRECISION_LEARNED 2
 int khip_set_precision(khip_ctx* ctx, int mode);
+
+typedef struct {
+    int32_t substeps;    /* plain steps per learned step (divides 12; default 6) */
+    int32_t check_every; /* learned steps between checks (default 10) */
+    double tol_km;       /* deviation from the plain step tolerated at a check (default 1e-5) */
+    double margin_km;    /* screening threshold raise (default 0.025) */
+    double e_max;        /* corrector / base-step domain: eccentricity below (default 0.1) */
+    double a_max_km;     /* corrector domain: semi-major axis below (default 12000) */
+} khip_learned_config;
+
+typedef struct {
+    int64_t learned, base, plain; /* objects per method after the last check */
+    int64_t rejected;             /* domain candidates sent to the plain method, all checks */
+    double dev_max_km;            /* largest deviation accepted at a check */
+    int64_t checks, steps;
+    int64_t events_verified;      /* events confirmed by the plain solve */
+    int64_t events_dropped;       /* events the plain solve rejected (not reported) */
+    int64_t events_unverified;    /* plain minimum outside the learned step (learned values kept) */
+} khip_learned_report;
+
+int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);
+/* Tallies since the last khip_set_objects. No reference counterpart. */
+int khip_get_learned_report(const khip_ctx* ctx, khip_learned_report* out);
+
+/* ---- Learned step corrector (no reference counterpart) ----------------
+ *
+ * A 10-128-128-128-6 perceptron (SiLU, BF16 weights on the matrix cores,
+ * FP32 accumulation) that predicts, from ten inputs describing the state in
+ * its local orbital frame, the difference between six FP64 RK4 steps of
+ * tau/6 and one mixed-precision RK4 step of tau. Trained on the device by
+ * the library's own forward and backward kernels (Adam on FP32 master
+ * parameters). Parameter layout: W1[128][10], b1[128], W2[128][128],
+ * b2[128], W3[128][128], b3[128], W4[6][128], b4[6] (row-major [out][in]). */
+#define KHIP_NET_PARAMS 35206
+#define KHIP_NET_INPUTS 10
+#define KHIP_NET_OUTPUTS 6
+
+/* Standardisation: input k = (feature_k - fmu[k]) / fsig[k]; frame
+ * component k of the correction = (output_k * osig[k] + omu[k]) * scale. */
+typedef struct {
+    float fmu[10], fsig[10];
+    float omu[6], osig[6];
+    double tau; /* step (s) the corrector is trained for */
+} khip_net_norm;
+
+typedef struct {
+    int32_t samples_per_orbit; /* samples emitted along each orbit */
+    int32_t spacing_steps;     /* tau-steps between two samples of one orbit */
+    int32_t iterations;        /* Adam steps */
+    int32_t batch;             /* columns per step, a multiple of 1024 */
+    double lr_max;             /* peak learning rate (5% warm-up, cosine decay) */
+    double e_max;              /* learned domain: osculating eccentricity below e_max, */
+    double a_max_km;           /* semi-major axis below a_max_km */
+    double tau;                /* step the corrector is trained for (s) */
+    uint64_t seed;             /* parameter initialisation and batch sampling */
+} khip_train_config;
+
+typedef struct {
+    int64_t samples;   /* valid training samples */
+    double loss_first; /* mean loss of the first 100 iterations */
+    double loss_last;  /* mean loss of the last 100 iterations */
+    double gen_ms;     /* device time of the sample generation */
+    double train_ms;   /* device time of the training loop */
+} khip_train_report;
+
+/* Generates the training set on the device from n seeded orbits (n x 6
+ * states at the context's epoch and time), then trains the corrector.
+ * curve (may be NULL) receives the mean loss of every iteration. */
+int khip_net_train(khip_ctx* ctx, int64_t n, const double* orbits, const khip_train_config* cfg,
+                   khip_train_report* report, float* curve);
+/* Loads / reads the parameters (KHIP_NET_PARAMS floats) and standardisation. */
+int khip_net_set(khip_ctx* ctx, const float* params, const khip_net_norm* norm);
+int khip_net_get(khip_ctx* ctx, float* params, khip_net_norm* norm);
+/* Network outputs (n x 6) for standardised inputs (n x 10). */
+int khip_net_infer(khip_ctx* ctx, int64_t n, const float* x, float* out);
+/* Gradient of the training loss mean_b (1/6) sum_k (out_bk - y_bk)^2 over
+ * the batch (n a multiple of 1024) with respect to the parameters, and the
+ * loss itself, as the training step computes them. */
+int khip_net_gradient(khip_ctx* ctx, int64_t n, const float* x, const float* y, float* grad, float* loss);
 
 /* Steps per host synchronisation in khip_run (default 256). */
 int khip_set_chunk(khip_ctx* ctx, int steps);

@@ -41,7 +41,13 @@ ABI_SYMBOLS = (
     "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
     "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous", "khip_set_precision",
+    "khip_net_train", "khip_net_set", "khip_net_get", "khip_net_infer", "khip_net_gradient",
+    "khip_set_learned", "khip_get_learned_report",
 )
+PRECISION_LEARNED = 2
+NET_PARAMS = 35206
+NET_INPUTS = 10
+NET_OUTPUTS = 6
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 # Every symbol include/kessler_host.h declares.
@@ -54,6 +60,34 @@ _P = C.c_void_p
 _I = C.c_int
 _L = C.c_int64
 _D = C.c_double
+
+
+class NetNorm(C.Structure):
+    """khip_net_norm: input / output standardisation and the trained step."""
+    _fields_ = [("fmu", C.c_float * 10), ("fsig", C.c_float * 10), ("omu", C.c_float * 6),
+                ("osig", C.c_float * 6), ("tau", _D)]
+
+
+class TrainConfig(C.Structure):
+    _fields_ = [("samples_per_orbit", C.c_int32), ("spacing_steps", C.c_int32),
+                ("iterations", C.c_int32), ("batch", C.c_int32), ("lr_max", _D), ("e_max", _D),
+                ("a_max_km", _D), ("tau", _D), ("seed", C.c_uint64)]
+
+
+class LearnedConfig(C.Structure):
+    _fields_ = [("substeps", C.c_int32), ("check_every", C.c_int32), ("tol_km", _D), ("margin_km", _D),
+                ("e_max", _D), ("a_max_km", _D)]
+
+
+LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", "<i8"), ("rejected", "<i8"),
+                                 ("dev_max_km", "<f8"), ("checks", "<i8"), ("steps", "<i8"),
+                                 ("events_verified", "<i8"), ("events_dropped", "<i8"),
+                                 ("events_unverified", "<i8")])
+
+
+class TrainReport(C.Structure):
+    _fields_ = [("samples", C.c_int64), ("loss_first", _D), ("loss_last", _D), ("gen_ms", _D),
+                ("train_ms", _D)]
 
 
 class ScreenConfig(C.Structure):
@@ -98,6 +132,13 @@ def library() -> C.CDLL:
         "khip_fp64_selfcheck": (_I, [_P, _L, _P, _P, _P]),
         "khip_get_previous": (_I, [_P, _L, _P, _P]),
         "khip_set_precision": (_I, [_P, _I]),
+        "khip_net_train": (_I, [_P, _L, _P, C.POINTER(TrainConfig), C.POINTER(TrainReport), _P]),
+        "khip_net_set": (_I, [_P, _P, C.POINTER(NetNorm)]),
+        "khip_net_get": (_I, [_P, _P, C.POINTER(NetNorm)]),
+        "khip_net_infer": (_I, [_P, _L, _P, _P]),
+        "khip_net_gradient": (_I, [_P, _L, _P, _P, _P, _P]),
+        "khip_set_learned": (_I, [_P, C.POINTER(LearnedConfig)]),
+        "khip_get_learned_report": (_I, [_P, _P]),
         "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
         "khost_impacts": (_L, [_L, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P]),
         "khost_cascade": (_L, [_L, C.c_uint64, _I, _D, _D, _D, _D, _D, _L, _P]),
@@ -243,6 +284,63 @@ class Engine:
 
     def reset_timing(self):
         self._check(self.lib.khip_reset_timing(self.h), "khip_reset_timing")
+
+    # ---- learned corrector -------------------------------------------------
+
+    def net_train(self, orbits, samples_per_orbit=10, spacing_steps=12, iterations=20000,
+                  batch=16384, lr_max=3e-3, e_max=0.1, a_max_km=12000.0, tau=60.0, seed=1):
+        """Generates the training set from `orbits` (n x 6) on the device and
+        trains the corrector there. Returns (report dict, loss curve)."""
+        orbits = np.ascontiguousarray(orbits, dtype=np.float64)
+        cfg = TrainConfig(samples_per_orbit, spacing_steps, iterations, batch, lr_max, e_max, a_max_km, tau,
+                          seed)
+        rep = TrainReport()
+        curve = np.zeros(iterations, dtype=np.float32)
+        self._check(self.lib.khip_net_train(self.h, orbits.shape[0], _ptr(orbits), C.byref(cfg),
+                                            C.byref(rep), _ptr(curve)), "khip_net_train")
+        return {k: getattr(rep, k) for k, _ in TrainReport._fields_}, curve
+
+    def net_set(self, params, norm: dict):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.shape == (NET_PARAMS,)
+        nn = NetNorm()
+        for k in ("fmu", "fsig", "omu", "osig"):
+            getattr(nn, k)[:] = [float(v) for v in norm[k]]
+        nn.tau = float(norm["tau"])
+        self._check(self.lib.khip_net_set(self.h, _ptr(params), C.byref(nn)), "khip_net_set")
+
+    def net_get(self):
+        params = np.zeros(NET_PARAMS, dtype=np.float32)
+        nn = NetNorm()
+        self._check(self.lib.khip_net_get(self.h, _ptr(params), C.byref(nn)), "khip_net_get")
+        norm = {k: np.array(getattr(nn, k)[:], dtype=np.float32) for k in ("fmu", "fsig", "omu", "osig")}
+        norm["tau"] = nn.tau
+        return params, norm
+
+    def net_infer(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros((x.shape[0], NET_OUTPUTS), dtype=np.float32)
+        self._check(self.lib.khip_net_infer(self.h, x.shape[0], _ptr(x), _ptr(out)), "khip_net_infer")
+        return out
+
+    def net_gradient(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        grad = np.zeros(NET_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self.lib.khip_net_gradient(self.h, x.shape[0], _ptr(x), _ptr(y), _ptr(grad),
+                                               C.byref(loss)), "khip_net_gradient")
+        return grad, loss.value
+
+    def set_learned(self, substeps=6, check_every=10, tol_km=1e-5, margin_km=0.025, e_max=0.1,
+                    a_max_km=12000.0):
+        cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km)
+        self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")
+
+    def learned_report(self):
+        out = np.zeros(1, dtype=LEARNED_REPORT_DTYPE)
+        self._check(self.lib.khip_get_learned_report(self.h, _ptr(out)), "khip_get_learned_report")
+        return {k: out[0][k].item() for k in LEARNED_REPORT_DTYPE.names}
 
     def fp64_selfcheck(self, a, b):
         a = np.ascontiguousarray(a, dtype=np.float64)

@@ -49,6 +49,7 @@
 #include "../../include/kessler_hip.h"
 #include "../host/orbital.hpp"
 #include "device_common.hpp"
+#include "mlp.hpp"
 
 using namespace kessler;
 using namespace kessler::dev;
@@ -104,6 +105,7 @@ struct GridState {
     unsigned int max_cell;
     unsigned int pad1;
     unsigned long long events;      // events appended since the last drain
+    unsigned long long ev_mark;     // events before the current learned step
 };
 
 struct DevEvent {
@@ -266,6 +268,252 @@ __global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* ali
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// ---------------------------------------------------------------------------
+// Learned form: steps of tau with a per-object method
+// ---------------------------------------------------------------------------
+//
+// Every object steps by one of three methods, chosen at check steps:
+//   CLASS_LEARNED  one mixed-precision RK4 step of tau + the corrector;
+//   CLASS_BASE     one mixed-precision RK4 step of tau (MEO, GEO);
+//   CLASS_PLAIN    `substeps` FP64 RK4 steps of tau / substeps (the plain
+//                  method; the fallback).
+// A check step runs the plain method for every object (the state it writes
+// is the plain result) and, for objects whose orbit lies in the learned or
+// base domain, also that method; an object keeps the cheaper method until
+// the next check only if it stayed within tol_km of the plain result.
+// Between checks each method runs on its own list of objects, so waves
+// are uniform.
+
+struct LearnedParams {
+    double tau;
+    int substeps;
+    int pad;
+    double e_max, a_max, tol_km;
+};
+
+struct CheckStats {
+    unsigned int count[3];        // objects per method after the check
+    unsigned int rejected;        // domain candidates sent to the plain method
+    unsigned long long dev_max;   // largest accepted deviation (km), double bits
+};
+
+// One object through `substeps` FP64 RK4 steps; moon13 holds the Moon every
+// tau / 12 over the step. Stops at the sub-step whose end is under the
+// surface (the object dies there, as a plain step of tau / substeps would).
+__device__ __forceinline__ bool plain_substeps(D3& r, D3& v, const MoonSample* m, double h, int sub) {
+    const int stride = 12 / sub;
+    for (int k = 0; k < sub; ++k) {
+        rk4(r, v, h, m[stride * k], m[stride * k + stride / 2], m[stride * (k + 1)]);
+        if (sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void store_state(Planes p, int i, D3 r, D3 v) {
+    p.x[i] = r.x; p.y[i] = r.y; p.z[i] = r.z;
+    p.vx[i] = v.x; p.vy[i] = v.y; p.vz[i] = v.z;
+}
+
+// Corrector on the 64 objects of a wave (lane = object). x: the lane's
+// standardised inputs; returns the lane's six outputs.
+__device__ __forceinline__ void net_wave(const float x[net::NIN], const uint16_t* frags, const float* P,
+                                         float o[net::NOUT]) {
+    const int lane = static_cast<int>(__lane_id());
+    const int j = lane & 15, g = lane >> 4;
+    net::u32x4 xin[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        float xt[net::NIN];
+#pragma unroll
+        for (int k = 0; k < net::NIN; ++k) xt[k] = __shfl(x[k], 16 * t + j, WAVE);
+        xin[t] = net::input_operand(xt, g);
+    }
+    net::f32x4 out[4];
+    net::forward_tiles<4>(xin, frags, P, lane, out);
+    // Output 4g' + e of column j of tile t sits in lane j + 16 g', element e.
+    const int mt = lane >> 4, mj = lane & 15;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float a0 = __shfl(out[t].x, mj, WAVE), a1 = __shfl(out[t].y, mj, WAVE);
+        const float a2 = __shfl(out[t].z, mj, WAVE), a3 = __shfl(out[t].w, mj, WAVE);
+        const float a4 = __shfl(out[t].x, mj + 16, WAVE), a5 = __shfl(out[t].y, mj + 16, WAVE);
+        if (mt == t) {
+            o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
+        }
+    }
+}
+
+// KIND: CLASS_LEARNED / CLASS_BASE / CLASS_PLAIN list kernels, or KIND_CHECK
+// over every object. Persistent grid, waves stride over tiles of 64.
+constexpr int KIND_CHECK = 3;
+
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8_t* alive, uint8_t* mode,
+                                                    const int* __restrict__ list, const int* __restrict__ counts,
+                                                    int n, const MoonSample* __restrict__ moon13,
+                                                    const uint16_t* __restrict__ ffwd, const float* __restrict__ P,
+                                                    net::NetNorm nn, LearnedParams lp, CheckStats* cs) {
+    constexpr bool USES_NET = KIND == net::CLASS_LEARNED || KIND == KIND_CHECK;
+    __shared__ uint4 s_frag[USES_NET ? net::NFRAG_FWD * net::FRAG_HALVES / 8 : 1];
+    if (USES_NET) {
+        const uint4* src = reinterpret_cast<const uint4*>(ffwd);
+        for (int k = threadIdx.x; k < net::NFRAG_FWD * net::FRAG_HALVES / 8; k += BLOCK) s_frag[k] = src[k];
+        __syncthreads();
+    }
+    const uint16_t* frags = reinterpret_cast<const uint16_t*>(s_frag);
+    const int count = KIND == KIND_CHECK ? n : counts[KIND];
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
+    const double h = lp.tau / lp.substeps;
+    unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
+    double dev_max = 0.0;
+    for (int tile = blockIdx.x * WAVES_PER_BLOCK + wave; tile * WAVE < count; tile += gridDim.x * WAVES_PER_BLOCK) {
+        const int k = tile * WAVE + lane;
+        const bool active = k < count;
+        const int i = active ? (KIND == KIND_CHECK ? k : list[k]) : 0;
+        bool on = false;
+        D3 r{0.0, 0.0, 0.0}, v{0.0, 0.0, 0.0};
+        if (active) {
+            r = D3{cur.x[i], cur.y[i], cur.z[i]};
+            v = D3{cur.vx[i], cur.vy[i], cur.vz[i]};
+            on = alive[i] != 0;
+            if (!on) store_state(nxt, i, r, v);  // frozen
+        }
+        int cls = KIND;
+        if (KIND == KIND_CHECK) cls = on ? net::domain_class(r, v, lp.e_max, lp.a_max) : net::CLASS_PLAIN;
+        // Base step and corrector (all lanes of a wave take part in the MFMA).
+        D3 rb = r, vb = v;
+        const bool want_base = on && (cls == net::CLASS_LEARNED || cls == net::CLASS_BASE);
+        if (want_base) rk4(rb, vb, lp.tau, moon13[0], moon13[6], moon13[12], AccelMixed());
+        if (USES_NET) {
+            const bool want_net = on && cls == net::CLASS_LEARNED;
+            if (__ballot(want_net) != 0ULL) {
+                float x[net::NIN];
+                net::Frame fr;
+                if (want_net) {
+                    net::features(r, v, moon13[0], nn, x, fr);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < net::NIN; ++q) x[q] = 0.0f;
+                }
+                float o[net::NOUT];
+                net_wave(x, frags, P, o);
+                if (want_net) net::apply_correction(rb, vb, o, nn, fr);
+            }
+        }
+        if (!on) continue;
+        if (KIND == net::CLASS_LEARNED || KIND == net::CLASS_BASE) {
+            store_state(nxt, i, rb, vb);
+            if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) alive[i] = 0;
+        } else {
+            D3 rp = r, vp = v;
+            const bool survives = plain_substeps(rp, vp, moon13, h, lp.substeps);
+            store_state(nxt, i, rp, vp);
+            if (!survives) alive[i] = 0;
+            if (KIND == KIND_CHECK) {
+                int m = net::CLASS_PLAIN;
+                if (survives && cls != net::CLASS_PLAIN) {
+                    const double dx = rb.x - rp.x, dy = rb.y - rp.y, dz = rb.z - rp.z;
+                    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                    if (d <= lp.tol_km) {
+                        m = cls;
+                        dev_max = d > dev_max ? d : dev_max;
+                    } else {
+                        ++rejected;
+                    }
+                }
+                mode[i] = static_cast<uint8_t>(m);
+                if (survives) ++cnt[m];
+            }
+        }
+    }
+    if (KIND == KIND_CHECK) {
+        for (int q = 0; q < 3; ++q) {
+            const unsigned int s = wave_sum(cnt[q]);
+            if (lane == 0 && s) atomicAdd(&cs->count[q], s);
+        }
+        const unsigned int rj = wave_sum(rejected);
+        if (lane == 0 && rj) atomicAdd(&cs->rejected, rj);
+        const unsigned long long dm =
+            wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(dev_max)));
+        if (lane == 0) atomicMax(&cs->dev_max, dm);
+    }
+}
+
+// Per-method index lists of the objects alive after a check.
+__global__ void __launch_bounds__(BLOCK) k_build_lists(const uint8_t* alive, const uint8_t* mode, int n,
+                                                        int* lists, int cap, int* counts) {
+    __shared__ int s_base[3];
+    __shared__ int s_wave[3][WAVES_PER_BLOCK];
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    const int m = i < n && alive[i] ? mode[i] : -1;
+    const int wave = threadIdx.x / WAVE;
+    const unsigned long long below = lanes_below();
+    int rank[3], tot[3];
+    for (int q = 0; q < 3; ++q) {
+        const unsigned long long b = __ballot(m == q);
+        rank[q] = __popcll(b & below);
+        if (__lane_id() == 0) s_wave[q][wave] = __popcll(b);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 3; ++q) {
+            int t = 0;
+            for (int w = 0; w < WAVES_PER_BLOCK; ++w) t += s_wave[q][w];
+            s_base[q] = t ? atomicAdd(&counts[q], t) : 0;
+        }
+    }
+    __syncthreads();
+    if (m >= 0) {
+        int off = s_base[m] + rank[m];
+        for (int w = 0; w < wave; ++w) off += s_wave[m][w];
+        lists[m * cap + off] = i;
+    }
+    (void)tot;
+}
+
+// Dense output: states at fraction s of the step from the cubic Hermite of
+// each object's start and end states (positions and velocities).
+__global__ void __launch_bounds__(BLOCK) k_dense_output(Planes a, Planes b, int n, double dt, double s,
+                                                         Planes out) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const double s2 = s * s, s3 = s2 * s;
+    const double c0 = 2 * s3 - 3 * s2 + 1, c1 = s3 - 2 * s2 + s, c2 = -2 * s3 + 3 * s2, c3 = s3 - s2;
+    const double d0 = 6 * s2 - 6 * s, d1 = 3 * s2 - 4 * s + 1, d2 = -6 * s2 + 6 * s, d3 = 3 * s2 - 2 * s;
+    const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
+    const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
+    double p[3], w[3];
+    for (int k = 0; k < 3; ++k) {
+        p[k] = x0[k] * c0 + v0[k] * dt * c1 + x1[k] * c2 + v1[k] * dt * c3;
+        w[k] = (x0[k] * d0 + v0[k] * dt * d1 + x1[k] * d2 + v1[k] * dt * d3) / dt;
+    }
+    out.x[i] = p[0]; out.y[i] = p[1]; out.z[i] = p[2];
+    out.vx[i] = w[0]; out.vy[i] = w[1]; out.vz[i] = w[2];
+}
+
+// Reductions for a midpoint grid: max of |v| over both ends, bounding box
+// of the midpoints, of the objects alive at the end of the step.
+__global__ void __launch_bounds__(BLOCK) k_reduce_mid(Planes a, Planes b, const uint8_t* alive, int n,
+                                                       StepPartial* partials) {
+    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    bool on = false;
+    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+    if (i < n && alive[i]) {
+        on = true;
+        rx = 0.5 * (a.x[i] + b.x[i]);
+        ry = 0.5 * (a.y[i] + b.y[i]);
+        rz = 0.5 * (a.z[i] + b.z[i]);
+        const double va = sqrt(a.vx[i] * a.vx[i] + a.vy[i] * a.vy[i] + a.vz[i] * a.vz[i]);
+        const double vb = sqrt(b.vx[i] * b.vx[i] + b.vy[i] * b.vy[i] + b.vz[i] * b.vz[i]);
+        vn = va > vb ? va : vb;
+    }
+    StepPartial p = object_partial(on, on, vn, rx, ry, rz);
+    const StepPartial t = block_reduce(p, s_rows);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 __global__ void k_reset(GridState* gs) {
     if (threadIdx.x != 0) return;
     gs->passes_max = 0;
@@ -302,9 +550,14 @@ __device__ __forceinline__ int bits_for(long long range) {
     return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
 }
 
+// midpoint != 0: the grid is built on the midpoint of each object's start and
+// end positions (learned form); a pair that comes within thr during the step
+// then has midpoints closer than vmax dt + thr + G vmax dt^3 (each object is
+// within vmax dt / 2 of its midpoint over the step), so cells of that size
+// and the 13 forward neighbours enumerate every such pair.
 __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
-                                                       int launched_passes) {
+                                                       int launched_passes, int midpoint) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     StepPartial acc = empty_step_partial();
     for (int b = threadIdx.x; b < nparts; b += BLOCK) combine(acc, partials[b]);
@@ -318,7 +571,7 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
     gs->alive = tot.alive;
     gs->propagated = tot.propagated;
     const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
-    const double vrel_max = 2.0 * vmax;
+    const double vrel_max = midpoint ? vmax : 2.0 * vmax;
     const double raw = vrel_max * dt + thr + GRAVITY_GRADIENT * vrel_max * dt * dt * dt;
     double cell = raw > 1.0 ? raw : 1.0;
     gs->cell_ref = cell;
@@ -366,8 +619,8 @@ __device__ __forceinline__ uint64_t pack_key(const GridState& g, double x, doubl
            (static_cast<uint64_t>(iy) << g.bits[2]) | static_cast<uint64_t>(iz);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, const uint8_t* alive, int n,
-                                                 const GridState* gs, uint64_t* keys, int* idx,
+__global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
+                                                 int n, const GridState* gs, uint64_t* keys, int* idx,
                                                  int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
@@ -379,7 +632,13 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, const uint8_t* alive
         const int e = base + r * BLOCK + threadIdx.x;
         if (e < n) {
             uint64_t k = DEAD_KEY;
-            if (alive[e]) k = pack_key(g, cur.x[e], cur.y[e], cur.z[e]);
+            if (alive[e]) {
+                if (midpoint)
+                    k = pack_key(g, 0.5 * (cur.x[e] + end.x[e]), 0.5 * (cur.y[e] + end.y[e]),
+                                 0.5 * (cur.z[e] + end.z[e]));
+                else
+                    k = pack_key(g, cur.x[e], cur.y[e], cur.z[e]);
+            }
             keys[e] = k;
             idx[e] = e;
             atomicAdd(&s_hist[static_cast<int>(k & 255u)], 1);
@@ -1086,6 +1345,121 @@ __global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, Pa
     store_counters(parts, cnt, s_cnt);
 }
 
+// ---------------------------------------------------------------------------
+// Learned form: plain FP64 re-verification of every reported event
+// ---------------------------------------------------------------------------
+//
+// An event the learned form reports in sub-step k of a learned step is
+// solved again on plain trajectories: both objects are stepped from the
+// learned step's start state by FP64 RK4 sub-steps (the plain method) to the
+// start and end of sub-step k, and the reference's exact test (filter_pair,
+// Hermite + golden section, radius test; collision.cpp:109-132) runs on that
+// pair of states. A minimum on a sub-step edge is looked for in the
+// neighbouring sub-step of the same learned step. The event's TCA, miss
+// distance, speed and collision flag become the plain solve's; an event the
+// plain solve rejects is dropped (pad = 1); one whose plain minimum leaves
+// the learned step keeps its learned values (pad = 2).
+
+struct ReverifyStats {
+    unsigned long long verified, dropped, kept_unverified;
+};
+
+// 0: event (s, miss, vrel); 1: rejected; 2: minimum at s ~ 0; 3: at s ~ 1.
+__device__ __forceinline__ int plain_refine(D3 ri, D3 vi, D3 rj, D3 vj, D3 ai, D3 wi, D3 aj, D3 wj, double dt,
+                                            double thr, double vmin2, double& s, double& miss, double& vrel) {
+    const int res = filter_pair(rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, dt,
+                                thr, vmin2);
+    if (res != FILTER_REFINE) return 1;
+    const Hermite h{{rj.x - ri.x, rj.y - ri.y, rj.z - ri.z},
+                    {(vj.x - vi.x) * dt, (vj.y - vi.y) * dt, (vj.z - vi.z) * dt},
+                    {aj.x - ai.x, aj.y - ai.y, aj.z - ai.z},
+                    {(wj.x - wi.x) * dt, (wj.y - wi.y) * dt, (wj.z - wi.z) * dt}};
+    s = golden_min(h);
+    if (s <= 1e-9) return 2;
+    if (s >= 1.0 - 1e-9) return 3;
+    const D3 p = h.pos(s);
+    miss = sqrt(p.x * p.x + p.y * p.y + p.z * p.z);
+    if (miss > thr) return 1;
+    const D3 w = h.vel(s);
+    vrel = sqrt(w.x * w.x + w.y * w.y + w.z * w.z) / dt;
+    return 0;
+}
+
+// States of one object at the start of sub-steps k and k + 1.
+__device__ __forceinline__ void plain_at(Planes s0, int i, const MoonSample* m, double h, int sub, int k, D3& ra,
+                                         D3& va, D3& rb, D3& vb) {
+    const int stride = 12 / sub;
+    D3 r{s0.x[i], s0.y[i], s0.z[i]}, v{s0.vx[i], s0.vy[i], s0.vz[i]};
+    for (int q = 0; q < k; ++q) rk4(r, v, h, m[stride * q], m[stride * q + stride / 2], m[stride * (q + 1)]);
+    ra = r;
+    va = v;
+    rk4(r, v, h, m[stride * k], m[stride * k + stride / 2], m[stride * (k + 1)]);
+    rb = r;
+    vb = v;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* events, unsigned long long cap,
+                                                     Planes s0, const double* radius,
+                                                     const MoonSample* __restrict__ moon13, double t_step,
+                                                     long long sub0, double h, int sub, double thr, double vmin2,
+                                                     ReverifyStats* rs) {
+    const unsigned long long e0 = gs->ev_mark;
+    const unsigned long long e1 = gs->events < cap ? gs->events : cap;
+    unsigned long long ver = 0, drop = 0, kept = 0;
+    for (unsigned long long e = e0 + static_cast<unsigned long long>(blockIdx.x) * BLOCK + threadIdx.x; e < e1;
+         e += static_cast<unsigned long long>(gridDim.x) * BLOCK) {
+        DevEvent ev = events[e];
+        const int i = ev.i, j = ev.j;
+        int k = static_cast<int>(ev.step - sub0);
+        int code = 1, tries = 0, dir = 0;
+        double s = 0.0, miss = 0.0, vrel = 0.0;
+        while (tries < 2 && k >= 0 && k < sub) {
+            D3 rai, vai, rbi, vbi, raj, vaj, rbj, vbj;
+            plain_at(s0, i, moon13, h, sub, k, rai, vai, rbi, vbi);
+            plain_at(s0, j, moon13, h, sub, k, raj, vaj, rbj, vbj);
+            code = plain_refine(rai, vai, raj, vaj, rbi, vbi, rbj, vbj, h, thr, vmin2, s, miss, vrel);
+            ++tries;
+            if (code == 2 && dir <= 0) {
+                dir = -1;
+                --k;
+            } else if (code == 3 && dir >= 0) {
+                dir = 1;
+                ++k;
+            } else {
+                break;
+            }
+        }
+        if (code == 0) {
+            ev.step = sub0 + k;
+            ev.t = t_step + static_cast<double>(k) * h + s * h;
+            ev.miss_km = miss;
+            ev.v_rel_km_s = vrel;
+            ev.collision = miss < radius[i] + radius[j] ? 1 : 0;
+            ev.pad = 0;
+            ++ver;
+        } else if ((code == 2 || code == 3) && (k < 0 || k >= sub)) {
+            ev.pad = 2;  // the plain minimum lies in the neighbouring learned step
+            ++kept;
+        } else {
+            ev.pad = 1;
+            ++drop;
+        }
+        events[e] = ev;
+    }
+    ver = wave_sum(ver);
+    drop = wave_sum(drop);
+    kept = wave_sum(kept);
+    if (__lane_id() == 0) {
+        if (ver) atomicAdd(&rs->verified, ver);
+        if (drop) atomicAdd(&rs->dropped, drop);
+        if (kept) atomicAdd(&rs->kept_unverified, kept);
+    }
+}
+
+__global__ void k_mark_events(GridState* gs, unsigned long long cap) {
+    if (threadIdx.x == 0) gs->ev_mark = gs->events < cap ? gs->events : cap;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
                                                    const PairPartial* parts, int nparts,
                                                    const PairPartial* dparts, int ndparts,
@@ -1224,6 +1598,33 @@ struct khip_ctx {
     int chunk = 256;
     int passes = 4;   // radix passes launched per step (adapted per chunk)
     int precision = KHIP_PRECISION_FP64;
+
+    // Learned corrector: FP32 master parameters, Adam moments, BF16
+    // fragments (forward and backward order), standardisation.
+    float* net_p = nullptr;
+    float* net_m1 = nullptr;
+    float* net_m2 = nullptr;
+    uint16_t* net_ffwd = nullptr;
+    uint16_t* net_fbwd = nullptr;
+    net::NetNorm net_norm{};
+    bool net_ready = false;
+
+    // Learned form: per-object method, per-method lists, dense-output
+    // states of the sub-step boundaries, check and re-verification tallies.
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0};
+    int64_t lrn_cap = 0;
+    uint8_t* mode = nullptr;
+    uint8_t* snap_mode = nullptr;
+    int* lists = nullptr;        // 3 * cap: learned, plain, base
+    int* snap_lists = nullptr;
+    int* counts = nullptr;       // 3 list sizes
+    CheckStats* cstats = nullptr;
+    ReverifyStats* rstats = nullptr;
+    double* substates = nullptr; // (substeps - 1) * 6 * cap: dense-output states
+    int dense_k = 0;
+    bool force_check = true;
+    khip_learned_report lrep{};
+    int n_cus = 256;
     int shrink_votes = 0;  // consecutive chunks that needed fewer passes
     int64_t last_rerun = 0;  // reasons of the last re-run (KHIP_RERUN_* bits)
 
@@ -1275,7 +1676,9 @@ void free_all(khip_ctx* ctx) {
                     ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
                     ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
-                    ctx->exact_parts, ctx->surv};
+                    ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
+                    ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
+                    ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -1393,14 +1796,13 @@ void mark(khip_ctx* ctx, int k) {
 
 // Enqueues the screening kernels for the step whose start / end states are
 // `before` / `after` (stats record `stat_slot`).
-bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t step, int stat_slot) {
+bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
+                    double thr, int midpoint) {
     const int n = static_cast<int>(ctx->n);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
-    const Planes pb = ctx->planes(before), pa = ctx->planes(after);
-    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), ctx->dt,
-                                     ctx->cfg.threshold_km, ctx->passes);
-    k_keys<<<tiles, BLOCK, 0, s>>>(pb, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
+    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint);
+    k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
                                    ctx->hist, tiles);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
@@ -1439,8 +1841,8 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
     P.surv_cap = static_cast<unsigned long long>(ctx->surv_cap);
     P.step = step;
     P.t0 = t0;
-    P.dt = ctx->dt;
-    P.thr = ctx->cfg.threshold_km;
+    P.dt = dt;
+    P.thr = thr;
     P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
     P.dt_f = static_cast<float>(P.dt);
     P.thr_f = static_cast<float>(P.thr);
@@ -1461,13 +1863,19 @@ bool enqueue_screen(khip_ctx* ctx, int before, int after, double t0, int64_t ste
 }
 
 // Called after the chunk's stream synchronisation.
-void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
+void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened, bool learned = false) {
     for (int64_t k = 0; k < steps; ++k) {
         hipEvent_t* e = ctx->ev_pool.data() + MARKS * k;
         float a = 0, b = 0, c = 0, d = 0, f = 0, g = 0;
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         ctx->tm.propagate_ms += a;
-        if (screened) {
+        if (learned) {
+            // Screening of all sub-steps and the re-verification as one figure.
+            (void)hipEventElapsedTime(&b, e[1], e[5]);
+            (void)hipEventElapsedTime(&d, e[0], e[5]);
+            ctx->tm.pairs_ms += b;
+            ctx->tm.total_ms += d;
+        } else if (screened) {
             (void)hipEventElapsedTime(&b, e[1], e[2]);
             (void)hipEventElapsedTime(&c, e[2], e[3]);
             (void)hipEventElapsedTime(&f, e[3], e[4]);
@@ -1536,10 +1944,13 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     // (collision.cpp:739-747): counting sort on the chunk's step index, then
     // each step's events sorted on their own, steps in parallel.
     std::vector<int64_t> start(static_cast<size_t>(steps) + 1, 0);
+    // Events the plain re-verification of the learned form rejected (pad 1).
+    raw.erase(std::remove_if(raw.begin(), raw.end(), [](const khip_event& e) { return e.pad == 1; }), raw.end());
+    const int64_t nk = static_cast<int64_t>(raw.size());
     for (const khip_event& e : raw) ++start[static_cast<size_t>(e.step - step0) + 1];
     for (int64_t k = 0; k < steps; ++k) start[k + 1] += start[k];
     const size_t out0 = ctx->host_events.size();
-    ctx->host_events.resize(out0 + static_cast<size_t>(ne));
+    ctx->host_events.resize(out0 + static_cast<size_t>(nk));
     khip_event* out = ctx->host_events.data() + out0;
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
     for (const khip_event& e : raw) out[fill[static_cast<size_t>(e.step - step0)]++] = e;
@@ -1606,7 +2017,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                         ctx->step_parts);
                 mark(ctx, 1);
                 if (screen) {
-                    if (!enqueue_screen(ctx, cur, nxt, t, ctx->step + done + k, static_cast<int>(k)))
+                    if (!enqueue_screen(ctx, ctx->planes(cur), ctx->planes(nxt), t, ctx->step + done + k,
+                                        static_cast<int>(k), ctx->dt, ctx->cfg.threshold_km, 0))
                         return false;
                 } else {
                     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0,
@@ -1631,6 +2043,234 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], ctx->snap_state, sizeof(double) * 6 * c,
                                          hipMemcpyDeviceToDevice, s));
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, ctx->snap_alive, c, hipMemcpyDeviceToDevice, s));
+        }
+        done += len;
+    }
+    ctx->step += nsteps;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Learned form: host orchestration
+// ---------------------------------------------------------------------------
+
+bool ensure_learned(khip_ctx* ctx) {
+    const int sub = ctx->lcfg.substeps;
+    if (ctx->lrn_cap >= ctx->cap && ctx->mode && ctx->dense_k >= sub - 1) return true;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    if (!dev_alloc(ctx, &ctx->mode, c) || !dev_alloc(ctx, &ctx->snap_mode, c) ||
+        !dev_alloc(ctx, &ctx->lists, 3 * c) || !dev_alloc(ctx, &ctx->snap_lists, 3 * c + 3) ||
+        !dev_alloc(ctx, &ctx->counts, 3) || !dev_alloc(ctx, &ctx->cstats, 1) || !dev_alloc(ctx, &ctx->rstats, 1) ||
+        !dev_alloc(ctx, &ctx->substates, static_cast<size_t>(std::max(1, sub - 1)) * 6 * c))
+        return false;
+    KHIP_TRY(ctx, hipMemset(ctx->mode, net::CLASS_PLAIN, c));
+    KHIP_TRY(ctx, hipMemset(ctx->counts, 0, 3 * sizeof(int)));
+    KHIP_TRY(ctx, hipMemset(ctx->cstats, 0, sizeof(CheckStats)));
+    ctx->lrn_cap = ctx->cap;
+    ctx->dense_k = sub - 1;
+    ctx->force_check = true;
+    return true;
+}
+
+Planes dense_planes(khip_ctx* ctx, int k) {
+    const size_t c = static_cast<size_t>(ctx->cap);
+    double* b = ctx->substates + static_cast<size_t>(k) * 6 * c;
+    return Planes{b, b + c, b + 2 * c, b + 3 * c, b + 4 * c, b + 5 * c};
+}
+
+// Moon every tau / 12 from t_start over nsteps learned steps, plus two
+// entries past the end.
+bool upload_moon12(khip_ctx* ctx, double t_start, int64_t nsteps) {
+    const int64_t entries = 12 * nsteps + 3;
+    if (entries > ctx->moon_cap) {
+        if (!dev_alloc(ctx, &ctx->moon, entries)) return false;
+        ctx->moon_cap = entries;
+    }
+    std::vector<MoonSample>& tab = ctx->moon_host;
+    tab.resize(static_cast<size_t>(entries));
+    const double q = ctx->dt / 12.0;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < entries; ++k) {
+        const Vec3 m = moon_position(ctx->epoch_jd + (t_start + static_cast<double>(k) * q) / 86400.0);
+        MoonSample& s = tab[static_cast<size_t>(k)];
+        s.m[0] = m.x; s.m[1] = m.y; s.m[2] = m.z;
+        const double mn = norm(m);
+        const double f = kessler::MU_MOON / (mn * mn * mn);
+        s.ind[0] = m.x * f; s.ind[1] = m.y * f; s.ind[2] = m.z * f;
+    }
+    KHIP_TRY(ctx, hipMemcpyAsync(ctx->moon, tab.data(), sizeof(MoonSample) * entries, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    return true;
+}
+
+__global__ void k_check_begin(CheckStats* cs, int* counts) {
+    if (threadIdx.x < 3) {
+        cs->count[threadIdx.x] = 0;
+        counts[threadIdx.x] = 0;
+    }
+}
+
+bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
+    if (!ctx->net_ready) {
+        ctx->error = "learned form: no corrector loaded or trained";
+        return false;
+    }
+    if (std::fabs(ctx->dt - ctx->net_norm.tau) > 1e-9 * ctx->net_norm.tau) {
+        ctx->error = "learned form: the step (khip_set_time dt) must equal the corrector's tau";
+        return false;
+    }
+    const khip_learned_config& L = ctx->lcfg;
+    if (12 % L.substeps != 0) {
+        ctx->error = "learned form: substeps must divide 12";
+        return false;
+    }
+    const int n = static_cast<int>(ctx->n);
+    hipStream_t s = ctx->stream;
+    const int sub = L.substeps;
+    const double h = ctx->dt / sub;
+    if (n == 0) {
+        for (int64_t k = 0; k < nsteps; ++k) {
+            if (screen)
+                for (int q = 0; q < sub; ++q)
+                    ctx->host_stats.push_back(khip_stats{0, 0, 0, std::max(1.0, ctx->cfg.threshold_km), 0, 0, 0, 0});
+            ctx->t += ctx->dt;
+            ctx->step += 1;
+        }
+        return true;
+    }
+    if (!ensure_learned(ctx)) return false;
+    LearnedParams lp;
+    lp.tau = ctx->dt;
+    lp.substeps = sub;
+    lp.pad = 0;
+    lp.e_max = L.e_max;
+    lp.a_max = L.a_max_km;
+    lp.tol_km = L.tol_km;
+    const double thr = ctx->cfg.threshold_km + L.margin_km;
+    const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    const int g_net = ctx->n_cus;
+    const int g_all = std::min(grid_of(n), 4 * ctx->n_cus);
+    int64_t done = 0, next_len = 1;
+    while (done < nsteps) {
+        const int64_t len = std::min<int64_t>(std::min<int64_t>(next_len, ctx->chunk), nsteps - done);
+        next_len = std::min<int64_t>(2 * next_len, ctx->chunk);
+        if (!ensure_stats(ctx, len * sub) ||
+            !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents)) ||
+            (ctx->timing && !ensure_event_pool(ctx, len)) || !upload_moon12(ctx, ctx->t, len))
+            return false;
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
+                                     hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_alive, ctx->alive, c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_mode, ctx->mode, c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_lists, ctx->lists, sizeof(int) * 3 * c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_lists + 3 * c, ctx->counts, sizeof(int) * 3, hipMemcpyDeviceToDevice, s));
+        CheckStats cs0;
+        KHIP_TRY(ctx, hipMemcpyAsync(&cs0, ctx->cstats, sizeof(cs0), hipMemcpyDeviceToHost, s));
+        KHIP_TRY(ctx, hipStreamSynchronize(s));
+        const int cur0 = ctx->cur;
+        const double t_chunk = ctx->t;
+        const bool force0 = ctx->force_check;
+        int64_t checks = 0;
+        for (;;) {
+            k_reset<<<1, 64, 0, s>>>(ctx->gs);
+            k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+            KHIP_TRY(ctx, hipMemsetAsync(ctx->rstats, 0, sizeof(ReverifyStats), s));
+            double t = t_chunk;
+            int cur = cur0;
+            bool force = force0;
+            checks = 0;
+            for (int64_t k = 0; k < len; ++k) {
+                const int nxt = cur ^ 1;
+                const int64_t macro = ctx->step + done + k;
+                const MoonSample* m13 = ctx->moon + 12 * k;
+                g_slot = static_cast<int>(k);
+                mark(ctx, 0);
+                const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
+                if (force || macro % L.check_every == 0) {
+                    k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
+                    k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
+                        pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
+                        ctx->net_norm, lp, ctx->cstats);
+                    k_build_lists<<<grid_of(n), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
+                                                               static_cast<int>(c), ctx->counts);
+                    force = false;
+                    ++checks;
+                } else {
+                    k_learned<net::CLASS_LEARNED><<<g_net, BLOCK, 0, s>>>(
+                        pc, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
+                        ctx->net_norm, lp, ctx->cstats);
+                    k_learned<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(
+                        pc, pn, ctx->alive, ctx->mode, ctx->lists + 2 * c, ctx->counts, n, m13, ctx->net_ffwd,
+                        ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                    k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
+                        pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
+                        ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                }
+                mark(ctx, 1);
+                if (screen) {
+                    k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
+                    for (int q = 0; q < sub; ++q) {
+                        const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
+                        if (q < sub - 1)
+                            k_dense_output<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, n, ctx->dt,
+                                                                        static_cast<double>(q + 1) / sub,
+                                                                        dense_planes(ctx, q));
+                        const Planes pa = q == sub - 1 ? pn : dense_planes(ctx, q);
+                        k_reduce_mid<<<grid_of(n), BLOCK, 0, s>>>(pb, pa, ctx->alive, n, ctx->step_parts);
+                        if (!enqueue_screen(ctx, pb, pa, t + q * h, macro * sub + q, static_cast<int>(k * sub + q), h,
+                                            thr, 1))
+                            return false;
+                    }
+                    k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
+                                                              static_cast<unsigned long long>(ctx->ev_cap), pc,
+                                                              ctx->radius, m13, t, macro * sub, h, sub, thr, vmin2,
+                                                              ctx->rstats);
+                } else {
+                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, ctx->dense_parts, 0,
+                                                 ctx->exact_parts, 0);
+                }
+                mark(ctx, 5);
+                t += ctx->dt;
+                cur = nxt;
+            }
+            KHIP_TRY(ctx, hipGetLastError());
+            bool overflow = false;
+            if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
+            if (!overflow) {
+                if (ctx->timing) accumulate_timing(ctx, len, screen != 0, true);
+                ctx->cur = cur;
+                ctx->t = t;
+                ctx->force_check = force;
+                CheckStats cs;
+                ReverifyStats rs;
+                KHIP_TRY(ctx, hipMemcpy(&cs, ctx->cstats, sizeof(cs), hipMemcpyDeviceToHost));
+                KHIP_TRY(ctx, hipMemcpy(&rs, ctx->rstats, sizeof(rs), hipMemcpyDeviceToHost));
+                khip_learned_report& r = ctx->lrep;
+                if (checks > 0) {
+                    r.learned = cs.count[net::CLASS_LEARNED];
+                    r.plain = cs.count[net::CLASS_PLAIN];
+                    r.base = cs.count[net::CLASS_BASE];
+                }
+                r.rejected = cs.rejected;
+                r.dev_max_km = __builtin_bit_cast(double, cs.dev_max);
+                r.checks += checks;
+                r.events_verified += static_cast<int64_t>(rs.verified);
+                r.events_dropped += static_cast<int64_t>(rs.dropped);
+                r.events_unverified += static_cast<int64_t>(rs.kept_unverified);
+                r.steps += len;
+                break;
+            }
+            ctx->tm.reruns += 1;
+            ctx->tm.rerun_reasons |= ctx->last_rerun;
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], ctx->snap_state, sizeof(double) * 6 * c,
+                                         hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, ctx->snap_alive, c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->mode, ctx->snap_mode, c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->lists, ctx->snap_lists, sizeof(int) * 3 * c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->counts, ctx->snap_lists + 3 * c, sizeof(int) * 3,
+                                         hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->cstats, &cs0, sizeof(cs0), hipMemcpyHostToDevice, s));
         }
         done += len;
     }
@@ -1668,6 +2308,290 @@ bool refresh_radius(khip_ctx* ctx) {
     for (int64_t i = 0; i < ctx->n; ++i)
         rad[i] = 0.5 * ctx->host_size[i] * 1e-3 * ctx->cfg.radius_scale;
     KHIP_TRY(ctx, hipMemcpy(ctx->radius, rad.data(), sizeof(double) * ctx->n, hipMemcpyHostToDevice));
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Learned corrector: parameters, training, inference
+// ---------------------------------------------------------------------------
+
+bool net_alloc(khip_ctx* ctx) {
+    if (ctx->net_p) return true;
+    return dev_alloc(ctx, &ctx->net_p, net::NPARAM) && dev_alloc(ctx, &ctx->net_m1, net::NPARAM) &&
+           dev_alloc(ctx, &ctx->net_m2, net::NPARAM) &&
+           dev_alloc(ctx, &ctx->net_ffwd, static_cast<size_t>(net::NFRAG_FWD) * net::FRAG_HALVES) &&
+           dev_alloc(ctx, &ctx->net_fbwd, static_cast<size_t>(net::NFRAG_BWD) * net::FRAG_HALVES);
+}
+
+void net_pack(khip_ctx* ctx) {
+    const int total = (net::NFRAG_FWD + net::NFRAG_BWD) * net::FRAG_HALVES;
+    net::k_net_pack<<<(total + 255) / 256, 256, 0, ctx->stream>>>(ctx->net_p, ctx->net_ffwd, ctx->net_fbwd);
+}
+
+bool net_upload(khip_ctx* ctx, const float* params) {
+    if (!net_alloc(ctx)) return false;
+    KHIP_TRY(ctx, hipMemcpyAsync(ctx->net_p, params, sizeof(float) * net::NPARAM, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    KHIP_TRY(ctx, hipMemsetAsync(ctx->net_m1, 0, sizeof(float) * net::NPARAM, ctx->stream));
+    KHIP_TRY(ctx, hipMemsetAsync(ctx->net_m2, 0, sizeof(float) * net::NPARAM, ctx->stream));
+    net_pack(ctx);
+    KHIP_TRY(ctx, hipGetLastError());
+    KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return true;
+}
+
+// PyTorch-style initialisation: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for
+// weights and biases, from a seeded splitmix64 stream.
+std::vector<float> net_init_params(uint64_t seed) {
+    std::vector<float> p(net::NPARAM);
+    uint64_t st = seed;
+    auto uni = [&](float bound) {
+        st += 1;
+        const uint64_t z = net::mix64(st);
+        const double u = static_cast<double>(z >> 11) * (1.0 / 9007199254740992.0);
+        return static_cast<float>((2.0 * u - 1.0) * bound);
+    };
+    struct L { int w, b, nin, nout; };
+    const L layers[4] = {{net::OFF_W1, net::OFF_B1, net::NIN, net::NH},
+                         {net::OFF_W2, net::OFF_B2, net::NH, net::NH},
+                         {net::OFF_W3, net::OFF_B3, net::NH, net::NH},
+                         {net::OFF_W4, net::OFF_B4, net::NH, net::NOUT}};
+    for (const L& l : layers) {
+        const float bound = 1.0f / std::sqrt(static_cast<float>(l.nin));
+        for (int k = 0; k < l.nin * l.nout; ++k) p[l.w + k] = uni(bound);
+        for (int k = 0; k < l.nout; ++k) p[l.b + k] = uni(bound);
+    }
+    return p;
+}
+
+// Device buffers of one training run (freed at the end of the run).
+struct TrainScratch {
+    std::vector<void*> ptrs;
+    template <typename T>
+    bool get(khip_ctx* ctx, T** p, size_t n) {
+        *p = nullptr;
+        KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)));
+        ptrs.push_back(*p);
+        return true;
+    }
+    ~TrainScratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+bool net_alloc_tensors(khip_ctx* ctx, TrainScratch& sc, int B, net::TrainTensors& T) {
+    const size_t b = static_cast<size_t>(B);
+    if (!sc.get(ctx, &T.hx, 16 * b) || !sc.get(ctx, &T.h1, 128 * b) || !sc.get(ctx, &T.h2, 128 * b) ||
+        !sc.get(ctx, &T.h3, 128 * b) || !sc.get(ctx, &T.g1, 128 * b) || !sc.get(ctx, &T.g2, 128 * b) ||
+        !sc.get(ctx, &T.g3, 128 * b) || !sc.get(ctx, &T.g4, 16 * b) || !sc.get(ctx, &T.loss, b / 16))
+        return false;
+    KHIP_TRY(ctx, hipMemsetAsync(T.hx, 0, sizeof(uint16_t) * 16 * b, ctx->stream));
+    return true;
+}
+
+constexpr int NET_KC = 1024;  // batch columns per weight-gradient chunk
+
+// One forward + backward + weight-gradient pass over a batch of B columns
+// (B a multiple of NET_KC); the chunk partials land in `part`.
+void net_fwdbwd(khip_ctx* ctx, const float* X, const float* Y, const uint32_t* pool, uint32_t npool,
+                uint64_t seed, int B, const net::TrainTensors& T, float* part) {
+    hipStream_t s = ctx->stream;
+    const int waves = B / 16;
+    net::k_net_fwdbwd<<<(waves + 3) / 4, 256, 0, s>>>(X, Y, pool, npool, seed, B, ctx->net_ffwd, ctx->net_fbwd,
+                                                      ctx->net_p, T);
+    const int tasks = net::DW_TILES * (B / NET_KC);
+    net::k_net_dw<<<(tasks + 3) / 4, 256, 0, s>>>(T, B, NET_KC, part);
+}
+
+bool net_train(khip_ctx* ctx, const double* orbits, int64_t n, const khip_train_config& cfg,
+               khip_train_report& rep, float* curve) {
+    hipStream_t s = ctx->stream;
+    TrainScratch sc;
+    const double tau = cfg.tau;
+    const int per = cfg.samples_per_orbit, every = cfg.spacing_steps;
+    const int steps = per * every;
+    // Moon every tau / 12 over the sampling window.
+    const int64_t nm = 12LL * steps + 1;
+    std::vector<MoonSample> moon(static_cast<size_t>(nm));
+    for (int64_t k = 0; k < nm; ++k) {
+        const double t = ctx->t + static_cast<double>(k) * tau / 12.0;
+        const Vec3 m = moon_position(ctx->epoch_jd + t / 86400.0);
+        MoonSample& ms = moon[static_cast<size_t>(k)];
+        ms.m[0] = m.x; ms.m[1] = m.y; ms.m[2] = m.z;
+        const double mn = norm(m);
+        const double f = kessler::MU_MOON / (mn * mn * mn);
+        ms.ind[0] = m.x * f; ms.ind[1] = m.y * f; ms.ind[2] = m.z * f;
+    }
+    const size_t ns = static_cast<size_t>(n) * per;
+    double* d_orb;
+    MoonSample* d_moon;
+    float *fx, *fy;
+    uint8_t* valid;
+    if (!sc.get(ctx, &d_orb, 6 * static_cast<size_t>(n)) || !sc.get(ctx, &d_moon, nm) ||
+        !sc.get(ctx, &fx, ns * net::NIN) || !sc.get(ctx, &fy, ns * net::NOUT) || !sc.get(ctx, &valid, ns))
+        return false;
+    KHIP_TRY(ctx, hipMemcpyAsync(d_orb, orbits, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(d_moon, moon.data(), sizeof(MoonSample) * nm, hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1, e2;
+    KHIP_TRY(ctx, hipEventCreate(&e0));
+    KHIP_TRY(ctx, hipEventCreate(&e1));
+    KHIP_TRY(ctx, hipEventCreate(&e2));
+    (void)hipEventRecord(e0, s);
+    net::k_net_samples<<<grid_of(n), 256, 0, s>>>(d_orb, static_cast<int>(n), steps, every, d_moon, tau, fx, fy,
+                                                  valid, per, cfg.e_max, cfg.a_max_km);
+    (void)hipEventRecord(e1, s);
+    KHIP_TRY(ctx, hipGetLastError());
+    // Standardisation from the valid samples (host, FP64 sums).
+    std::vector<float> hx(ns * net::NIN), hy(ns * net::NOUT);
+    std::vector<uint8_t> hv(ns);
+    KHIP_TRY(ctx, hipMemcpyAsync(hx.data(), fx, sizeof(float) * hx.size(), hipMemcpyDeviceToHost, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(hy.data(), fy, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(hv.data(), valid, ns, hipMemcpyDeviceToHost, s));
+    KHIP_TRY(ctx, hipStreamSynchronize(s));
+    float gen_ms = 0.0f;
+    (void)hipEventElapsedTime(&gen_ms, e0, e1);
+    std::vector<uint32_t> pool;
+    pool.reserve(ns);
+    double sx[net::NIN] = {}, sxx[net::NIN] = {}, sy[net::NOUT] = {}, syy[net::NOUT] = {};
+    for (size_t k = 0; k < ns; ++k) {
+        if (!hv[k]) continue;
+        pool.push_back(static_cast<uint32_t>(k));
+        for (int a = 0; a < net::NIN; ++a) {
+            const double v = hx[k * net::NIN + a];
+            sx[a] += v;
+            sxx[a] += v * v;
+        }
+        for (int a = 0; a < net::NOUT; ++a) {
+            const double v = hy[k * net::NOUT + a];
+            sy[a] += v;
+            syy[a] += v * v;
+        }
+    }
+    if (pool.size() < 1024) {
+        ctx->error = "too few valid training samples (orbits outside the learned domain?)";
+        return false;
+    }
+    const double cnt = static_cast<double>(pool.size());
+    net::NetNorm nn{};
+    nn.tau = tau;
+    for (int a = 0; a < net::NIN; ++a) {
+        const double mu = sx[a] / cnt, var = std::max(0.0, sxx[a] / cnt - mu * mu);
+        nn.fmu[a] = static_cast<float>(mu);
+        nn.fsig[a] = var > 1e-24 ? static_cast<float>(std::sqrt(var)) : 1.0f;
+    }
+    double vb[2] = {0.0, 0.0};
+    for (int a = 0; a < net::NOUT; ++a) {
+        const double mu = sy[a] / cnt, var = std::max(0.0, syy[a] / cnt - mu * mu);
+        nn.omu[a] = static_cast<float>(mu);
+        vb[a / 3] += var / 3.0;
+    }
+    for (int a = 0; a < net::NOUT; ++a) {
+        const double sd = std::sqrt(vb[a / 3]);
+        nn.osig[a] = sd > 1e-30 ? static_cast<float>(sd) : 1.0f;
+    }
+    net::k_net_standardise<<<static_cast<unsigned>((ns + 255) / 256), 256, 0, s>>>(fx, fy, ns, nn);
+    uint32_t* d_pool;
+    if (!sc.get(ctx, &d_pool, pool.size())) return false;
+    KHIP_TRY(ctx, hipMemcpyAsync(d_pool, pool.data(), sizeof(uint32_t) * pool.size(), hipMemcpyHostToDevice, s));
+    // Fresh parameters.
+    const std::vector<float> init = net_init_params(cfg.seed);
+    if (!net_upload(ctx, init.data())) return false;
+    const int B = cfg.batch;
+    net::TrainTensors T;
+    float *part, *losses;
+    if (!net_alloc_tensors(ctx, sc, B, T) || !sc.get(ctx, &part, static_cast<size_t>(B / NET_KC) * net::NPARAM) ||
+        !sc.get(ctx, &losses, cfg.iterations))
+        return false;
+    (void)hipEventRecord(e1, s);
+    const double b1 = 0.9, b2 = 0.999;
+    double pb1 = 1.0, pb2 = 1.0;
+    const int warm = std::max(1, cfg.iterations / 20);
+    for (int it = 0; it < cfg.iterations; ++it) {
+        double lr;
+        if (it < warm) {
+            lr = cfg.lr_max * (it + 1) / warm;
+        } else {
+            const double q = static_cast<double>(it - warm) / std::max(1, cfg.iterations - warm);
+            lr = cfg.lr_max * (0.01 + 0.99 * 0.5 * (1.0 + std::cos(3.141592653589793 * q)));
+        }
+        pb1 *= b1;
+        pb2 *= b2;
+        net_fwdbwd(ctx, fx, fy, d_pool, static_cast<uint32_t>(pool.size()),
+                   cfg.seed * 0x100000001b3ULL + static_cast<uint64_t>(it) * static_cast<uint64_t>(B), B, T, part);
+        net::AdamArgs a{static_cast<float>(lr), static_cast<float>(b1), static_cast<float>(b2), 1e-8f,
+                        static_cast<float>(1.0 - pb1), static_cast<float>(1.0 - pb2), B / NET_KC, 1};
+        net::k_net_adam<<<(net::NPARAM + 255) / 256, 256, 0, s>>>(ctx->net_p, ctx->net_m1, ctx->net_m2, part,
+                                                                  nullptr, a);
+        net_pack(ctx);
+        net::k_net_loss<<<1, 256, 0, s>>>(T.loss, B / 16, losses + it);
+    }
+    (void)hipEventRecord(e2, s);
+    KHIP_TRY(ctx, hipGetLastError());
+    std::vector<float> lh(static_cast<size_t>(cfg.iterations));
+    KHIP_TRY(ctx, hipMemcpyAsync(lh.data(), losses, sizeof(float) * lh.size(), hipMemcpyDeviceToHost, s));
+    KHIP_TRY(ctx, hipStreamSynchronize(s));
+    float train_ms = 0.0f;
+    (void)hipEventElapsedTime(&train_ms, e1, e2);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    // Per-batch sums of squared error -> mean per output.
+    const double scale = 1.0 / (6.0 * B);
+    const int win = std::min(100, cfg.iterations);
+    double first = 0.0, last = 0.0;
+    for (int k = 0; k < win; ++k) {
+        first += lh[k] * scale / win;
+        last += lh[cfg.iterations - 1 - k] * scale / win;
+    }
+    if (curve)
+        for (int k = 0; k < cfg.iterations; ++k) curve[k] = static_cast<float>(lh[k] * scale);
+    ctx->net_norm = nn;
+    ctx->net_ready = true;
+    rep.samples = static_cast<int64_t>(pool.size());
+    rep.loss_first = first;
+    rep.loss_last = last;
+    rep.train_ms = train_ms;
+    rep.gen_ms = gen_ms;
+    return true;
+}
+
+bool net_infer(khip_ctx* ctx, int64_t n, const float* x, float* out) {
+    TrainScratch sc;
+    float *dx, *dout;
+    if (!sc.get(ctx, &dx, static_cast<size_t>(n) * net::NIN) || !sc.get(ctx, &dout, static_cast<size_t>(n) * net::NOUT))
+        return false;
+    KHIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * net::NIN, hipMemcpyHostToDevice, ctx->stream));
+    const int64_t waves = (n + 63) / 64;
+    net::k_net_infer<<<static_cast<unsigned>((waves + 3) / 4), 256, 0, ctx->stream>>>(dx, static_cast<int>(n),
+                                                                                     ctx->net_ffwd, ctx->net_p, dout);
+    KHIP_TRY(ctx, hipGetLastError());
+    KHIP_TRY(ctx, hipMemcpyAsync(out, dout, sizeof(float) * n * net::NOUT, hipMemcpyDeviceToHost, ctx->stream));
+    KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return true;
+}
+
+bool net_gradient(khip_ctx* ctx, int64_t n, const float* x, const float* y, float* grad, float* loss) {
+    TrainScratch sc;
+    const int B = static_cast<int>(n);
+    float *dx, *dy, *part, *dgrad, *dloss;
+    net::TrainTensors T;
+    if (!sc.get(ctx, &dx, static_cast<size_t>(n) * net::NIN) || !sc.get(ctx, &dy, static_cast<size_t>(n) * net::NOUT) ||
+        !sc.get(ctx, &part, static_cast<size_t>(B / NET_KC) * net::NPARAM) || !sc.get(ctx, &dgrad, net::NPARAM) ||
+        !sc.get(ctx, &dloss, 1) || !net_alloc_tensors(ctx, sc, B, T))
+        return false;
+    hipStream_t s = ctx->stream;
+    KHIP_TRY(ctx, hipMemcpyAsync(dx, x, sizeof(float) * n * net::NIN, hipMemcpyHostToDevice, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(dy, y, sizeof(float) * n * net::NOUT, hipMemcpyHostToDevice, s));
+    net_fwdbwd(ctx, dx, dy, nullptr, 0, 0, B, T, part);
+    net::AdamArgs a{0.f, 0.f, 0.f, 0.f, 1.f, 1.f, B / NET_KC, 0};
+    net::k_net_adam<<<(net::NPARAM + 255) / 256, 256, 0, s>>>(ctx->net_p, ctx->net_m1, ctx->net_m2, part, dgrad, a);
+    net::k_net_loss<<<1, 256, 0, s>>>(T.loss, B / 16, dloss);
+    KHIP_TRY(ctx, hipGetLastError());
+    KHIP_TRY(ctx, hipMemcpyAsync(grad, dgrad, sizeof(float) * net::NPARAM, hipMemcpyDeviceToHost, s));
+    float l = 0.0f;
+    KHIP_TRY(ctx, hipMemcpyAsync(&l, dloss, sizeof(float), hipMemcpyDeviceToHost, s));
+    KHIP_TRY(ctx, hipStreamSynchronize(s));
+    if (loss) *loss = l / static_cast<float>(6.0 * B);
     return true;
 }
 
@@ -1709,6 +2633,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     }
     khip_ctx* ctx = new khip_ctx;
     ctx->device = device;
+    ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     char buf[512];
     std::snprintf(buf, sizeof(buf), "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName,
                   prop.multiProcessorCount, prop.totalGlobalMem / 1e9);
@@ -1744,6 +2669,8 @@ const char* khip_last_error(const khip_ctx* ctx) { return ctx ? ctx->error.c_str
 int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
                      const double* size_m) {
     if (!ctx || n < 0 || (n > 0 && !states) || n >= (1LL << 31) - TILE) return KHIP_ERR_ARG;
+    ctx->force_check = true;
+    ctx->lrep = khip_learned_report{};
     return status(ctx, upload_objects(ctx, n, states, alive, size_m));
 }
 
@@ -1812,6 +2739,7 @@ int khip_get_time(const khip_ctx* ctx, double* t, int64_t* step) {
 
 int khip_run(khip_ctx* ctx, int64_t nsteps, int screen) {
     if (!ctx || nsteps < 0) return KHIP_ERR_ARG;
+    if (ctx->precision == KHIP_PRECISION_LEARNED) return status(ctx, run_steps_learned(ctx, nsteps, screen));
     return status(ctx, run_steps(ctx, nsteps, screen));
 }
 
@@ -1844,7 +2772,7 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
             mark(ctx, 1);
             k_reduce<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n),
                                                   ctx->step_parts);
-            ok = enqueue_screen(ctx, 0, 1, t0, 0, 0);
+            ok = enqueue_screen(ctx, ctx->planes(0), ctx->planes(1), t0, 0, 0, dt, ctx->cfg.threshold_km, 0);
         } else {
             k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, ctx->dense_parts, 0,
                                          ctx->exact_parts, 0);
@@ -1920,9 +2848,81 @@ int khip_reset_timing(khip_ctx* ctx) {
 }
 
 int khip_set_precision(khip_ctx* ctx, int mode) {
-    if (!ctx || (mode != KHIP_PRECISION_FP64 && mode != KHIP_PRECISION_MIXED)) return KHIP_ERR_ARG;
+    if (!ctx || (mode != KHIP_PRECISION_FP64 && mode != KHIP_PRECISION_MIXED && mode != KHIP_PRECISION_LEARNED))
+        return KHIP_ERR_ARG;
     ctx->precision = mode;
+    ctx->force_check = true;
     return KHIP_OK;
+}
+
+int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
+    if (!ctx || !cfg || cfg->substeps < 1 || cfg->substeps > 12 || 12 % cfg->substeps != 0 ||
+        cfg->check_every < 1 || !(cfg->tol_km >= 0) || !(cfg->margin_km >= 0) || !(cfg->e_max >= 0) ||
+        !(cfg->a_max_km >= 0))
+        return KHIP_ERR_ARG;
+    ctx->lcfg = *cfg;
+    ctx->force_check = true;
+    return KHIP_OK;
+}
+
+int khip_get_learned_report(const khip_ctx* ctx, khip_learned_report* out) {
+    if (!ctx || !out) return KHIP_ERR_ARG;
+    *out = ctx->lrep;
+    return KHIP_OK;
+}
+
+int khip_net_train(khip_ctx* ctx, int64_t n, const double* orbits, const khip_train_config* cfg,
+                   khip_train_report* report, float* curve) {
+    if (!ctx || n <= 0 || n >= (1LL << 31) || !orbits || !cfg || cfg->samples_per_orbit <= 0 ||
+        cfg->spacing_steps <= 0 || cfg->iterations <= 0 || cfg->batch <= 0 || cfg->batch % NET_KC != 0 ||
+        !(cfg->tau > 0) || !(cfg->lr_max > 0) ||
+        static_cast<double>(n) * cfg->samples_per_orbit >= 4.0e9)
+        return KHIP_ERR_ARG;
+    khip_train_report r{};
+    const bool ok = net_train(ctx, orbits, n, *cfg, r, curve);
+    if (report) *report = r;
+    return status(ctx, ok);
+}
+
+int khip_net_set(khip_ctx* ctx, const float* params, const khip_net_norm* norm) {
+    if (!ctx || !params || !norm || !(norm->tau > 0)) return KHIP_ERR_ARG;
+    static_assert(sizeof(khip_net_norm) == sizeof(net::NetNorm), "norm layout");
+    if (!net_upload(ctx, params)) return KHIP_ERR_HIP;
+    std::memcpy(&ctx->net_norm, norm, sizeof(net::NetNorm));
+    ctx->net_ready = true;
+    return KHIP_OK;
+}
+
+int khip_net_get(khip_ctx* ctx, float* params, khip_net_norm* norm) {
+    if (!ctx) return KHIP_ERR_ARG;
+    if (!ctx->net_ready) {
+        ctx->error = "no corrector loaded or trained";
+        return KHIP_ERR_STATE;
+    }
+    if (params && hipMemcpy(params, ctx->net_p, sizeof(float) * net::NPARAM, hipMemcpyDeviceToHost) != hipSuccess) {
+        ctx->error = "hipMemcpy of corrector parameters failed";
+        return KHIP_ERR_HIP;
+    }
+    if (norm) std::memcpy(norm, &ctx->net_norm, sizeof(net::NetNorm));
+    return KHIP_OK;
+}
+
+int khip_net_infer(khip_ctx* ctx, int64_t n, const float* x, float* out) {
+    if (!ctx || n <= 0 || n >= (1LL << 31) || !x || !out) return KHIP_ERR_ARG;
+    if (!ctx->net_ready) {
+        ctx->error = "no corrector loaded or trained";
+        return KHIP_ERR_STATE;
+    }
+    return status(ctx, net_infer(ctx, n, x, out));
+}
+
+int khip_net_gradient(khip_ctx* ctx, int64_t n, const float* x, const float* y, float* grad, float* loss) {
+    if (!ctx || n <= 0 || n % NET_KC != 0 || n > (1 << 24) || !x || !y || !grad) return KHIP_ERR_ARG;
+    if (!ctx->net_ready) {
+        ctx->error = "no corrector loaded or trained";
+        return KHIP_ERR_STATE;
+    }
+    return status(ctx, net_gradient(ctx, n, x, y, grad, loss));
 }
 
 int khip_set_chunk(khip_ctx* ctx, int steps) {
