@@ -96,6 +96,7 @@ typedef struct {
     int64_t steps;
     int64_t reruns;         /* chunks run again after a device buffer grew */
     int64_t rerun_reasons;  /* OR of KHIP_RERUN_* over those re-runs */
+    double reverify_ms;     /* learned form: k_reverify, the plain re-solve of events */
 } khip_timing;
 
 #define KHIP_RERUN_PASSES 1    /* the cell-key extent needed more radix passes */

@@ -32,7 +32,7 @@ OBJECT_DTYPE = np.dtype([("id", "<i4"), ("event", "<i4"), ("alive", "<i4"), ("ty
 TIMING_DTYPE = np.dtype([("propagate_ms", "<f8"), ("grid_ms", "<f8"), ("pairs_ms", "<f8"),
                          ("pairs_sparse_ms", "<f8"), ("pairs_dense_ms", "<f8"), ("exact_ms", "<f8"),
                          ("total_ms", "<f8"), ("steps", "<i8"), ("reruns", "<i8"),
-                         ("rerun_reasons", "<i8")])
+                         ("rerun_reasons", "<i8"), ("reverify_ms", "<f8")])
 
 # Every symbol include/kessler_hip.h declares.
 ABI_SYMBOLS = (

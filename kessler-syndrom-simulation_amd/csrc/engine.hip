@@ -1778,7 +1778,11 @@ inline int grid_of(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK)
 // Timing marks of step slot `slot` in the current chunk: 0 before the
 // propagation, 1 after it, 2 before the pair kernels, 3 after k_pairs,
 // 4 after k_pairs_dense, 5 after k_exact.
-constexpr int MARKS = 6;
+// The learned form uses one slot per sub-step: marks 1-5 around that
+// sub-step's screening (1 before its reduction), and, in the slot of the
+// step's first sub-step, 6 / 7 around the propagation; mark 0 of the last
+// sub-step's slot follows the re-verification.
+constexpr int MARKS = 8;
 int g_slot = 0;
 
 bool ensure_event_pool(khip_ctx* ctx, int64_t steps) {
@@ -1863,19 +1867,13 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
 }
 
 // Called after the chunk's stream synchronisation.
-void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened, bool learned = false) {
+void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
     for (int64_t k = 0; k < steps; ++k) {
         hipEvent_t* e = ctx->ev_pool.data() + MARKS * k;
         float a = 0, b = 0, c = 0, d = 0, f = 0, g = 0;
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         ctx->tm.propagate_ms += a;
-        if (learned) {
-            // Screening of all sub-steps and the re-verification as one figure.
-            (void)hipEventElapsedTime(&b, e[1], e[5]);
-            (void)hipEventElapsedTime(&d, e[0], e[5]);
-            ctx->tm.pairs_ms += b;
-            ctx->tm.total_ms += d;
-        } else if (screened) {
+        if (screened) {
             (void)hipEventElapsedTime(&b, e[1], e[2]);
             (void)hipEventElapsedTime(&c, e[2], e[3]);
             (void)hipEventElapsedTime(&f, e[3], e[4]);
@@ -1890,6 +1888,37 @@ void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened, bool learned
         } else {
             (void)hipEventElapsedTime(&d, e[0], e[5]);
             ctx->tm.total_ms += d;
+        }
+        ctx->tm.steps += 1;
+    }
+}
+
+void accumulate_timing_learned(khip_ctx* ctx, int64_t steps, int sub, bool screened) {
+    for (int64_t k = 0; k < steps; ++k) {
+        hipEvent_t* e0 = ctx->ev_pool.data() + MARKS * (k * sub);
+        hipEvent_t* el = ctx->ev_pool.data() + MARKS * (k * sub + sub - 1);
+        float a = 0, d = 0;
+        (void)hipEventElapsedTime(&a, e0[6], e0[7]);
+        ctx->tm.propagate_ms += a;
+        (void)hipEventElapsedTime(&d, e0[6], screened ? el[0] : e0[7]);
+        ctx->tm.total_ms += d;
+        if (screened) {
+            for (int q = 0; q < sub; ++q) {
+                hipEvent_t* e = ctx->ev_pool.data() + MARKS * (k * sub + q);
+                float b = 0, c = 0, f = 0, g = 0;
+                (void)hipEventElapsedTime(&b, e[1], e[2]);
+                (void)hipEventElapsedTime(&c, e[2], e[3]);
+                (void)hipEventElapsedTime(&f, e[3], e[4]);
+                (void)hipEventElapsedTime(&g, e[4], e[5]);
+                ctx->tm.grid_ms += b;
+                ctx->tm.pairs_sparse_ms += c;
+                ctx->tm.pairs_dense_ms += f;
+                ctx->tm.exact_ms += g;
+                ctx->tm.pairs_ms += c + f + g;
+            }
+            float r = 0;
+            (void)hipEventElapsedTime(&r, el[5], el[0]);
+            ctx->tm.reverify_ms += r;
         }
         ctx->tm.steps += 1;
     }
@@ -2157,7 +2186,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         next_len = std::min<int64_t>(2 * next_len, ctx->chunk);
         if (!ensure_stats(ctx, len * sub) ||
             !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents)) ||
-            (ctx->timing && !ensure_event_pool(ctx, len)) || !upload_moon12(ctx, ctx->t, len))
+            (ctx->timing && !ensure_event_pool(ctx, len * sub)) || !upload_moon12(ctx, ctx->t, len))
             return false;
         KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
                                      hipMemcpyDeviceToDevice, s));
@@ -2184,8 +2213,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 const int nxt = cur ^ 1;
                 const int64_t macro = ctx->step + done + k;
                 const MoonSample* m13 = ctx->moon + 12 * k;
-                g_slot = static_cast<int>(k);
-                mark(ctx, 0);
+                g_slot = static_cast<int>(k * sub);
+                mark(ctx, 6);
                 const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
                 if (force || macro % L.check_every == 0) {
                     k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
@@ -2207,10 +2236,12 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
                         ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                 }
-                mark(ctx, 1);
+                mark(ctx, 7);
                 if (screen) {
                     k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                     for (int q = 0; q < sub; ++q) {
+                        g_slot = static_cast<int>(k * sub + q);
+                        mark(ctx, 1);
                         const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
                         if (q < sub - 1)
                             k_dense_output<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, n, ctx->dt,
@@ -2226,11 +2257,11 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                               static_cast<unsigned long long>(ctx->ev_cap), pc,
                                                               ctx->radius, m13, t, macro * sub, h, sub, thr, vmin2,
                                                               ctx->rstats);
+                    mark(ctx, 0);
                 } else {
                     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, ctx->dense_parts, 0,
                                                  ctx->exact_parts, 0);
                 }
-                mark(ctx, 5);
                 t += ctx->dt;
                 cur = nxt;
             }
@@ -2238,7 +2269,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             bool overflow = false;
             if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
             if (!overflow) {
-                if (ctx->timing) accumulate_timing(ctx, len, screen != 0, true);
+                if (ctx->timing) accumulate_timing_learned(ctx, len, sub, screen != 0);
                 ctx->cur = cur;
                 ctx->t = t;
                 ctx->force_check = force;
