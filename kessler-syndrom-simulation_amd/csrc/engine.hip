@@ -106,6 +106,8 @@ struct GridState {
     unsigned int pad1;
     unsigned long long events;      // events appended since the last drain
     unsigned long long ev_mark;     // events before the current learned step
+    int stamp;                      // screening counter (column-table entries of this screening)
+    int pad3;
 };
 
 struct DevEvent {
@@ -576,6 +578,7 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
     double cell = raw > 1.0 ? raw : 1.0;
     gs->cell_ref = cell;
     gs->doublings = 0;
+    gs->stamp += 1;
     if (gs->alive == 0) {
         gs->cell = cell;
         gs->npasses = 0;
@@ -610,18 +613,15 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
     if (gs->npasses > launched_passes) gs->passes_short = 1;
 }
 
-__device__ __forceinline__ uint64_t pack_key(const GridState& g, double x, double y, double z) {
-    const double c = g.cell;
-    const long long ix = static_cast<long long>(floor(x / c)) - g.imin[0];
-    const long long iy = static_cast<long long>(floor(y / c)) - g.imin[1];
-    const long long iz = static_cast<long long>(floor(z / c)) - g.imin[2];
-    return (static_cast<uint64_t>(ix) << (g.bits[1] + g.bits[2])) |
-           (static_cast<uint64_t>(iy) << g.bits[2]) | static_cast<uint64_t>(iz);
-}
 
+// Also writes, in object order (coalesced), each alive object's compact
+// FP32 record for the pair kernels: position relative to its cell's low
+// corner (FP64 subtraction, then rounding: the error DR_ERR_KM bounds) with
+// the object index in w, and velocity with the cell's packed z coordinate
+// in w. k_cell_write then gathers one 32-byte record per sorted object.
 __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  int n, const GridState* gs, uint64_t* keys, int* idx,
-                                                 int* hist, int ntiles) {
+                                                 int* hist, int ntiles, float4* rpos, float4* rvel) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
@@ -633,11 +633,22 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midp
         if (e < n) {
             uint64_t k = DEAD_KEY;
             if (alive[e]) {
-                if (midpoint)
-                    k = pack_key(g, 0.5 * (cur.x[e] + end.x[e]), 0.5 * (cur.y[e] + end.y[e]),
-                                 0.5 * (cur.z[e] + end.z[e]));
-                else
-                    k = pack_key(g, cur.x[e], cur.y[e], cur.z[e]);
+                const double x = cur.x[e], y = cur.y[e], z = cur.z[e];
+                const double gx = midpoint ? 0.5 * (x + end.x[e]) : x;
+                const double gy = midpoint ? 0.5 * (y + end.y[e]) : y;
+                const double gz = midpoint ? 0.5 * (z + end.z[e]) : z;
+                const double c = g.cell;
+                const long long ix = static_cast<long long>(floor(gx / c));
+                const long long iy = static_cast<long long>(floor(gy / c));
+                const long long iz = static_cast<long long>(floor(gz / c));
+                const uint64_t pz = static_cast<uint64_t>(iz - g.imin[2]);
+                k = (static_cast<uint64_t>(ix - g.imin[0]) << (g.bits[1] + g.bits[2])) |
+                    (static_cast<uint64_t>(iy - g.imin[1]) << g.bits[2]) | pz;
+                rpos[e] = make_float4(static_cast<float>(x - static_cast<double>(ix) * c),
+                                      static_cast<float>(y - static_cast<double>(iy) * c),
+                                      static_cast<float>(z - static_cast<double>(iz) * c), __int_as_float(e));
+                rvel[e] = make_float4(static_cast<float>(cur.vx[e]), static_cast<float>(cur.vy[e]),
+                                      static_cast<float>(cur.vz[e]), __int_as_float(static_cast<int>(pz)));
             }
             keys[e] = k;
             idx[e] = e;
@@ -803,72 +814,65 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
     if (threadIdx.x == 0) gs->ncells = static_cast<unsigned int>(carry);
 }
 
-// Compact per-object record for the pair kernel, in sorted order.
-//   pos: position relative to its cell's low corner (FP64 subtraction,
-//        rounded to float: the error DR_ERR_KM bounds), w = object index bits.
-//   vel: velocity rounded to float (DV_ERR_KM_S), w = the cell's packed z
-//        coordinate (int bits), from which pairs get their z cell offset.
+// Cell table and compact records in sorted order. Thread t of tile b takes
+// the ITEMS consecutive sorted positions b * TILE + t * ITEMS + r; one block
+// scan of the per-thread head counts numbers the cells.
+//   lpos: position relative to its cell's low corner, w = object index bits;
+//   lvel: velocity, w = the cell's packed z coordinate (int bits), from
+//         which pairs get their z cell offset.
 __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
-                                                       Planes cur, uint64_t* cell_key,
+                                                       const float4* __restrict__ rpos,
+                                                       const float4* __restrict__ rvel, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lpos, float4* lvel) {
-    __shared__ int s_wcnt[WAVES_PER_BLOCK];
-    __shared__ int s_run;
+    __shared__ int s_wave[WAVES_PER_BLOCK];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
     const int base = blockIdx.x * TILE;
     if (base >= A || g.npasses == 0) return;
     const uint64_t* keys = sorted_keys(g, k0, k1);
     const int* sidx = sorted_idx(g, i0, i1);
-    const int wave = threadIdx.x / WAVE;
-    const unsigned long long below = lanes_below();
-    const double cell = g.cell;
-    const int sy = g.bits[2], sx = g.bits[1] + g.bits[2];
-    const uint64_t mz = (1ULL << g.bits[2]) - 1ULL, my = (1ULL << g.bits[1]) - 1ULL;
-    if (threadIdx.x == 0) s_run = boff[blockIdx.x];
-    __syncthreads();
+    const int e0 = base + threadIdx.x * ITEMS;
+    uint64_t key[ITEMS];
+    int id[ITEMS];
+    bool head[ITEMS];
+    uint64_t prev = e0 > 0 && e0 - 1 < A ? keys[e0 - 1] : ~0ULL;
+    int cnt = 0;
+#pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + threadIdx.x;
+        const int e = e0 + r;
         const bool valid = e < A;
-        uint64_t key = 0;
-        bool head = false;
-        if (valid) {
-            key = keys[e];
-            head = e == 0 || key != keys[e - 1];
+        key[r] = valid ? keys[e] : 0ULL;
+        id[r] = valid ? sidx[e] : 0;
+        head[r] = valid && (e == 0 || key[r] != prev);
+        prev = key[r];
+        cnt += head[r] ? 1 : 0;
+    }
+    float4 p[ITEMS], v[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        if (e0 + r < A) {
+            p[r] = rpos[id[r]];
+            v[r] = rvel[id[r]];
         }
-        const unsigned long long hm = __ballot(head);
-        if (__lane_id() == 0) s_wcnt[wave] = __popcll(hm);
-        __syncthreads();
-        int pre = s_run;
-        for (int w = 0; w < wave; ++w) pre += s_wcnt[w];
-        if (valid) {
-            const int c = pre + __popcll(hm & below) + (head ? 1 : 0) - 1;
-            cell_of[e] = c;
-            if (head) {
-                cell_key[c] = key;
-                cell_begin[c] = e;
-            }
-            const int id = sidx[e];
-            const long long ix = static_cast<long long>(key >> sx) + g.imin[0];
-            const long long iy = static_cast<long long>((key >> sy) & my) + g.imin[1];
-            const long long iz = static_cast<long long>(key & mz) + g.imin[2];
-            lpos[e] = make_float4(static_cast<float>(cur.x[id] - static_cast<double>(ix) * cell),
-                                  static_cast<float>(cur.y[id] - static_cast<double>(iy) * cell),
-                                  static_cast<float>(cur.z[id] - static_cast<double>(iz) * cell),
-                                  __int_as_float(id));
-            lvel[e] = make_float4(static_cast<float>(cur.vx[id]), static_cast<float>(cur.vy[id]),
-                                  static_cast<float>(cur.vz[id]),
-                                  __int_as_float(static_cast<int>(key & mz)));
+    }
+    int tot;
+    const int ex = block_exclusive_scan(cnt, tot, s_wave);
+    int c = boff[blockIdx.x] + ex - 1;
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = e0 + r;
+        if (e >= A) break;
+        if (head[r]) {
+            ++c;
+            cell_key[c] = key[r];
+            cell_begin[c] = e;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int add = 0;
-            for (int w = 0; w < WAVES_PER_BLOCK; ++w) add += s_wcnt[w];
-            s_run += add;
-        }
-        __syncthreads();
+        cell_of[e] = c;
+        lpos[e] = p[r];
+        lvel[e] = v[r];
     }
 }
 
@@ -936,10 +940,33 @@ __device__ __forceinline__ int dense_items(int b, int k, const int2* rng) {
     return items;
 }
 
+// Column table: for every occupied (x, y) column of the packed key space,
+// the run [begin, end) of its cells in the sorted cell table, tagged with
+// the screening's stamp (entries of earlier screenings read as empty, so
+// the table is never cleared). Used when the column index fits COL_BITS.
+constexpr int COL_BITS = 22;
+
+__global__ void __launch_bounds__(BLOCK) k_columns(const GridState* gs, const uint64_t* cell_key, int4* coltab) {
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int nc = static_cast<int>(gs->ncells);
+    if (c >= nc || gs->bits[0] + gs->bits[1] > COL_BITS) return;
+    const int bz = gs->bits[2];
+    const uint64_t col = cell_key[c] >> bz;
+    const bool first = c == 0 || (cell_key[c - 1] >> bz) != col;
+    const bool last = c + 1 == nc || (cell_key[c + 1] >> bz) != col;
+    int4* e = coltab + col;
+    if (first) {
+        e->x = gs->stamp;
+        e->y = c;
+    }
+    if (last) e->z = c + 1;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint64_t* cell_key,
                                                        const int* cell_begin, int* cell_count,
                                                        int2* ranges, PairPartial* parts,
-                                                       DenseCell* dense, unsigned int item_cap) {
+                                                       DenseCell* dense, unsigned int item_cap,
+                                                       const int4* __restrict__ coltab) {
     __shared__ unsigned int s_max[WAVES_PER_BLOCK];
     __shared__ int s_wave[WAVES_PER_BLOCK];
     __shared__ int s_base[2];
@@ -954,6 +981,8 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         cnt = static_cast<unsigned int>(e - b);
         cell_count[c] = static_cast<int>(cnt);
         const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
+        const bool use_tab = bx + by <= COL_BITS;
+        const int stamp = gs->stamp;
         const uint64_t key = cell_key[c];
         const uint64_t mz = (1ULL << bz) - 1ULL;
         const long long x = static_cast<long long>(key >> (by + bz));
@@ -975,9 +1004,22 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
             if (axis_fits(nx, bx) && axis_fits(ny, by)) {
                 const uint64_t row = (static_cast<uint64_t>(nx) << (by + bz)) |
                                      (static_cast<uint64_t>(ny) << bz);
-                const int i0 = lower_bound_u64(cell_key, nc, row | static_cast<uint64_t>(zlo));
-                const int i1 = lower_bound_u64(cell_key, nc, (row | static_cast<uint64_t>(zhi)) + 1ULL);
-                if (i0 < i1) run = make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A);
+                int lo = 0, hi = nc;
+                if (use_tab) {
+                    const int4 t = coltab[row >> bz];
+                    if (t.x == stamp) {
+                        lo = t.y;
+                        hi = t.z;
+                    } else {
+                        hi = 0;
+                    }
+                }
+                if (lo < hi) {
+                    const int i0 = lo + lower_bound_u64(cell_key + lo, hi - lo, row | static_cast<uint64_t>(zlo));
+                    const int i1 =
+                        lo + lower_bound_u64(cell_key + lo, hi - lo, (row | static_cast<uint64_t>(zhi)) + 1ULL);
+                    if (i0 < i1) run = make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A);
+                }
             }
             rng[r] = run;
         }
@@ -1570,6 +1612,9 @@ struct khip_ctx {
     int2* ranges = nullptr;
     float4* lpos = nullptr;
     float4* lvel = nullptr;
+    int4* coltab = nullptr;  // column table, 1 << COL_BITS entries
+    float4* rpos = nullptr;  // the same records in object order (written by k_keys)
+    float4* rvel = nullptr;
     GridState* gs = nullptr;
     StepPartial* step_parts = nullptr;
     PairPartial* pair_parts = nullptr;
@@ -1673,7 +1718,8 @@ void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->snap_state,
+                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab,
+                    ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
                     ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
                     ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
@@ -1714,6 +1760,7 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
            dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
+           dev_alloc(ctx, &ctx->rpos, cap) && dev_alloc(ctx, &ctx->rvel, cap) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
            dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
@@ -1807,7 +1854,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     hipStream_t s = ctx->stream;
     k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint);
     k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
-                                   ctx->hist, tiles);
+                                   ctx->hist, tiles, ctx->rpos, ctx->rvel);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         if (pass > 0)
@@ -1820,12 +1867,13 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_cell_count<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
     k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
     k_cell_write<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
-                                         ctx->gs, ctx->bcount, pb, ctx->cell_key, ctx->cell_begin,
-                                         ctx->cell_of, ctx->lpos, ctx->lvel);
+                                         ctx->gs, ctx->bcount, ctx->rpos, ctx->rvel, ctx->cell_key,
+                                         ctx->cell_begin, ctx->cell_of, ctx->lpos, ctx->lvel);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
+    k_columns<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->coltab);
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
                                               ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
-                                              icap);
+                                              icap, ctx->coltab);
     k_items<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->dense, ctx->cell_begin, ctx->cell_count,
                                            ctx->ranges, ctx->items, icap);
     mark(ctx, 2);
@@ -2670,7 +2718,10 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
                   prop.multiProcessorCount, prop.totalGlobalMem / 1e9);
     ctx->name = buf;
     bool ok = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipMalloc(reinterpret_cast<void**>(&ctx->gs), sizeof(GridState)) == hipSuccess;
+              hipMalloc(reinterpret_cast<void**>(&ctx->gs), sizeof(GridState)) == hipSuccess &&
+              hipMemset(ctx->gs, 0, sizeof(GridState)) == hipSuccess &&
+              hipMalloc(reinterpret_cast<void**>(&ctx->coltab), sizeof(int4) << COL_BITS) == hipSuccess &&
+              hipMemset(ctx->coltab, 0, sizeof(int4) << COL_BITS) == hipSuccess;
     if (ok) {
         k_reset<<<1, 64, 0, ctx->stream>>>(ctx->gs);
         k_zero_events<<<1, 64, 0, ctx->stream>>>(ctx->gs);
