@@ -119,6 +119,13 @@ const char* khip_last_error(const khip_ctx* ctx);
 int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
                      const double* size_m);
 int64_t khip_object_count(const khip_ctx* ctx);
+/* Splits the uploaded population into nscen independent scenarios: objects
+ * [offsets[k], offsets[k + 1]) form scenario k (offsets[0] = 0,
+ * offsets[nscen] = n). Screening pairs objects of the same scenario only;
+ * everything else runs over the concatenated arrays in the same launches.
+ * khip_set_objects resets to one scenario. No reference counterpart (the
+ * reference runs one population per process). */
+int khip_set_scenarios(khip_ctx* ctx, int32_t nscen, const int64_t* offsets);
 /* Reads back states (n x 6) and alive flags; either pointer may be NULL. */
 int khip_get_objects(khip_ctx* ctx, double* states, uint8_t* alive);
 
@@ -204,6 +211,35 @@ typedef struct {
 } khip_learned_report;
 
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);
+/* Learned form: when enabled (and timing is off), each chunk of steps is
+ * captured once into a HIP graph and replayed for every later chunk of the
+ * same shape (length, check pattern, radix passes, buffers); per-step times
+ * and indices come from a device table, so replays need no re-capture.
+ * No reference counterpart. */
+int khip_set_graph(khip_ctx* ctx, int enabled);
+
+/* Launch and tiling parameters of the screening kernels. None changes which
+ * conjunctions are found; cell_mult > 1 enlarges the grid cell beyond the
+ * reference's (larger cells never change results, collision.cpp:337-338)
+ * and so changes the candidate counts in the statistics. */
+typedef struct {
+    double cell_mult;     /* >= 1 (default 1: the reference's cell) */
+    int32_t dense_min;    /* cells with at least this many objects use work items (32) */
+    int32_t long_range;   /* sparse-cell ranges longer than this use work items (64) */
+    int32_t seg;          /* partners per work item (256) */
+    int32_t dense_blocks; /* persistent grid of the dense-cell pair kernel (1024, <= 4096) */
+    int32_t exact_blocks; /* grid of the exact FP64 kernel (512, <= 2048) */
+    int32_t pad;
+} khip_tuning;
+int khip_set_tuning(khip_ctx* ctx, const khip_tuning* t);
+int khip_get_tuning(const khip_ctx* ctx, khip_tuning* t);
+/* Autotuner: times `steps` screened steps of the current population and
+ * configuration for candidate tunings (coordinate descent over the fields;
+ * cell_mult only if allow_cell), each from the same saved state, keeps the
+ * fastest and restores the population, time, events and statistics. */
+int khip_autotune(khip_ctx* ctx, int64_t steps, int allow_cell, khip_tuning* best, double* base_ms,
+                  double* best_ms);
+int khip_get_graph_counts(const khip_ctx* ctx, int64_t* captures, int64_t* launches);
 /* Tallies since the last khip_set_objects. No reference counterpart. */
 int khip_get_learned_report(const khip_ctx* ctx, khip_learned_report* out);
 

@@ -42,7 +42,8 @@ ABI_SYMBOLS = (
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
     "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous", "khip_set_precision",
     "khip_net_train", "khip_net_set", "khip_net_get", "khip_net_infer", "khip_net_gradient",
-    "khip_set_learned", "khip_get_learned_report",
+    "khip_set_learned", "khip_get_learned_report", "khip_set_scenarios", "khip_set_graph",
+    "khip_get_graph_counts", "khip_set_tuning", "khip_get_tuning", "khip_autotune",
 )
 PRECISION_LEARNED = 2
 NET_PARAMS = 35206
@@ -88,6 +89,11 @@ LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", 
 class TrainReport(C.Structure):
     _fields_ = [("samples", C.c_int64), ("loss_first", _D), ("loss_last", _D), ("gen_ms", _D),
                 ("train_ms", _D)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("cell_mult", _D), ("dense_min", C.c_int32), ("long_range", C.c_int32), ("seg", C.c_int32),
+                ("dense_blocks", C.c_int32), ("exact_blocks", C.c_int32), ("pad", C.c_int32)]
 
 
 class ScreenConfig(C.Structure):
@@ -139,6 +145,12 @@ def library() -> C.CDLL:
         "khip_net_gradient": (_I, [_P, _L, _P, _P, _P, _P]),
         "khip_set_learned": (_I, [_P, C.POINTER(LearnedConfig)]),
         "khip_get_learned_report": (_I, [_P, _P]),
+        "khip_set_scenarios": (_I, [_P, C.c_int32, _P]),
+        "khip_set_graph": (_I, [_P, _I]),
+        "khip_get_graph_counts": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
+        "khip_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+        "khip_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
+        "khip_autotune": (_I, [_P, _L, _I, C.POINTER(Tuning), C.POINTER(_D), C.POINTER(_D)]),
         "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
         "khost_impacts": (_L, [_L, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P]),
         "khost_cascade": (_L, [_L, C.c_uint64, _I, _D, _D, _D, _D, _D, _L, _P]),
@@ -336,6 +348,39 @@ class Engine:
                     a_max_km=12000.0):
         cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km)
         self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")
+
+    def set_scenarios(self, offsets):
+        """Objects [offsets[k], offsets[k+1]) form scenario k (screened separately)."""
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        self._check(self.lib.khip_set_scenarios(self.h, len(off) - 1, _ptr(off)), "khip_set_scenarios")
+
+    def set_graph(self, on: bool):
+        self._check(self.lib.khip_set_graph(self.h, 1 if on else 0), "khip_set_graph")
+
+    def graph_counts(self):
+        c, l = _L(), _L()
+        self._check(self.lib.khip_get_graph_counts(self.h, C.byref(c), C.byref(l)), "khip_get_graph_counts")
+        return c.value, l.value
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._check(self.lib.khip_get_tuning(self.h, C.byref(t)), "khip_get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad"}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = Tuning(cur["cell_mult"], cur["dense_min"], cur["long_range"], cur["seg"], cur["dense_blocks"],
+                   cur["exact_blocks"], 0)
+        self._check(self.lib.khip_set_tuning(self.h, C.byref(t)), "khip_set_tuning")
+
+    def autotune(self, steps=4, allow_cell=False):
+        """Returns (best tuning dict, base ms, best ms) for `steps` screened steps."""
+        t = Tuning()
+        b0, b1 = _D(), _D()
+        self._check(self.lib.khip_autotune(self.h, int(steps), 1 if allow_cell else 0, C.byref(t), C.byref(b0),
+                                           C.byref(b1)), "khip_autotune")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad"}, b0.value, b1.value
 
     def learned_report(self):
         out = np.zeros(1, dtype=LEARNED_REPORT_DTYPE)

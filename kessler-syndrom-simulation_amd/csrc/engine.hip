@@ -40,6 +40,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,7 +66,7 @@ constexpr int MAX_PASSES = 8;
 // Cells with at least DENSE_MIN objects (fresh fragment clouds, crowded
 // shells) are screened by work items: CHUNK own objects (one per lane)
 // against up to SEG partners read by the whole wave at once.
-constexpr int DENSE_MIN = 32;
+constexpr int DENSE_MIN = 32;  // defaults; the tuned values live in GridState
 // A sparse cell's range longer than LONG_RANGE (a neighbouring cloud) is
 // also handed to the work items instead of being walked lane by lane.
 constexpr int LONG_RANGE = 64;
@@ -73,6 +74,16 @@ constexpr int CHUNK = 64;
 constexpr int SEG = 256;
 constexpr int DENSE_BLOCKS = 1024;
 constexpr int EXACT_BLOCKS = 512;
+constexpr int MAX_DENSE_BLOCKS = 4096;  // partial-record capacity for tuned grids
+constexpr int MAX_EXACT_BLOCKS = 2048;
+
+// Launch and tiling parameters the autotuner chooses. None of them changes
+// which pairs are found; cell_mult > 1 (larger cells, collision.cpp:337-338)
+// changes the candidate count only.
+struct TileCfg {
+    double cell_mult;
+    int dense_min, long_range, seg, dense_blocks, exact_blocks, pad;
+};
 
 // Device-resident grid parameters and per-step accumulators.
 struct GridState {
@@ -87,6 +98,8 @@ struct GridState {
     double cell_ref;                // the reference's cell size before any doubling
     long long imin[3];              // smallest cell coordinate per axis
     int bits[3];                    // packed bits per axis (x, y, z)
+    int sbits;                      // scenario-id bits above x (0: one scenario)
+    int dense_min, long_range, seg; // pair tiling of this screening (TileCfg)
     int npasses;                    // radix passes needed (0: nothing alive)
     int doublings;                  // times the cell was doubled to fit the key
     unsigned int ncells;
@@ -559,7 +572,8 @@ __device__ __forceinline__ int bits_for(long long range) {
 // and the 13 forward neighbours enumerate every such pair.
 __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
-                                                       int launched_passes, int midpoint) {
+                                                       int launched_passes, int midpoint, int sbits,
+                                                       TileCfg tc) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     StepPartial acc = empty_step_partial();
     for (int b = threadIdx.x; b < nparts; b += BLOCK) combine(acc, partials[b]);
@@ -577,6 +591,10 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
     const double raw = vrel_max * dt + thr + GRAVITY_GRADIENT * vrel_max * dt * dt * dt;
     double cell = raw > 1.0 ? raw : 1.0;
     gs->cell_ref = cell;
+    cell = cell * tc.cell_mult;
+    gs->dense_min = tc.dense_min;
+    gs->long_range = tc.long_range;
+    gs->seg = tc.seg;
     gs->doublings = 0;
     gs->stamp += 1;
     if (gs->alive == 0) {
@@ -592,8 +610,11 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
     // extreme coordinates. A cell larger than the reference's never changes
     // which pairs pass the filters (collision.cpp:337-338); it is doubled
     // only if the packed key would not fit.
+    // Concatenated scenarios: the scenario id is the key's top field, so
+    // cells (and the 13 forward neighbours) never mix scenarios.
+    gs->sbits = sbits;
     for (;;) {
-        int total = 0;
+        int total = sbits;
         for (int a = 0; a < 3; ++a) {
             const long long i0 = static_cast<long long>(floor(lo[a] / cell));
             const long long i1 = static_cast<long long>(floor(hi[a] / cell));
@@ -620,8 +641,9 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
 // the object index in w, and velocity with the cell's packed z coordinate
 // in w. k_cell_write then gathers one 32-byte record per sorted object.
 __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
-                                                 int n, const GridState* gs, uint64_t* keys, int* idx,
-                                                 int* hist, int ntiles, float4* rpos, float4* rvel) {
+                                                 const int* __restrict__ scen, int n, const GridState* gs,
+                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rpos,
+                                                 float4* rvel) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
@@ -644,6 +666,8 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midp
                 const uint64_t pz = static_cast<uint64_t>(iz - g.imin[2]);
                 k = (static_cast<uint64_t>(ix - g.imin[0]) << (g.bits[1] + g.bits[2])) |
                     (static_cast<uint64_t>(iy - g.imin[1]) << g.bits[2]) | pz;
+                if (g.sbits)
+                    k |= static_cast<uint64_t>(scen[e]) << (g.bits[0] + g.bits[1] + g.bits[2]);
                 rpos[e] = make_float4(static_cast<float>(x - static_cast<double>(ix) * c),
                                       static_cast<float>(y - static_cast<double>(iy) * c),
                                       static_cast<float>(z - static_cast<double>(iz) * c), __int_as_float(e));
@@ -916,27 +940,27 @@ __device__ __forceinline__ int lower_bound_u64(const uint64_t* ck, int n, uint64
     return lo;
 }
 
-__device__ __forceinline__ int segments(int n) { return (n + SEG - 1) / SEG; }
+__device__ __forceinline__ int segments(int n, int seg) { return (n + seg - 1) / seg; }
 
 // Dense cell with k objects at sorted positions [b, b + k): per chunk of
 // CHUNK own objects, the range-0 tail from the chunk start plus ranges 1-4,
 // each cut into SEG-long segments.
 // Items of a sparse cell (one chunk, k < DENSE_MIN): its ranges longer than
 // LONG_RANGE objects (range 0 counted from the cell start).
-__device__ __forceinline__ int long_range_items(const int2* rng) {
+__device__ __forceinline__ int long_range_items(const int2* rng, int long_range, int seg) {
     int items = 0;
     for (int r = 0; r < NRANGES; ++r) {
         const int len = rng[r].y - rng[r].x;
-        if (len > LONG_RANGE) items += segments(len);
+        if (len > long_range) items += segments(len, seg);
     }
     return items;
 }
 
-__device__ __forceinline__ int dense_items(int b, int k, const int2* rng) {
+__device__ __forceinline__ int dense_items(int b, int k, const int2* rng, int seg) {
     int fixed = 0;
-    for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x);
+    for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x, seg);
     int items = 0;
-    for (int i = 0; i * CHUNK < k; ++i) items += segments(rng[0].y - (b + i * CHUNK)) + fixed;
+    for (int i = 0; i * CHUNK < k; ++i) items += segments(rng[0].y - (b + i * CHUNK), seg) + fixed;
     return items;
 }
 
@@ -944,12 +968,12 @@ __device__ __forceinline__ int dense_items(int b, int k, const int2* rng) {
 // the run [begin, end) of its cells in the sorted cell table, tagged with
 // the screening's stamp (entries of earlier screenings read as empty, so
 // the table is never cleared). Used when the column index fits COL_BITS.
-constexpr int COL_BITS = 22;
+constexpr int COL_BITS = 24;
 
 __global__ void __launch_bounds__(BLOCK) k_columns(const GridState* gs, const uint64_t* cell_key, int4* coltab) {
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     const int nc = static_cast<int>(gs->ncells);
-    if (c >= nc || gs->bits[0] + gs->bits[1] > COL_BITS) return;
+    if (c >= nc || gs->sbits + gs->bits[0] + gs->bits[1] > COL_BITS) return;
     const int bz = gs->bits[2];
     const uint64_t col = cell_key[c] >> bz;
     const bool first = c == 0 || (cell_key[c - 1] >> bz) != col;
@@ -981,11 +1005,12 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         cnt = static_cast<unsigned int>(e - b);
         cell_count[c] = static_cast<int>(cnt);
         const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
-        const bool use_tab = bx + by <= COL_BITS;
+        const bool use_tab = gs->sbits + bx + by <= COL_BITS;
         const int stamp = gs->stamp;
         const uint64_t key = cell_key[c];
         const uint64_t mz = (1ULL << bz) - 1ULL;
-        const long long x = static_cast<long long>(key >> (by + bz));
+        const uint64_t scen_part = gs->sbits ? (key >> (bx + by + bz)) << (bx + by + bz) : 0ULL;
+        const long long x = static_cast<long long>((key >> (by + bz)) & ((1ULL << bx) - 1ULL));
         const long long y = static_cast<long long>((key >> bz) & ((1ULL << by) - 1ULL));
         const long long z = static_cast<long long>(key & mz);
         int2 rng[NRANGES];
@@ -1002,7 +1027,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
             const long long nx = x + dx, ny = y + dy;
             int2 run = make_int2(0, 0);
             if (axis_fits(nx, bx) && axis_fits(ny, by)) {
-                const uint64_t row = (static_cast<uint64_t>(nx) << (by + bz)) |
+                const uint64_t row = scen_part | (static_cast<uint64_t>(nx) << (by + bz)) |
                                      (static_cast<uint64_t>(ny) << bz);
                 int lo = 0, hi = nc;
                 if (use_tab) {
@@ -1024,8 +1049,8 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
             rng[r] = run;
         }
         for (int r = 0; r < NRANGES; ++r) ranges[c * NRANGES + r] = rng[r];
-        items = static_cast<int>(cnt) >= DENSE_MIN ? dense_items(b, static_cast<int>(cnt), rng)
-                                                   : long_range_items(rng);
+        items = static_cast<int>(cnt) >= gs->dense_min ? dense_items(b, static_cast<int>(cnt), rng, gs->seg)
+                                                       : long_range_items(rng, gs->long_range, gs->seg);
     }
     // Block-aggregated registration of dense cells: two atomics per block.
     int tot_items, tot_dense;
@@ -1058,6 +1083,7 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
                                                   const int2* ranges, PairItem* items,
                                                   unsigned int item_cap) {
     const int nd = static_cast<int>(gs->n_dense);
+    const int seg = gs->seg, long_range = gs->long_range, dmin = gs->dense_min;
     if (blockIdx.x == 0 && threadIdx.x == 0 && gs->n_items > gs->items_max) gs->items_max = gs->n_items;
     for (int d = blockIdx.x; d < nd; d += gridDim.x) {
         const DenseCell dc = dense[d];
@@ -1066,34 +1092,34 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
         const int k = cell_count[c];
         int2 rng[NRANGES];
         for (int r = 0; r < NRANGES; ++r) rng[r] = ranges[c * NRANGES + r];
-        if (k < DENSE_MIN) {
+        if (k < dmin) {
             // Sparse cell: only its long ranges, own objects = the whole cell.
             if (threadIdx.x != 0) continue;
             int off = dc.item_base;
             for (int r = 0; r < NRANGES; ++r) {
                 const int len = rng[r].y - rng[r].x;
-                if (len <= LONG_RANGE) continue;
-                for (int sgi = 0; sgi * SEG < len; ++sgi, ++off) {
+                if (len <= long_range) continue;
+                for (int sgi = 0; sgi * seg < len; ++sgi, ++off) {
                     if (static_cast<unsigned int>(off) >= item_cap) continue;
-                    items[off] = PairItem{b, k, rng[r].x + sgi * SEG, min(SEG, len - sgi * SEG), r, 0, 0, 0};
+                    items[off] = PairItem{b, k, rng[r].x + sgi * seg, min(seg, len - sgi * seg), r, 0, 0, 0};
                 }
             }
             continue;
         }
         int fixed = 0;
-        for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x);
+        for (int r = 1; r < NRANGES; ++r) fixed += segments(rng[r].y - rng[r].x, seg);
         const int nch = (k + CHUNK - 1) / CHUNK;
         for (int i = threadIdx.x; i < nch; i += BLOCK) {
             int off = dc.item_base + i * fixed;
-            for (int j = 0; j < i; ++j) off += segments(rng[0].y - (b + j * CHUNK));
+            for (int j = 0; j < i; ++j) off += segments(rng[0].y - (b + j * CHUNK), seg);
             const int own_begin = b + i * CHUNK;
             const int own_count = min(CHUNK, k - i * CHUNK);
             for (int r = 0; r < NRANGES; ++r) {
                 const int r0 = r == 0 ? own_begin : rng[r].x;
                 const int len = rng[r].y - r0;
-                for (int sgi = 0; sgi * SEG < len; ++sgi, ++off) {
+                for (int sgi = 0; sgi * seg < len; ++sgi, ++off) {
                     if (static_cast<unsigned int>(off) >= item_cap) continue;
-                    items[off] = PairItem{own_begin, own_count, r0 + sgi * SEG, min(SEG, len - sgi * SEG),
+                    items[off] = PairItem{own_begin, own_count, r0 + sgi * seg, min(seg, len - sgi * seg),
                                           r, 0, 0, 0};
                 }
             }
@@ -1104,6 +1130,13 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
 // ---------------------------------------------------------------------------
 // Pairs
 // ---------------------------------------------------------------------------
+
+// Time and index of one learned step, read by the kernels from a per-chunk
+// device table so that a captured chunk graph replays for later chunks.
+struct StepInfo {
+    double t;         // start of the step (s since epoch)
+    long long macro;  // global learned-step index
+};
 
 struct PairParams {
     Planes before, after;
@@ -1120,6 +1153,8 @@ struct PairParams {
     unsigned long long surv_cap;
     long long step;
     double t0, dt, thr, vmin2;
+    const StepInfo* info;   // learned form: step and t0 from here (sub-step q of nsub)
+    int sub_q, sub_n;
     float dt_f, thr_f, vmin2_f;
     float gdt3_f;   // GRAVITY_GRADIENT * dt^3
     float base_f;   // threshold + 1e-3 + DR_ERR_KM + DV_ERR_KM_S * dt
@@ -1174,8 +1209,14 @@ __device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, i
     DevEvent ev;
     ev.i = i;
     ev.j = j;
-    ev.step = P.step;
-    ev.t = P.t0 + s * dt;
+    long long step = P.step;
+    double t0 = P.t0;
+    if (P.info) {
+        step = P.info->macro * P.sub_n + P.sub_q;
+        t0 = P.info->t + static_cast<double>(P.sub_q) * dt;
+    }
+    ev.step = step;
+    ev.t = t0 + s * dt;
     ev.miss_km = miss;
     ev.v_rel_km_s = sqrt(w.x * w.x + w.y * w.y + w.z * w.z) / dt;
     ev.collision = miss < P.radius[i] + P.radius[j] ? 1 : 0;
@@ -1247,12 +1288,13 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
     unsigned int cand = 0;
     if (p < A && gs->npasses > 0) {
         const int c = P.cell_of[p];
-        if (P.cell_count[c] < DENSE_MIN) {
+        const int long_range = gs->long_range;
+        if (P.cell_count[c] < gs->dense_min) {
             const float cellf = static_cast<float>(gs->cell);
             const float4 la = P.lpos[p], va = P.lvel[p];
             for (int r = 0; r < NRANGES; ++r) {
                 const int2 run = P.ranges[c * NRANGES + r];
-                if (run.y - run.x > LONG_RANGE) continue;  // a work item covers it
+                if (run.y - run.x > long_range) continue;  // a work item covers it
                 int dx, dy;
                 range_offset(r, dx, dy);
                 const float ox = static_cast<float>(dx) * cellf, oy = static_cast<float>(dy) * cellf;
@@ -1442,9 +1484,11 @@ __device__ __forceinline__ void plain_at(Planes s0, int i, const MoonSample* m, 
 
 __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* events, unsigned long long cap,
                                                      Planes s0, const double* radius,
-                                                     const MoonSample* __restrict__ moon13, double t_step,
-                                                     long long sub0, double h, int sub, double thr, double vmin2,
+                                                     const MoonSample* __restrict__ moon13, const StepInfo* info,
+                                                     double h, int sub, double thr, double vmin2,
                                                      ReverifyStats* rs) {
+    const double t_step = info->t;
+    const long long sub0 = info->macro * sub;
     const unsigned long long e0 = gs->ev_mark;
     const unsigned long long e1 = gs->events < cap ? gs->events : cap;
     unsigned long long ver = 0, drop = 0, kept = 0;
@@ -1495,6 +1539,14 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
         if (ver) atomicAdd(&rs->verified, ver);
         if (drop) atomicAdd(&rs->dropped, drop);
         if (kept) atomicAdd(&rs->kept_unverified, kept);
+    }
+}
+
+__global__ void k_zero_reverify(ReverifyStats* rs) {
+    if (threadIdx.x == 0) {
+        rs->verified = 0;
+        rs->dropped = 0;
+        rs->kept_unverified = 0;
     }
 }
 
@@ -1612,7 +1664,11 @@ struct khip_ctx {
     int2* ranges = nullptr;
     float4* lpos = nullptr;
     float4* lvel = nullptr;
+    TileCfg tile{1.0, DENSE_MIN, LONG_RANGE, SEG, DENSE_BLOCKS, EXACT_BLOCKS, 0};
     int4* coltab = nullptr;  // column table, 1 << COL_BITS entries
+    int* scen = nullptr;     // scenario id per object (concatenated scenarios)
+    int sbits = 0;
+    std::vector<int64_t> scen_off;
     float4* rpos = nullptr;  // the same records in object order (written by k_keys)
     float4* rvel = nullptr;
     GridState* gs = nullptr;
@@ -1670,6 +1726,16 @@ struct khip_ctx {
     bool force_check = true;
     khip_learned_report lrep{};
     int n_cus = 256;
+
+    // Learned-form step table and the captured chunk graph.
+    StepInfo* steptab = nullptr;
+    int64_t steptab_cap = 0;
+    bool use_graph = false;
+    uint64_t gen = 1;               // bumped by every (re)allocation and configuration change
+    // Instantiated chunk graphs keyed by what they were captured for (most
+    // recently used last; at most kGraphCache).
+    std::vector<std::pair<std::vector<int64_t>, hipGraphExec_t>> graphs;
+    int64_t graph_launches = 0, graph_captures = 0;
     int shrink_votes = 0;  // consecutive chunks that needed fewer passes
     int64_t last_rerun = 0;  // reasons of the last re-run (KHIP_RERUN_* bits)
 
@@ -1705,6 +1771,7 @@ bool fail(khip_ctx* ctx, const std::string& what, hipError_t e) {
 
 template <typename T>
 bool dev_alloc(khip_ctx* ctx, T** p, size_t count) {
+    ctx->gen += 1;  // pointers baked into a captured graph change
     if (*p) {
         (void)hipFree(*p);
         *p = nullptr;
@@ -1714,11 +1781,26 @@ bool dev_alloc(khip_ctx* ctx, T** p, size_t count) {
     return true;
 }
 
+// Temporary device buffers of one call (freed when it returns).
+struct TrainScratch {
+    std::vector<void*> ptrs;
+    template <typename T>
+    bool get(khip_ctx* ctx, T** p, size_t n) {
+        *p = nullptr;
+        KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)));
+        ptrs.push_back(*p);
+        return true;
+    }
+    ~TrainScratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
 void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab,
+                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab, ctx->scen, ctx->steptab,
                     ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
                     ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
@@ -1765,8 +1847,8 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
            dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
            dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) &&
-           dev_alloc(ctx, &ctx->dense_parts, DENSE_BLOCKS) &&
-           dev_alloc(ctx, &ctx->exact_parts, EXACT_BLOCKS) &&
+           dev_alloc(ctx, &ctx->dense_parts, MAX_DENSE_BLOCKS) &&
+           dev_alloc(ctx, &ctx->exact_parts, MAX_EXACT_BLOCKS) &&
            ensure_surv(ctx, std::max<int64_t>(ctx->surv_cap, 1 << 24)) &&
            dev_alloc(ctx, &ctx->dense, cap) && ensure_items(ctx, std::max<int64_t>(ctx->item_cap, 1 << 20));
 }
@@ -1848,13 +1930,15 @@ void mark(khip_ctx* ctx, int k) {
 // Enqueues the screening kernels for the step whose start / end states are
 // `before` / `after` (stats record `stat_slot`).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
-                    double thr, int midpoint) {
+                    double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1) {
     const int n = static_cast<int>(ctx->n);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
-    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint);
-    k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, n, ctx->gs, ctx->keys[0], ctx->idx[0],
-                                   ctx->hist, tiles, ctx->rpos, ctx->rvel);
+    const TileCfg& tc = ctx->tile;
+    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint,
+                                     ctx->sbits, tc);
+    k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
+                                   ctx->idx[0], ctx->hist, tiles, ctx->rpos, ctx->rvel);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         if (pass > 0)
@@ -1874,7 +1958,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
                                               ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
                                               icap, ctx->coltab);
-    k_items<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->dense, ctx->cell_begin, ctx->cell_count,
+    k_items<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, ctx->dense, ctx->cell_begin, ctx->cell_count,
                                            ctx->ranges, ctx->items, icap);
     mark(ctx, 2);
     PairParams P;
@@ -1893,6 +1977,9 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.surv_cap = static_cast<unsigned long long>(ctx->surv_cap);
     P.step = step;
     P.t0 = t0;
+    P.info = info;
+    P.sub_q = sub_q;
+    P.sub_n = sub_n;
     P.dt = dt;
     P.thr = thr;
     P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
@@ -1903,13 +1990,13 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
     k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     mark(ctx, 3);
-    k_pairs_dense<<<DENSE_BLOCKS, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
+    k_pairs_dense<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
                                                  ctx->dense_parts);
     mark(ctx, 4);
-    k_exact<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
+    k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
     mark(ctx, 5);
     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n),
-                                 ctx->dense_parts, DENSE_BLOCKS, ctx->exact_parts, EXACT_BLOCKS);
+                                 ctx->dense_parts, tc.dense_blocks, ctx->exact_parts, tc.exact_blocks);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
 }
@@ -2131,6 +2218,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
 // Learned form: host orchestration
 // ---------------------------------------------------------------------------
 
+constexpr size_t kGraphCache = 8;
+
 bool ensure_learned(khip_ctx* ctx) {
     const int sub = ctx->lcfg.substeps;
     if (ctx->lrn_cap >= ctx->cap && ctx->mode && ctx->dense_k >= sub - 1) return true;
@@ -2248,71 +2337,132 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         const int cur0 = ctx->cur;
         const double t_chunk = ctx->t;
         const bool force0 = ctx->force_check;
-        int64_t checks = 0;
         for (;;) {
-            k_reset<<<1, 64, 0, s>>>(ctx->gs);
-            k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
-            KHIP_TRY(ctx, hipMemsetAsync(ctx->rstats, 0, sizeof(ReverifyStats), s));
-            double t = t_chunk;
-            int cur = cur0;
+            // Step table of the chunk and the host's copy of the check pattern.
+            if (len > ctx->steptab_cap) {
+                if (!dev_alloc(ctx, &ctx->steptab, len)) return false;
+                ctx->steptab_cap = len;
+            }
+            std::vector<StepInfo> tab(static_cast<size_t>(len));
+            std::vector<int64_t> pattern;
+            int64_t checks = 0;
             bool force = force0;
-            checks = 0;
+            double t = t_chunk;
             for (int64_t k = 0; k < len; ++k) {
-                const int nxt = cur ^ 1;
-                const int64_t macro = ctx->step + done + k;
-                const MoonSample* m13 = ctx->moon + 12 * k;
-                g_slot = static_cast<int>(k * sub);
-                mark(ctx, 6);
-                const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
-                if (force || macro % L.check_every == 0) {
-                    k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
-                    k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
-                        pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
-                        ctx->net_norm, lp, ctx->cstats);
-                    k_build_lists<<<grid_of(n), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
-                                                               static_cast<int>(c), ctx->counts);
+                tab[k].t = t;
+                tab[k].macro = ctx->step + done + k;
+                const bool chk = force || tab[k].macro % L.check_every == 0;
+                pattern.push_back(chk ? 1 : 0);
+                if (chk) {
                     force = false;
                     ++checks;
-                } else {
-                    k_learned<net::CLASS_LEARNED><<<g_net, BLOCK, 0, s>>>(
-                        pc, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
-                        ctx->net_norm, lp, ctx->cstats);
-                    k_learned<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(
-                        pc, pn, ctx->alive, ctx->mode, ctx->lists + 2 * c, ctx->counts, n, m13, ctx->net_ffwd,
-                        ctx->net_p, ctx->net_norm, lp, ctx->cstats);
-                    k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
-                        pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
-                        ctx->net_p, ctx->net_norm, lp, ctx->cstats);
-                }
-                mark(ctx, 7);
-                if (screen) {
-                    k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
-                    for (int q = 0; q < sub; ++q) {
-                        g_slot = static_cast<int>(k * sub + q);
-                        mark(ctx, 1);
-                        const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
-                        if (q < sub - 1)
-                            k_dense_output<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, n, ctx->dt,
-                                                                        static_cast<double>(q + 1) / sub,
-                                                                        dense_planes(ctx, q));
-                        const Planes pa = q == sub - 1 ? pn : dense_planes(ctx, q);
-                        k_reduce_mid<<<grid_of(n), BLOCK, 0, s>>>(pb, pa, ctx->alive, n, ctx->step_parts);
-                        if (!enqueue_screen(ctx, pb, pa, t + q * h, macro * sub + q, static_cast<int>(k * sub + q), h,
-                                            thr, 1))
-                            return false;
-                    }
-                    k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
-                                                              static_cast<unsigned long long>(ctx->ev_cap), pc,
-                                                              ctx->radius, m13, t, macro * sub, h, sub, thr, vmin2,
-                                                              ctx->rstats);
-                    mark(ctx, 0);
-                } else {
-                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, ctx->dense_parts, 0,
-                                                 ctx->exact_parts, 0);
                 }
                 t += ctx->dt;
-                cur = nxt;
             }
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->steptab, tab.data(), sizeof(StepInfo) * len, hipMemcpyHostToDevice, s));
+            const int cur_end = (len & 1) ? cur0 ^ 1 : cur0;
+            // The chunk's launches: plain kernel arguments only (pointers,
+            // sizes, configuration); the per-step time and index come from
+            // the step table, so the same sequence serves every chunk of the
+            // same shape.
+            auto enqueue_chunk = [&]() -> bool {
+                k_reset<<<1, 64, 0, s>>>(ctx->gs);
+                k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+                k_zero_reverify<<<1, 64, 0, s>>>(ctx->rstats);
+                int cur = cur0;
+                for (int64_t k = 0; k < len; ++k) {
+                    const int nxt = cur ^ 1;
+                    const MoonSample* m13 = ctx->moon + 12 * k;
+                    const StepInfo* info = ctx->steptab + k;
+                    g_slot = static_cast<int>(k * sub);
+                    mark(ctx, 6);
+                    const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
+                    if (pattern[static_cast<size_t>(k)]) {
+                        k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
+                        k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
+                            pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
+                            ctx->net_norm, lp, ctx->cstats);
+                        k_build_lists<<<grid_of(n), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
+                                                                   static_cast<int>(c), ctx->counts);
+                    } else {
+                        k_learned<net::CLASS_LEARNED><<<g_net, BLOCK, 0, s>>>(
+                            pc, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                        k_learned<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(
+                            pc, pn, ctx->alive, ctx->mode, ctx->lists + 2 * c, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                        k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
+                            pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                    }
+                    mark(ctx, 7);
+                    if (screen) {
+                        k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
+                        for (int q = 0; q < sub; ++q) {
+                            g_slot = static_cast<int>(k * sub + q);
+                            mark(ctx, 1);
+                            const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
+                            if (q < sub - 1)
+                                k_dense_output<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, n, ctx->dt,
+                                                                            static_cast<double>(q + 1) / sub,
+                                                                            dense_planes(ctx, q));
+                            const Planes pa = q == sub - 1 ? pn : dense_planes(ctx, q);
+                            k_reduce_mid<<<grid_of(n), BLOCK, 0, s>>>(pb, pa, ctx->alive, n, ctx->step_parts);
+                            if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q), h, thr, 1, info, q,
+                                                sub))
+                                return false;
+                        }
+                        k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
+                                                                  static_cast<unsigned long long>(ctx->ev_cap), pc,
+                                                                  ctx->radius, m13, info, h, sub, thr, vmin2,
+                                                                  ctx->rstats);
+                        mark(ctx, 0);
+                    } else {
+                        k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, ctx->dense_parts,
+                                                     0, ctx->exact_parts, 0);
+                    }
+                    cur = nxt;
+                }
+                return true;
+            };
+            if (ctx->use_graph && !ctx->timing) {
+                std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n};
+                key.insert(key.end(), pattern.begin(), pattern.end());
+                hipGraphExec_t exec = nullptr;
+                for (size_t g = 0; g < ctx->graphs.size(); ++g) {
+                    if (ctx->graphs[g].first != key) continue;
+                    exec = ctx->graphs[g].second;
+                    std::rotate(ctx->graphs.begin() + g, ctx->graphs.begin() + g + 1, ctx->graphs.end());
+                    break;
+                }
+                if (!exec) {
+                    // Entries built for an older generation can never match again.
+                    for (size_t g = 0; g < ctx->graphs.size();) {
+                        if (ctx->graphs[g].first[0] != key[0] || ctx->graphs.size() >= kGraphCache) {
+                            (void)hipGraphExecDestroy(ctx->graphs[g].second);
+                            ctx->graphs.erase(ctx->graphs.begin() + g);
+                        } else {
+                            ++g;
+                        }
+                    }
+                    KHIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    const bool ok = enqueue_chunk();
+                    hipGraph_t graph = nullptr;
+                    const hipError_t ec = hipStreamEndCapture(s, &graph);
+                    if (!ok) return false;
+                    if (ec != hipSuccess) return fail(ctx, "hipStreamEndCapture", ec);
+                    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    if (ei != hipSuccess) return fail(ctx, "hipGraphInstantiate", ei);
+                    ctx->graphs.emplace_back(key, exec);
+                    ctx->graph_captures += 1;
+                }
+                KHIP_TRY(ctx, hipGraphLaunch(exec, s));
+                ctx->graph_launches += 1;
+            } else if (!enqueue_chunk()) {
+                return false;
+            }
+            const int cur = cur_end;
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
             if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
@@ -2355,6 +2505,124 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     }
     ctx->step += nsteps;
     return true;
+}
+
+// Autotuner: times `steps` screened steps of the current configuration for
+// each candidate launch / tiling (coordinate descent, one knob at a time),
+// each trial from the same saved population and time, and keeps the
+// fastest. The population, time, events and statistics are restored
+// afterwards. Tilings never change results; cell_mult > 1 (allow_cell)
+// changes the candidate counts in the statistics only.
+bool autotune(khip_ctx* ctx, int64_t steps, bool allow_cell, double& base_ms, double& best_ms) {
+    hipStream_t s = ctx->stream;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    const bool learned = ctx->precision == KHIP_PRECISION_LEARNED;
+    if (learned && !ensure_learned(ctx)) return false;
+    TrainScratch sc;
+    double* st;
+    uint8_t *al, *md = nullptr;
+    int* ls = nullptr;
+    CheckStats* cs = nullptr;
+    if (!sc.get(ctx, &st, 6 * c) || !sc.get(ctx, &al, c)) return false;
+    if (learned && (!sc.get(ctx, &md, c) || !sc.get(ctx, &ls, 3 * c + 3) || !sc.get(ctx, &cs, 1))) return false;
+    KHIP_TRY(ctx, hipMemcpyAsync(st, ctx->state[ctx->cur], sizeof(double) * 6 * c, hipMemcpyDeviceToDevice, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(al, ctx->alive, c, hipMemcpyDeviceToDevice, s));
+    if (learned) {
+        KHIP_TRY(ctx, hipMemcpyAsync(md, ctx->mode, c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ls, ctx->lists, sizeof(int) * 3 * c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ls + 3 * c, ctx->counts, sizeof(int) * 3, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(cs, ctx->cstats, sizeof(CheckStats), hipMemcpyDeviceToDevice, s));
+    }
+    const double t0 = ctx->t;
+    const int64_t step0 = ctx->step;
+    const int cur0 = ctx->cur;
+    const bool force0 = ctx->force_check, timing0 = ctx->timing, graph0 = ctx->use_graph;
+    const khip_learned_report lrep0 = ctx->lrep;
+    const khip_timing tm0 = ctx->tm;
+    std::vector<khip_event> ev0;
+    std::vector<khip_stats> st0;
+    ev0.swap(ctx->host_events);
+    st0.swap(ctx->host_stats);
+    ctx->timing = false;
+    ctx->use_graph = false;
+    auto restore = [&]() -> bool {
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], st, sizeof(double) * 6 * c, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, al, c, hipMemcpyDeviceToDevice, s));
+        if (learned) {
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->mode, md, c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->lists, ls, sizeof(int) * 3 * c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->counts, ls + 3 * c, sizeof(int) * 3, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->cstats, cs, sizeof(CheckStats), hipMemcpyDeviceToDevice, s));
+        }
+        ctx->t = t0;
+        ctx->step = step0;
+        ctx->cur = cur0;
+        ctx->force_check = force0;
+        ctx->lrep = lrep0;
+        ctx->host_events.clear();
+        ctx->host_stats.clear();
+        return true;
+    };
+    auto trial = [&](const TileCfg& tc, double& ms) -> bool {
+        ctx->tile = tc;
+        ctx->gen += 1;
+        if (!restore()) return false;
+        KHIP_TRY(ctx, hipStreamSynchronize(s));
+        const auto w0 = std::chrono::steady_clock::now();
+        if (!(learned ? run_steps_learned(ctx, steps, 1) : run_steps(ctx, steps, 1))) return false;
+        KHIP_TRY(ctx, hipStreamSynchronize(s));
+        ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+        return true;
+    };
+    TileCfg bestc = ctx->tile;
+    double bt = 0.0;
+    bool ok = trial(bestc, bt) && trial(bestc, bt);  // the first run grows buffers
+    base_ms = bt;
+    struct Knob {
+        int TileCfg::*field;
+        std::vector<int> values;
+    };
+    const Knob knobs[] = {{&TileCfg::dense_min, {16, 32, 64, 128}},
+                          {&TileCfg::long_range, {32, 64, 128, 256}},
+                          {&TileCfg::seg, {128, 256, 512}},
+                          {&TileCfg::dense_blocks, {512, 1024, 2048, 4096}},
+                          {&TileCfg::exact_blocks, {256, 512, 1024, 2048}}};
+    for (const Knob& k : knobs) {
+        for (int v : k.values) {
+            if (!ok || bestc.*(k.field) == v) continue;
+            TileCfg cand = bestc;
+            cand.*(k.field) = v;
+            double t = 0.0;
+            ok = trial(cand, t);
+            if (ok && t < 0.98 * bt) {
+                bestc = cand;
+                bt = t;
+            }
+        }
+    }
+    if (allow_cell) {
+        for (double m : {1.1, 1.25, 1.5}) {
+            if (!ok) break;
+            TileCfg cand = bestc;
+            cand.cell_mult = m;
+            double t = 0.0;
+            ok = trial(cand, t);
+            if (ok && t < 0.98 * bt) {
+                bestc = cand;
+                bt = t;
+            }
+        }
+    }
+    best_ms = bt;
+    ctx->tile = bestc;
+    ctx->gen += 1;
+    const bool restored = restore();
+    ctx->host_events.swap(ev0);
+    ctx->host_stats.swap(st0);
+    ctx->timing = timing0;
+    ctx->use_graph = graph0;
+    ctx->tm = tm0;
+    return ok && restored;
 }
 
 bool upload_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_t* alive,
@@ -2443,20 +2711,6 @@ std::vector<float> net_init_params(uint64_t seed) {
     return p;
 }
 
-// Device buffers of one training run (freed at the end of the run).
-struct TrainScratch {
-    std::vector<void*> ptrs;
-    template <typename T>
-    bool get(khip_ctx* ctx, T** p, size_t n) {
-        *p = nullptr;
-        KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)));
-        ptrs.push_back(*p);
-        return true;
-    }
-    ~TrainScratch() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-};
 
 bool net_alloc_tensors(khip_ctx* ctx, TrainScratch& sc, int B, net::TrainTensors& T) {
     const size_t b = static_cast<size_t>(B);
@@ -2626,6 +2880,7 @@ bool net_train(khip_ctx* ctx, const double* orbits, int64_t n, const khip_train_
         for (int k = 0; k < cfg.iterations; ++k) curve[k] = static_cast<float>(lh[k] * scale);
     ctx->net_norm = nn;
     ctx->net_ready = true;
+    ctx->gen += 1;
     rep.samples = static_cast<int64_t>(pool.size());
     rep.loss_first = first;
     rep.loss_last = last;
@@ -2740,6 +2995,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
 void khip_destroy(khip_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     free_all(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -2753,7 +3009,35 @@ int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8
     if (!ctx || n < 0 || (n > 0 && !states) || n >= (1LL << 31) - TILE) return KHIP_ERR_ARG;
     ctx->force_check = true;
     ctx->lrep = khip_learned_report{};
+    ctx->sbits = 0;
+    ctx->scen_off.clear();
     return status(ctx, upload_objects(ctx, n, states, alive, size_m));
+}
+
+int khip_set_scenarios(khip_ctx* ctx, int32_t nscen, const int64_t* offsets) {
+    if (!ctx || nscen < 1 || nscen > (1 << 16) || !offsets || offsets[0] != 0 || offsets[nscen] != ctx->n)
+        return KHIP_ERR_ARG;
+    for (int32_t k = 0; k < nscen; ++k)
+        if (offsets[k + 1] < offsets[k]) return KHIP_ERR_ARG;
+    ctx->scen_off.assign(offsets, offsets + nscen + 1);
+    ctx->gen += 1;
+    if (nscen == 1) {
+        ctx->sbits = 0;
+        return KHIP_OK;
+    }
+    const size_t c = static_cast<size_t>(std::max<int64_t>(ctx->cap, 1));
+    if (!dev_alloc(ctx, &ctx->scen, c)) return KHIP_ERR_HIP;
+    std::vector<int> sid(c, 0);
+    for (int32_t k = 0; k < nscen; ++k)
+        for (int64_t i = offsets[k]; i < offsets[k + 1]; ++i) sid[static_cast<size_t>(i)] = k;
+    if (hipMemcpy(ctx->scen, sid.data(), sizeof(int) * c, hipMemcpyHostToDevice) != hipSuccess) {
+        ctx->error = "hipMemcpy of scenario ids failed";
+        return KHIP_ERR_HIP;
+    }
+    int b = 0;
+    while ((1LL << b) < nscen) ++b;
+    ctx->sbits = b;
+    return KHIP_OK;
 }
 
 int64_t khip_object_count(const khip_ctx* ctx) { return ctx ? ctx->n : -1; }
@@ -2800,6 +3084,7 @@ int khip_set_config(khip_ctx* ctx, const khip_screen_config* cfg) {
         !(cfg->min_encounter_speed_km_s >= 0))
         return KHIP_ERR_ARG;
     ctx->cfg = *cfg;
+    ctx->gen += 1;
     return status(ctx, refresh_radius(ctx));
 }
 
@@ -2809,6 +3094,7 @@ int khip_set_time(khip_ctx* ctx, double epoch_jd, double t, double dt) {
     ctx->t = t;
     ctx->dt = dt;
     ctx->step = 0;
+    ctx->gen += 1;
     return KHIP_OK;
 }
 
@@ -2944,6 +3230,52 @@ int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
         return KHIP_ERR_ARG;
     ctx->lcfg = *cfg;
     ctx->force_check = true;
+    ctx->gen += 1;
+    return KHIP_OK;
+}
+
+int khip_set_tuning(khip_ctx* ctx, const khip_tuning* t) {
+    static_assert(sizeof(khip_tuning) == sizeof(TileCfg), "tuning layout");
+    if (!ctx || !t || !(t->cell_mult >= 1.0) || t->cell_mult > 4.0 || t->dense_min < 2 || t->long_range < 1 ||
+        t->seg < 1 || t->dense_blocks < 1 || t->dense_blocks > MAX_DENSE_BLOCKS || t->exact_blocks < 1 ||
+        t->exact_blocks > MAX_EXACT_BLOCKS)
+        return KHIP_ERR_ARG;
+    std::memcpy(&ctx->tile, t, sizeof(TileCfg));
+    ctx->gen += 1;
+    return KHIP_OK;
+}
+
+int khip_get_tuning(const khip_ctx* ctx, khip_tuning* t) {
+    if (!ctx || !t) return KHIP_ERR_ARG;
+    std::memcpy(t, &ctx->tile, sizeof(TileCfg));
+    return KHIP_OK;
+}
+
+int khip_autotune(khip_ctx* ctx, int64_t steps, int allow_cell, khip_tuning* best, double* base_ms,
+                  double* best_ms) {
+    if (!ctx || steps < 1 || steps > 4096) return KHIP_ERR_ARG;
+    if (ctx->n == 0) {
+        ctx->error = "autotune: no objects";
+        return KHIP_ERR_STATE;
+    }
+    double b0 = 0.0, b1 = 0.0;
+    const bool ok = autotune(ctx, steps, allow_cell != 0, b0, b1);
+    if (best) std::memcpy(best, &ctx->tile, sizeof(TileCfg));
+    if (base_ms) *base_ms = b0;
+    if (best_ms) *best_ms = b1;
+    return status(ctx, ok);
+}
+
+int khip_set_graph(khip_ctx* ctx, int enabled) {
+    if (!ctx) return KHIP_ERR_ARG;
+    ctx->use_graph = enabled != 0;
+    return KHIP_OK;
+}
+
+int khip_get_graph_counts(const khip_ctx* ctx, int64_t* captures, int64_t* launches) {
+    if (!ctx) return KHIP_ERR_ARG;
+    if (captures) *captures = ctx->graph_captures;
+    if (launches) *launches = ctx->graph_launches;
     return KHIP_OK;
 }
 
@@ -2972,6 +3304,7 @@ int khip_net_set(khip_ctx* ctx, const float* params, const khip_net_norm* norm) 
     if (!net_upload(ctx, params)) return KHIP_ERR_HIP;
     std::memcpy(&ctx->net_norm, norm, sizeof(net::NetNorm));
     ctx->net_ready = true;
+    ctx->gen += 1;
     return KHIP_OK;
 }
 
