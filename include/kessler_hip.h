@@ -103,6 +103,7 @@ typedef struct {
 #define KHIP_RERUN_EVENTS 2    /* event buffer */
 #define KHIP_RERUN_ITEMS 4     /* dense-cell work items */
 #define KHIP_RERUN_SURVIVORS 8 /* pre-filter survivor queue */
+#define KHIP_RERUN_BREAKUPS 16 /* breakup jobs, fragment samples or reserved object slots */
 
 /* Number of visible HIP devices (0 when none or the runtime fails). */
 int khip_device_count(void);
@@ -182,7 +183,7 @@ int khip_reset_timing(khip_ctx* ctx);
  * steps, and the first step after khip_set_objects / khip_set_precision /
  * khip_set_learned) runs the plain method for every object and keeps the
  * cheaper method for an object only if it stayed within tol_km of the plain
- * result. Screening covers each sub-step of tau / substeps on the cubic
+ * result in position and within tol_a_km in semi-major axis. Screening covers each sub-step of tau / substeps on the cubic
  * Hermite dense output of the step, with a grid on midpoints and the
  * threshold raised by margin_km; every event found is then solved again on
  * plain FP64 trajectories from the step's start (filter_pair, Hermite +
@@ -198,6 +199,8 @@ typedef struct {
     double margin_km;    /* screening threshold raise (default 0.025) */
     double e_max;        /* corrector / base-step domain: eccentricity below (default 0.1) */
     double a_max_km;     /* corrector domain: semi-major axis below (default 12000) */
+    double tol_a_km;     /* semi-major-axis deviation tolerated at a check (default 5e-8: a
+                            systematic 5e-8 km per 60 s step drifts ~5 m along track in 24 h) */
 } khip_learned_config;
 
 typedef struct {
@@ -217,6 +220,50 @@ int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);
  * and indices come from a device table, so replays need no re-capture.
  * No reference counterpart. */
 int khip_set_graph(khip_ctx* ctx, int enabled);
+
+/* ---- In-run breakups (cascade mode) ---------------------------------------
+ *
+ * Replaces Fragmentation::apply after each step's screening (main.cpp:408-415,
+ * breakup.cpp:316-338) on the device: collisions of the step in (t, i, j)
+ * order, parents alive -> the NASA Standard Breakup Model (target = heavier
+ * parent; specific energy against the catastrophic threshold; mass budgets;
+ * N = Poisson(0.1 M^0.75 L_c^-1.71), at most 200,000; sizes, area-to-mass,
+ * log-normal isotropic delta-v; hyperbolic fragments dropped; RK4 to the end
+ * of the step). New objects are appended after the population (indices
+ * n, n + 1, ...) without a host round trip. The draws come from a
+ * counter-based generator, not the reference's mt19937_64 sequence: the
+ * fragments follow the model's distributions, not the reference's exact
+ * draws (the host restatement in kessler_host.h keeps those). Plain and
+ * mixed forms only; the learned form returns an error while enabled. */
+typedef struct {
+    int32_t enabled;
+    int32_t reserve;             /* object slots reserved per chunk (grown when short) */
+    double lc_m;                 /* smallest fragment (default 0.1 m) */
+    double catastrophic_j_per_g; /* threshold (default 40 J/g) */
+    uint64_t seed;
+} khip_breakup_config;
+
+/* BreakupEvent (breakup.hpp:86-104) of one breakup. */
+typedef struct {
+    double t, v_rel_km_s, energy_j_per_g, target_mass_kg, projectile_mass_kg, sbm_mass_kg;
+    double expected_fragments, fragments_mass_kg;
+    double position[3];          /* target at the collision (km) */
+    int32_t id, target, projectile, catastrophic;
+    int32_t drawn;               /* fragments drawn from the Poisson law */
+    int32_t fragments;           /* fragments created (objects first_index ..) */
+    int32_t escaped;             /* hyperbolic fragments dropped */
+    int32_t first_index;
+} khip_breakup_event;
+
+int khip_set_breakup(khip_ctx* ctx, const khip_breakup_config* cfg);
+/* Object masses (kg, n = khip_object_count; 0 = the model's 100 kg fallback),
+ * Object::mass_kg (orbital.hpp:67-84). Damaged targets lose mass. */
+int khip_set_masses(khip_ctx* ctx, int64_t n, const double* mass_kg);
+int khip_get_masses(khip_ctx* ctx, int64_t n, double* mass_kg);
+/* Characteristic sizes L (m), Object::size_m, of the n current objects. */
+int khip_get_sizes(const khip_ctx* ctx, int64_t n, double* size_m);
+/* Moves up to cap breakup records (in order) to out; out == NULL queries. */
+int khip_take_breakups(khip_ctx* ctx, khip_breakup_event* out, int64_t cap, int64_t* n);
 
 /* Launch and tiling parameters of the screening kernels. None changes which
  * conjunctions are found; cell_mult > 1 enlarges the grid cell beyond the

@@ -44,7 +44,13 @@ ABI_SYMBOLS = (
     "khip_net_train", "khip_net_set", "khip_net_get", "khip_net_infer", "khip_net_gradient",
     "khip_set_learned", "khip_get_learned_report", "khip_set_scenarios", "khip_set_graph",
     "khip_get_graph_counts", "khip_set_tuning", "khip_get_tuning", "khip_autotune",
+    "khip_set_breakup", "khip_set_masses", "khip_get_masses", "khip_get_sizes", "khip_take_breakups",
 )
+BREAKUP_DTYPE = np.dtype([("t", "<f8"), ("v_rel_km_s", "<f8"), ("energy_j_per_g", "<f8"),
+                          ("target_mass_kg", "<f8"), ("projectile_mass_kg", "<f8"), ("sbm_mass_kg", "<f8"),
+                          ("expected_fragments", "<f8"), ("fragments_mass_kg", "<f8"), ("position", "<f8", (3,)),
+                          ("id", "<i4"), ("target", "<i4"), ("projectile", "<i4"), ("catastrophic", "<i4"),
+                          ("drawn", "<i4"), ("fragments", "<i4"), ("escaped", "<i4"), ("first_index", "<i4")])
 PRECISION_LEARNED = 2
 NET_PARAMS = 35206
 NET_INPUTS = 10
@@ -77,7 +83,7 @@ class TrainConfig(C.Structure):
 
 class LearnedConfig(C.Structure):
     _fields_ = [("substeps", C.c_int32), ("check_every", C.c_int32), ("tol_km", _D), ("margin_km", _D),
-                ("e_max", _D), ("a_max_km", _D)]
+                ("e_max", _D), ("a_max_km", _D), ("tol_a_km", _D)]
 
 
 LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", "<i8"), ("rejected", "<i8"),
@@ -89,6 +95,11 @@ LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", 
 class TrainReport(C.Structure):
     _fields_ = [("samples", C.c_int64), ("loss_first", _D), ("loss_last", _D), ("gen_ms", _D),
                 ("train_ms", _D)]
+
+
+class BreakupConfig(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("reserve", C.c_int32), ("lc_m", _D), ("catastrophic_j_per_g", _D),
+                ("seed", C.c_uint64)]
 
 
 class Tuning(C.Structure):
@@ -149,6 +160,11 @@ def library() -> C.CDLL:
         "khip_set_graph": (_I, [_P, _I]),
         "khip_get_graph_counts": (_I, [_P, C.POINTER(_L), C.POINTER(_L)]),
         "khip_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+        "khip_set_breakup": (_I, [_P, C.POINTER(BreakupConfig)]),
+        "khip_set_masses": (_I, [_P, _L, _P]),
+        "khip_get_masses": (_I, [_P, _L, _P]),
+        "khip_get_sizes": (_I, [_P, _L, _P]),
+        "khip_take_breakups": (_I, [_P, _P, _L, C.POINTER(_L)]),
         "khip_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
         "khip_autotune": (_I, [_P, _L, _I, C.POINTER(Tuning), C.POINTER(_D), C.POINTER(_D)]),
         "khost_generate": (_I, [_L, C.c_uint64, _P, _P]),
@@ -345,8 +361,8 @@ class Engine:
         return grad, loss.value
 
     def set_learned(self, substeps=6, check_every=10, tol_km=1e-5, margin_km=0.025, e_max=0.1,
-                    a_max_km=12000.0):
-        cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km)
+                    a_max_km=12000.0, tol_a_km=5e-8):
+        cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km, tol_a_km)
         self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")
 
     def set_scenarios(self, offsets):
@@ -361,6 +377,33 @@ class Engine:
         c, l = _L(), _L()
         self._check(self.lib.khip_get_graph_counts(self.h, C.byref(c), C.byref(l)), "khip_get_graph_counts")
         return c.value, l.value
+
+    # ---- in-run breakups -------------------------------------------------------
+
+    def set_breakup(self, enabled=True, reserve=65536, lc_m=0.1, catastrophic_j_per_g=40.0, seed=1):
+        cfg = BreakupConfig(1 if enabled else 0, int(reserve), float(lc_m), float(catastrophic_j_per_g), int(seed))
+        self._check(self.lib.khip_set_breakup(self.h, C.byref(cfg)), "khip_set_breakup")
+
+    def set_masses(self, mass_kg):
+        m = np.ascontiguousarray(mass_kg, dtype=np.float64)
+        self._check(self.lib.khip_set_masses(self.h, len(m), _ptr(m)), "khip_set_masses")
+
+    def masses(self):
+        m = np.zeros(self.n)
+        self._check(self.lib.khip_get_masses(self.h, len(m), _ptr(m)), "khip_get_masses")
+        return m
+
+    def sizes(self):
+        m = np.zeros(self.n)
+        self._check(self.lib.khip_get_sizes(self.h, len(m), _ptr(m)), "khip_get_sizes")
+        return m
+
+    def take_breakups(self) -> np.ndarray:
+        n = _L()
+        self._check(self.lib.khip_take_breakups(self.h, None, 0, C.byref(n)), "khip_take_breakups")
+        out = np.zeros(n.value, dtype=BREAKUP_DTYPE)
+        self._check(self.lib.khip_take_breakups(self.h, _ptr(out), n.value, C.byref(n)), "khip_take_breakups")
+        return out[: n.value]
 
     def tuning(self) -> dict:
         t = Tuning()

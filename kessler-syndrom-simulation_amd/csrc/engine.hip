@@ -51,6 +51,7 @@
 #include "../host/orbital.hpp"
 #include "device_common.hpp"
 #include "mlp.hpp"
+#include "breakup_dev.hpp"
 
 using namespace kessler;
 using namespace kessler::dev;
@@ -303,8 +304,14 @@ struct LearnedParams {
     double tau;
     int substeps;
     int pad;
-    double e_max, a_max, tol_km;
+    double e_max, a_max, tol_km, tol_a_km;
 };
+
+// Semi-major axis from the orbital energy (bound orbits; large otherwise).
+__device__ __forceinline__ double semi_major(D3 r, D3 v) {
+    const double en = 0.5 * (v.x * v.x + v.y * v.y + v.z * v.z) - dev::MU_EARTH / sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+    return en < 0.0 ? -dev::MU_EARTH / (2.0 * en) : 1e30;
+}
 
 struct CheckStats {
     unsigned int count[3];        // objects per method after the check
@@ -430,7 +437,11 @@ __global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8
                 if (survives && cls != net::CLASS_PLAIN) {
                     const double dx = rb.x - rp.x, dy = rb.y - rp.y, dz = rb.z - rp.z;
                     const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                    if (d <= lp.tol_km) {
+                    // A per-step error in the semi-major axis grows into an
+                    // along-track drift quadratic in time, so it is bounded
+                    // separately from the position.
+                    const double da = fabs(semi_major(rb, vb) - semi_major(rp, vp));
+                    if (d <= lp.tol_km && da <= lp.tol_a_km) {
                         m = cls;
                         dev_max = d > dev_max ? d : dev_max;
                     } else {
@@ -1554,6 +1565,276 @@ __global__ void k_mark_events(GridState* gs, unsigned long long cap) {
     if (threadIdx.x == 0) gs->ev_mark = gs->events < cap ? gs->events : cap;
 }
 
+// ---------------------------------------------------------------------------
+// In-run breakups (plain and mixed forms): collisions of the step -> jobs ->
+// fragment samples -> mass budgets -> new objects in reserved slots
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(BLOCK) k_bu_events(const GridState* gs, const DevEvent* __restrict__ events,
+                                                      unsigned long long cap, Planes pb, Planes pa, uint8_t* alive,
+                                                      double* mass, const double* size_m, const int* scen, double t0,
+                                                      double dt, bu::BreakupCfg cfg, bu::BreakupState* st,
+                                                      bu::Job* jobs, bu::LogRecord* logs, int log_cap) {
+    __shared__ int s_n;
+    __shared__ unsigned long long s_idx[bu::MAX_JOBS];
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const unsigned long long e0 = gs->ev_mark, e1 = gs->events < cap ? gs->events : cap;
+    for (unsigned long long e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+        if (!events[e].collision) continue;
+        const int k = atomicAdd(&s_n, 1);
+        if (k < bu::MAX_JOBS) s_idx[k] = e;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    st->njobs = 0;
+    st->nsamples = 0;
+    const int n = s_n < bu::MAX_JOBS ? s_n : bu::MAX_JOBS;
+    if (s_n > bu::MAX_JOBS) st->jobs_short = 1;
+    // (t, i, j) order, as the host applies a step's collisions.
+    for (int a = 1; a < n; ++a) {
+        const unsigned long long x = s_idx[a];
+        int b = a - 1;
+        while (b >= 0) {
+            const DevEvent& p = events[s_idx[b]];
+            const DevEvent& q = events[x];
+            const bool after = p.t > q.t || (p.t == q.t && (p.i > q.i || (p.i == q.i && p.j > q.j)));
+            if (!after) break;
+            s_idx[b + 1] = s_idx[b];
+            --b;
+        }
+        s_idx[b + 1] = x;
+    }
+    for (int c = 0; c < n; ++c) {
+        const DevEvent ev = events[s_idx[c]];
+        const int ii = ev.i, jj = ev.j;
+        if (!alive[ii] || !alive[jj]) continue;
+        // Hermite state of both objects at the collision instant (collision.cpp:289-292).
+        const double s = (ev.t - t0) / dt;
+        D3 r[2], v[2];
+        const int idx[2] = {ii, jj};
+        for (int q = 0; q < 2; ++q) {
+            const int o = idx[q];
+            const Hermite h{{pb.x[o], pb.y[o], pb.z[o]},
+                            {pb.vx[o] * dt, pb.vy[o] * dt, pb.vz[o] * dt},
+                            {pa.x[o], pa.y[o], pa.z[o]},
+                            {pa.vx[o] * dt, pa.vy[o] * dt, pa.vz[o] * dt}};
+            r[q] = h.pos(s);
+            const D3 w = h.vel(s);
+            v[q] = D3{w.x / dt, w.y / dt, w.z / dt};
+        }
+        const double mi = mass[ii] > 0 ? mass[ii] : bu::FALLBACK_MASS_KG;
+        const double mj = mass[jj] > 0 ? mass[jj] : bu::FALLBACK_MASS_KG;
+        const int tq = mi >= mj ? 0 : 1, pq = 1 - tq;
+        const int tgt = idx[tq], prj = idx[pq];
+        const double mt = tq == 0 ? mi : mj, mp = tq == 0 ? mj : mi;
+        const double dvx = v[tq].x - v[pq].x, dvy = v[tq].y - v[pq].y, dvz = v[tq].z - v[pq].z;
+        const double vrel = sqrt(dvx * dvx + dvy * dvy + dvz * dvz);
+        const double energy = 500.0 * mp * vrel * vrel / mt;
+        const bool cat = energy >= cfg.catastrophic_j_per_g;
+        double b0, b1;
+        if (cat) {
+            b0 = mt;
+            b1 = mp;
+        } else {
+            b1 = mp;
+            b0 = fmax(0.0, mp * vrel * vrel - mp);
+        }
+        const double M = b0 + b1;
+        alive[prj] = 0;
+        if (cat) alive[tgt] = 0;
+        else mass[tgt] = fmax(mt - b0, 0.0);
+        const double expected = M > 0 ? 0.1 * pow(M, 0.75) * pow(cfg.lc_m, -bu::SLOPE) : 0.0;
+        const int id = st->next_event++;
+        bu::Rng g(cfg.seed, static_cast<uint64_t>(id), 0xffffffffULL);
+        long long cnt = expected > 0 ? g.poisson(expected) : 0;
+        if (cnt > bu::MAX_FRAGS_PER_EVENT) cnt = bu::MAX_FRAGS_PER_EVENT;
+        if (st->nsamples + cnt > cfg.sample_cap) {
+            st->samples_short = 1;
+            cnt = 0;
+        }
+        bu::Job jb;
+        jb.t = ev.t;
+        jb.remaining = t0 + dt - ev.t;
+        jb.vrel = vrel;
+        jb.p_target = M > 0 ? b0 / M : 0.0;
+        jb.r[0] = r[tq];
+        jb.v[0] = v[tq];
+        jb.r[1] = r[pq];
+        jb.v[1] = v[pq];
+        jb.size[0] = size_m[tgt];
+        jb.size[1] = size_m[prj];
+        jb.budget[0] = b0;
+        jb.budget[1] = b1;
+        jb.count = static_cast<int>(cnt);
+        jb.event = id;
+        jb.sample_base = st->nsamples;
+        jb.scen = scen ? scen[tgt] : 0;
+        jb.kept = jb.escaped = jb.base = 0;
+        jb.used_mass = 0.0;
+        jb.log = st->nlog < log_cap ? st->nlog : -1;
+        if (jb.log >= 0) {
+            bu::LogRecord& L = logs[jb.log];
+            L.t = ev.t;
+            L.v_rel_km_s = vrel;
+            L.energy_j_per_g = energy;
+            L.target_mass_kg = mt;
+            L.projectile_mass_kg = mp;
+            L.sbm_mass_kg = M;
+            L.expected_fragments = expected;
+            L.fragments_mass_kg = 0.0;
+            L.px = r[tq].x;
+            L.py = r[tq].y;
+            L.pz = r[tq].z;
+            L.id = id;
+            L.target = tgt;
+            L.projectile = prj;
+            L.catastrophic = cat ? 1 : 0;
+            L.drawn = jb.count;
+            L.fragments = 0;
+            L.escaped = 0;
+            L.first = 0;
+        }
+        ++st->nlog;
+        st->nsamples += jb.count;
+        jobs[st->njobs++] = jb;
+    }
+}
+
+// Last job whose first sample is <= f.
+__device__ __forceinline__ int job_of(const bu::Job* jobs, int njobs, int f) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].sample_base <= f) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Fragment sample f (of job jb): its own counter-based stream.
+__device__ void sample_one(const bu::Job& jb, int f, bu::Sample* samples, const bu::BreakupCfg& cfg) {
+    const int k = f - jb.sample_base;
+    bu::Rng g(cfg.seed, static_cast<uint64_t>(jb.event), static_cast<uint64_t>(k));
+    bu::Sample sm;
+    sm.rank = -1;
+    sm.side = g.uniform() < jb.p_target ? 0 : 1;
+    const double lmax = jb.size[sm.side];
+    if (lmax <= cfg.lc_m) {
+        sm.side = -1;
+        samples[f] = sm;
+        return;
+    }
+    // Size by inverting the truncated power law on [L_c, parent size].
+    const double rr = pow(lmax / cfg.lc_m, -bu::SLOPE);
+    const double L = cfg.lc_m * pow(1.0 - g.uniform() * (1.0 - rr), -1.0 / bu::SLOPE);
+    const double chi = bu::area_to_mass_log(g, log10(L));
+    const double area = 0.540424 * L * L;
+    const double m = area / pow(10.0, chi);
+    double dv_ms = pow(10.0, 0.9 * chi + 2.9 + 0.4 * g.normal());
+    dv_ms = fmin(dv_ms, jb.vrel * 1000.0);
+    const double cz = 2.0 * g.uniform() - 1.0;
+    const double phi = 2.0 * bu::PI_D * g.uniform();
+    const double sz = sqrt(fmax(0.0, 1.0 - cz * cz));
+    const double a = dv_ms * 1e-3;
+    sm.dv = D3{sz * cos(phi) * a, sz * sin(phi) * a, cz * a};
+    sm.L = L;
+    sm.chi = chi;
+    sm.m = m;
+    samples[f] = sm;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_bu_sample(const bu::BreakupState* st, const bu::Job* __restrict__ jobs,
+                                                      bu::Sample* samples, bu::BreakupCfg cfg) {
+    const int ns = st->nsamples, nj = st->njobs;
+    for (int f = blockIdx.x * BLOCK + threadIdx.x; f < ns; f += gridDim.x * BLOCK)
+        sample_one(jobs[job_of(jobs, nj, f)], f, samples, cfg);
+}
+
+// One lane per job, fragments in order: the parent's mass budget (a fragment
+// that would exceed it is skipped, later smaller ones may still fit), then
+// the hyperbolic check; kept fragments get consecutive ranks.
+__global__ void __launch_bounds__(64) k_bu_budget(const bu::BreakupState* st, bu::Job* jobs, bu::Sample* samples,
+                                                   bu::LogRecord* logs) {
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= st->njobs) return;
+    bu::Job& jb = jobs[j];
+    double used[2] = {0.0, 0.0};
+    int kept = 0, esc = 0;
+    double mass_kept = 0.0;
+    for (int k = 0; k < jb.count; ++k) {
+        bu::Sample& sm = samples[jb.sample_base + k];
+        if (sm.side < 0) continue;
+        if (used[sm.side] + sm.m > jb.budget[sm.side]) continue;
+        used[sm.side] += sm.m;
+        const D3 r = jb.r[sm.side];
+        const D3 v{jb.v[sm.side].x + sm.dv.x, jb.v[sm.side].y + sm.dv.y, jb.v[sm.side].z + sm.dv.z};
+        if (0.5 * (v.x * v.x + v.y * v.y + v.z * v.z) - dev::MU_EARTH / sqrt(r.x * r.x + r.y * r.y + r.z * r.z) >=
+            0.0) {
+            ++esc;
+            continue;
+        }
+        sm.rank = kept++;
+        mass_kept += sm.m;
+    }
+    jb.kept = kept;
+    jb.escaped = esc;
+    jb.used_mass = mass_kept;
+    if (jb.log >= 0) {
+        logs[jb.log].fragments = kept;
+        logs[jb.log].escaped = esc;
+        logs[jb.log].fragments_mass_kg = mass_kept;
+    }
+}
+
+__global__ void k_bu_bases(bu::BreakupState* st, bu::Job* jobs, bu::LogRecord* logs, int reserve) {
+    if (threadIdx.x != 0) return;
+    for (int j = 0; j < st->njobs; ++j) {
+        jobs[j].base = st->frag_used;
+        if (jobs[j].log >= 0) logs[jobs[j].log].first = st->frag_used;
+        st->frag_used += jobs[j].kept;
+    }
+    if (st->frag_used > reserve) st->reserve_short = 1;
+}
+
+// New objects: birth at the parent's collision state plus delta-v, RK4 over
+// the rest of the step (Moon from the step's samples), object arrays.
+__global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, const bu::Job* __restrict__ jobs,
+                                                     const bu::Sample* __restrict__ samples, int n0, int reserve,
+                                                     Planes pa, uint8_t* alive, double* radius, double* size_m,
+                                                     double* mass, int* scen, const MoonSample* __restrict__ moon3,
+                                                     double t0, double dt, bu::BreakupCfg cfg) {
+    const int ns = st->nsamples, nj = st->njobs;
+    for (int f = blockIdx.x * BLOCK + threadIdx.x; f < ns; f += gridDim.x * BLOCK) {
+        const bu::Sample sm = samples[f];
+        if (sm.rank < 0) continue;
+        const bu::Job& jb = jobs[job_of(jobs, nj, f)];
+        const int off = jb.base + sm.rank;
+        if (off >= reserve) continue;
+        const int slot = n0 + off;
+        D3 r = jb.r[sm.side];
+        D3 v{jb.v[sm.side].x + sm.dv.x, jb.v[sm.side].y + sm.dv.y, jb.v[sm.side].z + sm.dv.z};
+        if (jb.remaining > 0.0) {
+            const double f0 = (jb.t - t0) / dt;
+            const MoonSample m0 = bu::moon_at(moon3, f0);
+            const MoonSample m1 = bu::moon_at(moon3, f0 + 0.5 * jb.remaining / dt);
+            const MoonSample m2 = bu::moon_at(moon3, 1.0);
+            rk4(r, v, jb.remaining, m0, m1, m2);
+        }
+        pa.x[slot] = r.x;
+        pa.y[slot] = r.y;
+        pa.z[slot] = r.z;
+        pa.vx[slot] = v.x;
+        pa.vy[slot] = v.y;
+        pa.vz[slot] = v.z;
+        alive[slot] = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) >= dev::R_EARTH ? 1 : 0;
+        radius[slot] = 0.5 * sm.L * 1e-3 * cfg.radius_scale;
+        size_m[slot] = sm.L;
+        mass[slot] = sm.m;
+        if (scen) scen[slot] = jb.scen;
+    }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
                                                    const PairPartial* parts, int nparts,
                                                    const PairPartial* dparts, int ndparts,
@@ -1665,6 +1946,23 @@ struct khip_ctx {
     float4* lpos = nullptr;
     float4* lvel = nullptr;
     TileCfg tile{1.0, DENSE_MIN, LONG_RANGE, SEG, DENSE_BLOCKS, EXACT_BLOCKS, 0};
+
+    // In-run breakups (plain and mixed forms).
+    khip_breakup_config bcfg{0, 65536, 0.1, 40.0, 1};
+    int64_t n_launch = 0;          // objects the per-step kernels cover (n + reserved slots)
+    double* mass = nullptr;        // kg per object (0: the model's fallback)
+    double* snap_mass = nullptr;
+    bu::BreakupState* bstate = nullptr;
+    bu::BreakupState* snap_bstate = nullptr;
+    bu::Job* jobs = nullptr;
+    bu::Sample* samples = nullptr;
+    int sample_cap = 0;
+    bu::LogRecord* blogs = nullptr;
+    int blog_cap = 0;
+    int bu_reserve = 0;
+    bool reset_breakups = true;    // breakup ids restart with a new population
+    std::vector<khip_breakup_event> host_breakups;
+    size_t last_chunk_events = 0;  // events the last drained chunk appended
     int4* coltab = nullptr;  // column table, 1 << COL_BITS entries
     int* scen = nullptr;     // scenario id per object (concatenated scenarios)
     int sbits = 0;
@@ -1712,7 +2010,7 @@ struct khip_ctx {
 
     // Learned form: per-object method, per-method lists, dense-output
     // states of the sub-step boundaries, check and re-verification tallies.
-    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0};
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8};
     int64_t lrn_cap = 0;
     uint8_t* mode = nullptr;
     uint8_t* snap_mode = nullptr;
@@ -1724,6 +2022,7 @@ struct khip_ctx {
     double* substates = nullptr; // (substeps - 1) * 6 * cap: dense-output states
     int dense_k = 0;
     bool force_check = true;
+    bool reset_checks = true;    // zero the device check tallies before the next learned run
     khip_learned_report lrep{};
     int n_cus = 256;
 
@@ -1753,7 +2052,7 @@ struct khip_ctx {
         const size_t c = static_cast<size_t>(cap);
         return Planes{b, b + c, b + 2 * c, b + 3 * c, b + 4 * c, b + 5 * c};
     }
-    int ntiles() const { return static_cast<int>((n + TILE - 1) / TILE); }
+    int ntiles() const { return static_cast<int>((n_launch + TILE - 1) / TILE); }
 };
 
 namespace {
@@ -1800,7 +2099,8 @@ void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab, ctx->scen, ctx->steptab,
+                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab, ctx->scen, ctx->steptab, ctx->mass, ctx->snap_mass,
+                    ctx->bstate, ctx->snap_bstate, ctx->jobs, ctx->samples, ctx->blogs,
                     ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
                     ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
@@ -1834,7 +2134,8 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
     for (int b = 0; b < 2; ++b)
         if (!dev_alloc(ctx, &ctx->state[b], 6 * static_cast<size_t>(cap))) return false;
     return dev_alloc(ctx, &ctx->alive, cap) && dev_alloc(ctx, &ctx->radius, cap) &&
-           dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->keys[0], cap) &&
+           dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->mass, cap) &&
+           dev_alloc(ctx, &ctx->snap_mass, cap) && dev_alloc(ctx, &ctx->keys[0], cap) &&
            dev_alloc(ctx, &ctx->keys[1], cap) && dev_alloc(ctx, &ctx->idx[0], cap) &&
            dev_alloc(ctx, &ctx->idx[1], cap) && dev_alloc(ctx, &ctx->hist, RADIX * tiles) &&
            dev_alloc(ctx, &ctx->totals, RADIX) && dev_alloc(ctx, &ctx->bcount, tiles) &&
@@ -1931,7 +2232,7 @@ void mark(khip_ctx* ctx, int k) {
 // `before` / `after` (stats record `stat_slot`).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1) {
-    const int n = static_cast<int>(ctx->n);
+    const int n = static_cast<int>(ctx->n_launch);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
@@ -2115,6 +2416,7 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     for (int64_t k = 0; k < steps; ++k) start[k + 1] += start[k];
     const size_t out0 = ctx->host_events.size();
     ctx->host_events.resize(out0 + static_cast<size_t>(nk));
+    ctx->last_chunk_events = static_cast<size_t>(nk);
     khip_event* out = ctx->host_events.data() + out0;
     std::vector<int64_t> fill(start.begin(), start.end() - 1);
     for (const khip_event& e : raw) out[fill[static_cast<size_t>(e.step - step0)]++] = e;
@@ -2131,10 +2433,36 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     return true;
 }
 
+bool grow_preserve(khip_ctx* ctx, int64_t want);
+
+__global__ void k_bu_chunk_begin(bu::BreakupState* st) {
+    if (threadIdx.x != 0) return;
+    st->njobs = st->nsamples = st->frag_used = 0;
+    st->jobs_short = st->samples_short = st->reserve_short = 0;
+    st->nlog = 0;
+}
+
+constexpr int kBreakupLogCap = 1 << 16;
+
+bool ensure_breakup(khip_ctx* ctx) {
+    if (!ctx->bstate) {
+        if (!dev_alloc(ctx, &ctx->bstate, 1) || !dev_alloc(ctx, &ctx->snap_bstate, 1) ||
+            !dev_alloc(ctx, &ctx->jobs, bu::MAX_JOBS) || !dev_alloc(ctx, &ctx->blogs, kBreakupLogCap))
+            return false;
+        KHIP_TRY(ctx, hipMemset(ctx->bstate, 0, sizeof(bu::BreakupState)));
+        ctx->blog_cap = kBreakupLogCap;
+    }
+    if (ctx->sample_cap == 0) {
+        if (!dev_alloc(ctx, &ctx->samples, 1 << 20)) return false;
+        ctx->sample_cap = 1 << 20;
+    }
+    if (ctx->bu_reserve < ctx->bcfg.reserve) ctx->bu_reserve = ctx->bcfg.reserve;
+    return true;
+}
+
 bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
-    const int n = static_cast<int>(ctx->n);
     hipStream_t s = ctx->stream;
-    if (n == 0) {
+    if (ctx->n == 0) {
         for (int64_t k = 0; k < nsteps; ++k) {
             if (screen)
                 ctx->host_stats.push_back(khip_stats{0, 0, 0, std::max(1.0, ctx->cfg.threshold_km), 0, 0, 0, 0});
@@ -2143,10 +2471,17 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
         }
         return true;
     }
+    // In-run breakups act on the collisions the screening finds.
+    const bool bk = ctx->bcfg.enabled && screen;
+    if (bk && !ensure_breakup(ctx)) return false;
+    if (bk && ctx->reset_breakups) {
+        KHIP_TRY(ctx, hipMemsetAsync(ctx->bstate, 0, sizeof(bu::BreakupState), s));
+        ctx->reset_breakups = false;
+    }
     int64_t done = 0;
     // Chunks start at one step and double up to ctx->chunk: a buffer that
-    // must grow (events, work items, survivors, radix passes) costs the re-run
-    // of a short chunk only, usually right at the start.
+    // must grow (events, work items, survivors, radix passes, breakup room)
+    // costs the re-run of a short chunk only, usually right at the start.
     int64_t next_len = 1;
     while (done < nsteps) {
         const int64_t len = std::min<int64_t>(std::min<int64_t>(next_len, ctx->chunk), nsteps - done);
@@ -2155,14 +2490,42 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             (ctx->timing && !ensure_event_pool(ctx, len)))
             return false;
         if (!upload_moon(ctx, ctx->t, len)) return false;
-        // Snapshot of the chunk's starting population, for a re-run.
-        const size_t c = static_cast<size_t>(ctx->cap);
-        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
-                                     hipMemcpyDeviceToDevice, s));
-        KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_alive, ctx->alive, c, hipMemcpyDeviceToDevice, s));
-        const int cur0 = ctx->cur;
         const double t_chunk = ctx->t;
         for (;;) {
+            // Breakup room: the per-step kernels cover `reserve` dead slots past
+            // the population; fragments of this chunk take them in order.
+            const int64_t n0 = ctx->n;
+            int reserve = 0;
+            if (bk) {
+                reserve = ctx->bu_reserve;
+                if (!grow_preserve(ctx, n0 + reserve)) return false;
+                if (n0 + reserve >= (1LL << 31) - TILE) {
+                    ctx->error = "breakup room exceeds the object index range";
+                    return false;
+                }
+                KHIP_TRY(ctx, hipMemsetAsync(ctx->alive + n0, 0, static_cast<size_t>(reserve), s));
+                k_bu_chunk_begin<<<1, 64, 0, s>>>(ctx->bstate);
+            }
+            const int n = static_cast<int>(n0 + reserve);
+            ctx->n_launch = n;
+            // Snapshot of the chunk's starting population, for a re-run.
+            const size_t c = static_cast<size_t>(ctx->cap);
+            const int cur0 = ctx->cur;
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[cur0], sizeof(double) * 6 * c,
+                                         hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_alive, ctx->alive, c, hipMemcpyDeviceToDevice, s));
+            if (bk) {
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_mass, ctx->mass, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_bstate, ctx->bstate, sizeof(bu::BreakupState),
+                                             hipMemcpyDeviceToDevice, s));
+            }
+            bu::BreakupCfg bc;
+            bc.lc_m = ctx->bcfg.lc_m;
+            bc.catastrophic_j_per_g = ctx->bcfg.catastrophic_j_per_g;
+            bc.radius_scale = ctx->cfg.radius_scale;
+            bc.seed = ctx->bcfg.seed;
+            bc.reserve = reserve;
+            bc.sample_cap = ctx->sample_cap;
             k_reset<<<1, 64, 0, s>>>(ctx->gs);
             k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
             double t = t_chunk;
@@ -2181,9 +2544,26 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                         ctx->step_parts);
                 mark(ctx, 1);
                 if (screen) {
+                    if (bk) k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                     if (!enqueue_screen(ctx, ctx->planes(cur), ctx->planes(nxt), t, ctx->step + done + k,
                                         static_cast<int>(k), ctx->dt, ctx->cfg.threshold_km, 0))
                         return false;
+                    if (bk) {
+                        const Planes pb = ctx->planes(cur), pa = ctx->planes(nxt);
+                        int* sc = ctx->sbits ? ctx->scen : nullptr;
+                        k_bu_events<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->events,
+                                                        static_cast<unsigned long long>(ctx->ev_cap), pb, pa,
+                                                        ctx->alive, ctx->mass, ctx->size_m, sc, t, ctx->dt, bc,
+                                                        ctx->bstate, ctx->jobs, ctx->blogs, ctx->blog_cap);
+                        k_bu_sample<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples, bc);
+                        k_bu_budget<<<bu::MAX_JOBS / 64, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
+                                                                      ctx->blogs);
+                        k_bu_bases<<<1, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->blogs, reserve);
+                        k_bu_write<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
+                                                                  static_cast<int>(n0), reserve, pa, ctx->alive,
+                                                                  ctx->radius, ctx->size_m, ctx->mass, sc,
+                                                                  ctx->moon + 2 * k, t, ctx->dt, bc);
+                    }
                 } else {
                     k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0,
                                                  ctx->dense_parts, 0, ctx->exact_parts, 0);
@@ -2195,19 +2575,67 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
             if (!drain_chunk(ctx, ctx->step + done, len, screen, overflow)) return false;
+            bu::BreakupState bs{};
+            if (bk) {
+                KHIP_TRY(ctx, hipMemcpy(&bs, ctx->bstate, sizeof(bs), hipMemcpyDeviceToHost));
+                if (bs.jobs_short || bs.samples_short || bs.reserve_short) {
+                    if (!overflow) {
+                        // drain_chunk kept this chunk's events and statistics: drop them.
+                        ctx->host_events.resize(ctx->host_events.size() - ctx->last_chunk_events);
+                        ctx->host_stats.resize(ctx->host_stats.size() - static_cast<size_t>(len));
+                    }
+                    overflow = true;
+                }
+            }
             if (!overflow) {
                 if (ctx->timing) accumulate_timing(ctx, len, screen != 0);
                 ctx->cur = cur;
                 ctx->t = t;
+                if (bk) {
+                    const int used = std::min(bs.frag_used, reserve);
+                    const int nl = std::min(bs.nlog, ctx->blog_cap);
+                    std::vector<bu::LogRecord> lg(static_cast<size_t>(nl));
+                    if (nl > 0)
+                        KHIP_TRY(ctx, hipMemcpy(lg.data(), ctx->blogs, sizeof(bu::LogRecord) * nl,
+                                                hipMemcpyDeviceToHost));
+                    for (const bu::LogRecord& r : lg) {
+                        khip_breakup_event e;
+                        std::memcpy(&e, &r, sizeof(e));
+                        e.first_index = static_cast<int32_t>(n0) + r.first;
+                        ctx->host_breakups.push_back(e);
+                    }
+                    if (used > 0) {
+                        std::vector<double> sz(static_cast<size_t>(used));
+                        KHIP_TRY(ctx, hipMemcpy(sz.data(), ctx->size_m + n0, sizeof(double) * used,
+                                                hipMemcpyDeviceToHost));
+                        ctx->host_size.insert(ctx->host_size.end(), sz.begin(), sz.end());
+                        ctx->n = n0 + used;
+                    }
+                    ctx->n_launch = ctx->n;
+                }
                 break;
             }
-            // A device buffer was too small: restore the chunk's start and re-run.
+            // A device buffer was too small: restore the chunk's start, grow, re-run.
             ctx->tm.reruns += 1;
             ctx->tm.rerun_reasons |= ctx->last_rerun;
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->state[cur0], ctx->snap_state, sizeof(double) * 6 * c,
                                          hipMemcpyDeviceToDevice, s));
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->alive, ctx->snap_alive, c, hipMemcpyDeviceToDevice, s));
+            if (bk) {
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->mass, ctx->snap_mass, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->bstate, ctx->snap_bstate, sizeof(bu::BreakupState),
+                                             hipMemcpyDeviceToDevice, s));
+                if (bs.reserve_short) ctx->bu_reserve = std::max(2 * ctx->bu_reserve, bs.frag_used + 4096);
+                if (bs.samples_short) {
+                    const int want = std::max(2 * ctx->sample_cap, bs.nsamples + (1 << 16));
+                    if (!dev_alloc(ctx, &ctx->samples, static_cast<size_t>(want))) return false;
+                    ctx->sample_cap = want;
+                }
+                ctx->tm.rerun_reasons |= KHIP_RERUN_BREAKUPS;
+                KHIP_TRY(ctx, hipStreamSynchronize(s));
+            }
         }
+        ctx->n_launch = ctx->n;
         done += len;
     }
     ctx->step += nsteps;
@@ -2290,7 +2718,12 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         ctx->error = "learned form: substeps must divide 12";
         return false;
     }
+    if (ctx->bcfg.enabled) {
+        ctx->error = "learned form: in-run breakups run in the plain and mixed forms only";
+        return false;
+    }
     const int n = static_cast<int>(ctx->n);
+    ctx->n_launch = ctx->n;
     hipStream_t s = ctx->stream;
     const int sub = L.substeps;
     const double h = ctx->dt / sub;
@@ -2305,6 +2738,10 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         return true;
     }
     if (!ensure_learned(ctx)) return false;
+    if (ctx->reset_checks) {
+        KHIP_TRY(ctx, hipMemsetAsync(ctx->cstats, 0, sizeof(CheckStats), s));
+        ctx->reset_checks = false;
+    }
     LearnedParams lp;
     lp.tau = ctx->dt;
     lp.substeps = sub;
@@ -2312,6 +2749,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     lp.e_max = L.e_max;
     lp.a_max = L.a_max_km;
     lp.tol_km = L.tol_km;
+    lp.tol_a_km = L.tol_a_km;
     const double thr = ctx->cfg.threshold_km + L.margin_km;
     const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
     const size_t c = static_cast<size_t>(ctx->cap);
@@ -2629,6 +3067,7 @@ bool upload_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_
                     const double* size_m) {
     if (!ensure_capacity(ctx, n)) return false;
     ctx->n = n;
+    ctx->n_launch = n;
     ctx->cur = 0;
     ctx->host_size.assign(static_cast<size_t>(n), 0.8);
     if (n == 0) return true;
@@ -2646,6 +3085,55 @@ bool upload_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_
     for (int64_t i = 0; i < n; ++i)
         rad[i] = 0.5 * ctx->host_size[i] * 1e-3 * ctx->cfg.radius_scale;
     KHIP_TRY(ctx, hipMemcpy(ctx->radius, rad.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    std::vector<double> sz(c, 0.0);
+    std::copy(ctx->host_size.begin(), ctx->host_size.end(), sz.begin());
+    KHIP_TRY(ctx, hipMemcpy(ctx->size_m, sz.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    KHIP_TRY(ctx, hipMemset(ctx->mass, 0, sizeof(double) * c));
+    return true;
+}
+
+// Grows the capacity to at least `want` objects keeping the live population
+// (states, alive flags, sizes, radii, masses, scenario ids).
+bool grow_preserve(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->cap) return true;
+    const int64_t n = ctx->n;
+    const size_t c0 = static_cast<size_t>(ctx->cap);
+    std::vector<double> st(6 * static_cast<size_t>(n)), ms(static_cast<size_t>(n));
+    std::vector<uint8_t> al(static_cast<size_t>(n));
+    std::vector<int> sc;
+    for (int k = 0; k < 6; ++k)
+        KHIP_TRY(ctx, hipMemcpy(st.data() + k * n, ctx->state[ctx->cur] + k * c0, sizeof(double) * n,
+                                hipMemcpyDeviceToHost));
+    KHIP_TRY(ctx, hipMemcpy(al.data(), ctx->alive, n, hipMemcpyDeviceToHost));
+    KHIP_TRY(ctx, hipMemcpy(ms.data(), ctx->mass, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (ctx->sbits) {
+        sc.resize(static_cast<size_t>(n));
+        KHIP_TRY(ctx, hipMemcpy(sc.data(), ctx->scen, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
+    if (!ensure_capacity(ctx, want)) return false;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    std::vector<double> planes(6 * c, 0.0);
+    for (int k = 0; k < 6; ++k) std::copy(st.begin() + k * n, st.begin() + (k + 1) * n, planes.begin() + k * c);
+    for (int b = 0; b < 2; ++b)
+        KHIP_TRY(ctx, hipMemcpy(ctx->state[b], planes.data(), sizeof(double) * 6 * c, hipMemcpyHostToDevice));
+    al.resize(c, 0);
+    KHIP_TRY(ctx, hipMemcpy(ctx->alive, al.data(), c, hipMemcpyHostToDevice));
+    ms.resize(c, 0.0);
+    KHIP_TRY(ctx, hipMemcpy(ctx->mass, ms.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    std::vector<double> sz(c, 0.0), rad(c, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        sz[i] = ctx->host_size[i];
+        rad[i] = 0.5 * ctx->host_size[i] * 1e-3 * ctx->cfg.radius_scale;
+    }
+    KHIP_TRY(ctx, hipMemcpy(ctx->size_m, sz.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    KHIP_TRY(ctx, hipMemcpy(ctx->radius, rad.data(), sizeof(double) * c, hipMemcpyHostToDevice));
+    if (ctx->sbits) {
+        if (!dev_alloc(ctx, &ctx->scen, c)) return false;
+        sc.resize(c, 0);
+        KHIP_TRY(ctx, hipMemcpy(ctx->scen, sc.data(), sizeof(int) * c, hipMemcpyHostToDevice));
+    }
+    ctx->cur = 0;
+    ctx->force_check = true;
     return true;
 }
 
@@ -3008,6 +3496,9 @@ int khip_set_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8
                      const double* size_m) {
     if (!ctx || n < 0 || (n > 0 && !states) || n >= (1LL << 31) - TILE) return KHIP_ERR_ARG;
     ctx->force_check = true;
+    ctx->reset_checks = true;
+    ctx->reset_breakups = true;
+    ctx->host_breakups.clear();
     ctx->lrep = khip_learned_report{};
     ctx->sbits = 0;
     ctx->scen_off.clear();
@@ -3226,11 +3717,58 @@ int khip_set_precision(khip_ctx* ctx, int mode) {
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
     if (!ctx || !cfg || cfg->substeps < 1 || cfg->substeps > 12 || 12 % cfg->substeps != 0 ||
         cfg->check_every < 1 || !(cfg->tol_km >= 0) || !(cfg->margin_km >= 0) || !(cfg->e_max >= 0) ||
-        !(cfg->a_max_km >= 0))
+        !(cfg->a_max_km >= 0) || !(cfg->tol_a_km >= 0))
         return KHIP_ERR_ARG;
     ctx->lcfg = *cfg;
     ctx->force_check = true;
     ctx->gen += 1;
+    return KHIP_OK;
+}
+
+int khip_set_breakup(khip_ctx* ctx, const khip_breakup_config* cfg) {
+    if (!ctx || !cfg || cfg->reserve < 1 || !(cfg->lc_m > 0) || !(cfg->catastrophic_j_per_g > 0)) return KHIP_ERR_ARG;
+    ctx->bcfg = *cfg;
+    ctx->bu_reserve = cfg->reserve;
+    ctx->gen += 1;
+    return KHIP_OK;
+}
+
+int khip_set_masses(khip_ctx* ctx, int64_t n, const double* mass_kg) {
+    if (!ctx || n != ctx->n || (n > 0 && !mass_kg)) return KHIP_ERR_ARG;
+    if (n > 0 && hipMemcpy(ctx->mass, mass_kg, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        ctx->error = "hipMemcpy of masses failed";
+        return KHIP_ERR_HIP;
+    }
+    return KHIP_OK;
+}
+
+int khip_get_masses(khip_ctx* ctx, int64_t n, double* mass_kg) {
+    if (!ctx || n != ctx->n || (n > 0 && !mass_kg)) return KHIP_ERR_ARG;
+    if (n > 0 && hipMemcpy(mass_kg, ctx->mass, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        ctx->error = "hipMemcpy of masses failed";
+        return KHIP_ERR_HIP;
+    }
+    return KHIP_OK;
+}
+
+int khip_get_sizes(const khip_ctx* ctx, int64_t n, double* size_m) {
+    if (!ctx || n != ctx->n || (n > 0 && !size_m)) return KHIP_ERR_ARG;
+    std::copy(ctx->host_size.begin(), ctx->host_size.begin() + n, size_m);
+    return KHIP_OK;
+}
+
+int khip_take_breakups(khip_ctx* ctx, khip_breakup_event* out, int64_t cap, int64_t* n) {
+    static_assert(sizeof(khip_breakup_event) == sizeof(bu::LogRecord), "breakup record layout");
+    if (!ctx || !n) return KHIP_ERR_ARG;
+    const int64_t have = static_cast<int64_t>(ctx->host_breakups.size());
+    if (!out) {
+        *n = have;
+        return KHIP_OK;
+    }
+    const int64_t m = std::min(have, cap);
+    std::copy(ctx->host_breakups.begin(), ctx->host_breakups.begin() + m, out);
+    ctx->host_breakups.erase(ctx->host_breakups.begin(), ctx->host_breakups.begin() + m);
+    *n = m;
     return KHIP_OK;
 }
 

@@ -1,0 +1,146 @@
+"""GPU tier: in-run breakups on the device (khip_set_breakup).
+
+Scene: pairs of LEO objects set on head-on-ish courses (relative speed
+10 km/s) that meet half-way through the first 10 s step, far from each
+other; radius scale 1000 so every meeting is a collision, and every
+collision is catastrophic (500 m_p v^2 / m_t = 25,000 J/g >> 40). For the
+distributions: 40 pairs of 10 t / 5 t parents 0.5 m across, so the
+fragments' mass stays far below the budget and the drawn size law is what
+is kept. For determinism and chunking: 200 pairs of 1 t / 0.5 t parents.
+
+The device draws come from a counter-based generator, so the fragments are
+checked against the model's closed forms rather than against the
+reference's exact draws:
+  * fragment count per event ~ Poisson(0.1 M^0.75 L_c^-1.71), M = 1,500 kg;
+  * sizes follow the power law truncated to [L_c, parent size]
+    (Kolmogorov-Smirnov against the exact CDF);
+  * log10(delta-v [m/s]) - (0.9 log10(A/m) + 2.9) ~ N(0, 0.4);
+  * fragment mass never exceeds the parents' mass; both parents die;
+  * fragments are screened from the next step on; results do not depend on
+    chunking or on how often the reserved object room had to grow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPOCH = 2461253.9166666665
+MU = 398600.4418
+LC = 0.1
+SLOPE = 1.71
+PARENT_L = 0.5
+
+
+def _scene(npairs=200, seed=3, masses_kg=(1000.0, 500.0)):
+    rng = np.random.default_rng(seed)
+    states, sizes, masses = [], [], []
+    for _ in range(npairs):
+        rhat = rng.normal(size=3)
+        rhat /= np.linalg.norm(rhat)
+        r0 = rhat * (6378.137 + 600.0)
+        vc = math.sqrt(MU / np.linalg.norm(r0))
+        a = np.cross(rhat, rng.normal(size=3))
+        a /= np.linalg.norm(a)
+        vt = a * vc
+        theta = 2 * math.asin(10.0 / (2 * vc))  # |vp - vt| = 10 km/s
+        b = np.cross(rhat, a)
+        vp = (a * math.cos(theta) + b * math.sin(theta)) * vc
+        for v, size, m in ((vt, PARENT_L, masses_kg[0]), (vp, PARENT_L, masses_kg[1])):
+            states.append(np.concatenate([r0 - v * 5.0, v]))  # meet at t = 5 s
+            sizes.append(size)
+            masses.append(m)
+    return np.array(states), np.array(sizes), np.array(masses)
+
+
+def _run(engine, steps=3, chunk=256, reserve=65536, seed=9, npairs=200, masses_kg=(1000.0, 500.0)):
+    s, size, mass = _scene(npairs, masses_kg=masses_kg)
+    engine.set_precision(0)
+    engine.set_chunk(chunk)
+    engine.set_objects(s, np.ones(len(s), np.uint8), size)
+    engine.set_masses(mass)
+    engine.set_config(5.0, 1000.0, 0.01)
+    engine.set_time(EPOCH, 0.0, 10.0)
+    engine.set_breakup(True, reserve=reserve, lc_m=LC, seed=seed)
+    try:
+        engine.run(steps, screen=True)
+    finally:
+        engine.set_breakup(False)
+        engine.set_chunk(256)
+    states, alive = engine.get_objects()
+    return (states, alive, engine.take_events(), engine.take_breakups(), engine.sizes(), engine.masses(),
+            engine.take_stats())
+
+
+def _ks(sample, cdf):
+    x = np.sort(sample)
+    f = cdf(x)
+    n = len(x)
+    return max(np.max(np.arange(1, n + 1) / n - f), np.max(f - np.arange(n) / n))
+
+
+def test_breakups_follow_the_model(engine):
+    npairs = 40
+    n0 = 2 * npairs
+    states, alive, ev, bk, size, mass, _ = _run(engine, steps=1, npairs=npairs, masses_kg=(10000.0, 5000.0))
+    coll = ev[ev["collision"] == 1]
+    print(f"{len(bk)} breakups, {int(bk['fragments'].sum())} fragments, {int(bk['escaped'].sum())} escaped; "
+          f"collisions in the step: {len(coll)}")
+    assert len(bk) == npairs and len(coll) >= npairs
+    assert (bk["catastrophic"] == 1).all()
+    assert np.allclose(bk["energy_j_per_g"], 500 * 5000 * 100 / 10000, rtol=1e-3)
+    # Both parents of every event die.
+    assert not alive[:n0].any()
+    # Counts: Poisson(expected).
+    expected = 0.1 * 15000 ** 0.75 * LC ** -SLOPE
+    assert np.allclose(bk["expected_fragments"], expected, rtol=1e-12)
+    z = (bk["drawn"] - expected) / math.sqrt(expected)
+    print(f"drawn per event: mean z {z.mean():.3f}, var {z.var():.3f}")
+    assert abs(z.mean()) < 0.6 and 0.4 < z.var() < 1.8
+    # Mass budget and bookkeeping.
+    assert (bk["fragments_mass_kg"] <= bk["sbm_mass_kg"] + 1e-9).all()
+    nfrag = int(bk["fragments"].sum())
+    assert len(states) == n0 + nfrag
+    order = np.argsort(bk["first_index"])
+    starts = bk["first_index"][order]
+    assert starts[0] == n0 and np.array_equal(np.diff(starts), bk["fragments"][order][:-1])
+    frag = slice(n0, n0 + nfrag)
+    assert alive[frag].mean() > 0.99
+    # Sizes: the power law truncated to [L_c, parent size].
+    L = size[frag]
+    assert (L >= LC).all() and (L <= PARENT_L).all()
+    assert (bk["fragments_mass_kg"] < 0.5 * bk["sbm_mass_kg"]).all()  # the budget never binds here
+    cdf = lambda x: (1 - (x / LC) ** -SLOPE) / (1 - (PARENT_L / LC) ** -SLOPE)
+    d = _ks(L, cdf)
+    print(f"sizes: KS distance {d:.4f} over {len(L)} fragments")
+    assert d < 1.63 / math.sqrt(len(L))  # 1% level
+    # Delta-v law: fragment velocity minus its parent's at the end of the step
+    # (the parents are frozen there; same place, same gravity). The parent is
+    # the one of the event whose velocity is closer.
+    ev_of = order[np.searchsorted(starts, np.arange(n0, n0 + nfrag), side="right") - 1]
+    vt = states[bk["target"][ev_of], 3:]
+    vp = states[bk["projectile"][ev_of], 3:]
+    vf = states[frag, 3:]
+    near_t = np.linalg.norm(vf - vt, axis=1) < np.linalg.norm(vf - vp, axis=1)
+    dv = np.linalg.norm(vf - np.where(near_t[:, None], vt, vp), axis=1) * 1e3
+    chi = np.log10(0.540424 * L * L / mass[frag])
+    res = np.log10(dv) - (0.9 * chi + 2.9)
+    print(f"delta-v residual: mean {res.mean():.4f}, std {res.std():.4f}")
+    assert abs(res.mean()) < 0.02 and abs(res.std() - 0.4) < 0.02
+
+
+def test_breakups_are_deterministic_and_chunk_independent(engine):
+    a = _run(engine, steps=4, chunk=256, reserve=65536)
+    b = _run(engine, steps=4, chunk=1, reserve=1000)  # the reserved room grows several times
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert len(a[2]) == len(b[2]) and np.array_equal(np.sort(a[2], order=["step", "t", "i", "j"]),
+                                                     np.sort(b[2], order=["step", "t", "i", "j"]))
+    assert np.array_equal(a[3], b[3])
+    assert np.array_equal(a[6]["candidate_pairs"], b[6]["candidate_pairs"])
+    # Fragments take part in screening from the step after their birth: the
+    # siblings of each cloud are candidate pairs (and recede, so the exact
+    # filter rejects them).
+    st = a[6]
+    print(f"candidate pairs per step: {st['candidate_pairs'].tolist()}, refined {st['refined_pairs'].tolist()}")
+    assert st["candidate_pairs"][0] < 10000 and st["candidate_pairs"][1] > 1_000_000
