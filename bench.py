@@ -229,6 +229,11 @@ def recall_against(e_ref, e_new):
     return out
 
 
+# HBM bytes per k_pairs launch of the learned workload, measured by two
+# rocprofv3 PMC passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from copy
+# kernels of known size in the same runs).
+K_PAIRS_TRAFFIC_B = 136.5e6
+
 # BF16 matrix-core flops of one corrector evaluation (10-128-128-128-6;
 # the input and output layers count once although they run hi/lo splits).
 FLOP_PER_INFERENCE = 2.0 * (10 * 128 + 128 * 128 + 128 * 128 + 128 * 6)
@@ -251,7 +256,7 @@ def learned_workload(kamd, args):
     pop = kamd.generate(1_000_000, seed=0, mix=(0.7, 0.1, 0.1, 0.1, 1.5))
     alive0 = pop["alive"].astype(np.uint8)
     eng = kamd.Engine(0)
-    train_rep = train_corrector(eng, kamd)
+    train_rep = train_corrector(eng, kamd, args.train_iters)
 
     def run(precision, dt, nsteps, nwarm):
         eng.set_precision(precision)
@@ -299,9 +304,13 @@ def learned_workload(kamd, args):
     if units is not None:
         ach = FLOP_PER_CANDIDATE * units / (ms * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "valu-fp32", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
+                "traffic": K_PAIRS_TRAFFIC_B if dominant == "k_pairs" else None,
                 "avg_launch_us": 1e3 * ms / launches_sub, "units_per_launch": units / launches_sub,
                 "unit_of_work": "candidate pair (60 FP32 flops)"}
+        if dominant == "k_pairs":
+            roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, "
+                                      "calibrated on copy kernels: profiles/learned1M_pmc_traffic.md")
     else:
         roof = {"kernel": dominant, "bound": "valu-fp64", "achieved": None, "peak": 78.6, "unit": "TFLOP/s",
                 "frac": None, "traffic": None, "avg_launch_us": 1e3 * ms / launches_sub}
@@ -338,11 +347,11 @@ def learned_workload(kamd, args):
     print(json.dumps(out))
 
 
-def train_corrector(eng, kamd):
+def train_corrector(eng, kamd, iterations=20000):
     eng.set_time(EPOCH, 0.0, 60.0)
     t0 = time.perf_counter()
     rep, _ = eng.net_train(kamd.generate(500000, seed=101)["state"], samples_per_orbit=10, spacing_steps=12,
-                           iterations=20000, batch=16384, lr_max=3e-3, seed=1)
+                           iterations=iterations, batch=16384, lr_max=3e-3, seed=1)
     rep["wall_s"] = time.perf_counter() - t0
     return rep
 
@@ -373,7 +382,7 @@ def ensemble_workload(kamd, args):
     pop = np.concatenate(pops)
     alive0 = pop["alive"].astype(np.uint8)
     eng = kamd.Engine(0)
-    train = train_corrector(eng, kamd)
+    train = train_corrector(eng, kamd, args.train_iters)
 
     def run(precision, dt, nsteps, piece):
         eng.set_precision(precision)
@@ -462,6 +471,8 @@ def main():
                     help="timed steps (learned: 60 s steps, default 1440; cascade: 10 s steps, default 8640)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--train-iters", type=int, default=20000,
+                    help="corrector training steps before the timed runs (fewer only for profiling runs)")
     ap.add_argument("--workload", default="learned", choices=("learned", "cascade", "precision", "ensemble"),
                     help="learned: the learned corrector at 1M objects vs plain FP64 (default, the headline "
                          "line); cascade: plain FP64 at cascade scale; precision: mixed precision at 1M; "
