@@ -43,7 +43,7 @@ METRIC = "object-steps/s incl. screening, learned/mixed vs plain FP64 at <=10 m/
 EPOCH = 2461253.916666667
 DT = 10.0
 THRESHOLD_KM = 5.0
-# Algorithmic work per unit (see DESIGN.md): FP32 flops per candidate pair of
+# Algorithmic work per unit: FP32 flops per candidate pair of
 # the pre-filter (prefilter_keep + building dr, dv), HBM bytes per propagated
 # object of the RK4 step (FP64 state in and out, alive flag).
 FLOP_PER_CANDIDATE = 60.0
@@ -142,7 +142,7 @@ def match_events(e_ref, e_new):
 
 
 def precision_workload(kamd, args):
-    """1,000,000 objects (LEO shells densified), dt 10 s, 24 h,
+    """Mixed precision at 1M: 1,000,000 objects (LEO shells densified), dt 10 s, 24 h,
     screening at 2 km; the precision form (FP32 J2 + Moon, FP64 state and
     central term) against the plain FP64 path on the same catalogue."""
     thr = 2.0
@@ -241,7 +241,7 @@ PEAK_BF16_TFLOPS = 2500.0
 
 
 def learned_workload(kamd, args):
-    """1,000,000 synthetic objects (configs[0] orbit mix, LEO
+    """Learned form at 1M: 1,000,000 synthetic objects (the catalogue orbit mix, LEO
     shells densified), 24 h, screening at 2 km. The corrector is trained on
     the device first (5M samples from 500,000 seeded orbits, seed 101; not
     timed). Timed: the learned form at 60 s steps (corrector + mixed RK4,
@@ -357,9 +357,9 @@ def train_corrector(eng, kamd, iterations=20000):
 
 
 def ensemble_workload(kamd, args):
-    """64 Monte Carlo cascade scenarios of uneven size in one
+    """Ensemble: 64 Monte Carlo cascade scenarios of uneven size in one
     concatenated population (per-scenario offsets; screening pairs objects of
-    one scenario only). Scenario k: a seeded catalogue (configs[0] orbit mix,
+    one scenario only). Scenario k: a seeded catalogue (the catalogue orbit mix,
     seed k) of N_k objects, N_k log-uniform in [30k, 300k], plus B_k seeded
     catastrophic breakups at t = 0 (B_k uniform in 0..60, 100-2000 kg,
     8-12 km/s, L_c = 0.1 m). The learned form (60 s steps, plain fallback,
@@ -480,7 +480,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: --gpus must be 1 "
-                                   "(multi-GPU sharding is a design in DESIGN.md, not built)"}))
+                                   "(multi-GPU sharding is designed, not built)"}))
         sys.exit(2)
 
     kamd = load_package()

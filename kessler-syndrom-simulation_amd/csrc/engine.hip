@@ -498,42 +498,57 @@ __global__ void __launch_bounds__(BLOCK) k_build_lists(const uint8_t* alive, con
     (void)tot;
 }
 
-// Dense output: states at fraction s of the step from the cubic Hermite of
-// each object's start and end states (positions and velocities).
-__global__ void __launch_bounds__(BLOCK) k_dense_output(Planes a, Planes b, int n, double dt, double s,
-                                                         Planes out) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= n) return;
+// Dense output: state at fraction s of the step from the cubic Hermite of an
+// object's start and end states (positions and velocities).
+__device__ __forceinline__ void hermite_state(const double x0[3], const double v0[3], const double x1[3],
+                                              const double v1[3], double dt, double s, double p[3], double w[3]) {
     const double s2 = s * s, s3 = s2 * s;
     const double c0 = 2 * s3 - 3 * s2 + 1, c1 = s3 - 2 * s2 + s, c2 = -2 * s3 + 3 * s2, c3 = s3 - s2;
     const double d0 = 6 * s2 - 6 * s, d1 = 3 * s2 - 4 * s + 1, d2 = -6 * s2 + 6 * s, d3 = 3 * s2 - 2 * s;
-    const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
-    const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
-    double p[3], w[3];
     for (int k = 0; k < 3; ++k) {
         p[k] = x0[k] * c0 + v0[k] * dt * c1 + x1[k] * c2 + v1[k] * dt * c3;
         w[k] = (x0[k] * d0 + v0[k] * dt * d1 + x1[k] * d2 + v1[k] * dt * d3) / dt;
     }
-    out.x[i] = p[0]; out.y[i] = p[1]; out.z[i] = p[2];
-    out.vx[i] = w[0]; out.vy[i] = w[1]; out.vz[i] = w[2];
 }
 
-// Reductions for a midpoint grid: max of |v| over both ends, bounding box
-// of the midpoints, of the objects alive at the end of the step.
-__global__ void __launch_bounds__(BLOCK) k_reduce_mid(Planes a, Planes b, const uint8_t* alive, int n,
-                                                       StepPartial* partials) {
+// Sub-step q of sub of a learned step from a (start) to b (end): its start
+// state (a itself for q = 0, else the dense output at q / sub, recomputed
+// bit for bit as the previous sub-step wrote it) and end state (b itself
+// for the last sub-step, else the dense output at (q + 1) / sub, written to
+// out), and the reductions for its midpoint grid: max of |v| over both ends
+// and the bounding box of the midpoints, of the objects alive at the end of
+// the step. One pass over a and b.
+__global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uint8_t* alive, int n, double dt,
+                                                    int q, int sub, Planes out, StepPartial* partials) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     bool on = false;
     double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-    if (i < n && alive[i]) {
-        on = true;
-        rx = 0.5 * (a.x[i] + b.x[i]);
-        ry = 0.5 * (a.y[i] + b.y[i]);
-        rz = 0.5 * (a.z[i] + b.z[i]);
-        const double va = sqrt(a.vx[i] * a.vx[i] + a.vy[i] * a.vy[i] + a.vz[i] * a.vz[i]);
-        const double vb = sqrt(b.vx[i] * b.vx[i] + b.vy[i] * b.vy[i] + b.vz[i] * b.vz[i]);
-        vn = va > vb ? va : vb;
+    if (i < n) {
+        const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
+        const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
+        double ps[3], ws[3], pe[3], we[3];
+        if (q == 0) {
+            for (int k = 0; k < 3; ++k) ps[k] = x0[k], ws[k] = v0[k];
+        } else {
+            hermite_state(x0, v0, x1, v1, dt, static_cast<double>(q) / sub, ps, ws);
+        }
+        if (q == sub - 1) {
+            for (int k = 0; k < 3; ++k) pe[k] = x1[k], we[k] = v1[k];
+        } else {
+            hermite_state(x0, v0, x1, v1, dt, static_cast<double>(q + 1) / sub, pe, we);
+            out.x[i] = pe[0]; out.y[i] = pe[1]; out.z[i] = pe[2];
+            out.vx[i] = we[0]; out.vy[i] = we[1]; out.vz[i] = we[2];
+        }
+        if (alive[i]) {
+            on = true;
+            rx = 0.5 * (ps[0] + pe[0]);
+            ry = 0.5 * (ps[1] + pe[1]);
+            rz = 0.5 * (ps[2] + pe[2]);
+            const double va = sqrt(ws[0] * ws[0] + ws[1] * ws[1] + ws[2] * ws[2]);
+            const double vb = sqrt(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
+            vn = va > vb ? va : vb;
+        }
     }
     StepPartial p = object_partial(on, on, vn, rx, ry, rz);
     const StepPartial t = block_reduce(p, s_rows);
@@ -647,14 +662,14 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
 
 
 // Also writes, in object order (coalesced), each alive object's compact
-// FP32 record for the pair kernels: position relative to its cell's low
-// corner (FP64 subtraction, then rounding: the error DR_ERR_KM bounds) with
-// the object index in w, and velocity with the cell's packed z coordinate
-// in w. k_cell_write then gathers one 32-byte record per sorted object.
+// 32-byte FP32 record for the pair kernels: position relative to its cell's
+// low corner (FP64 subtraction, then rounding: the error DR_ERR_KM bounds)
+// with the object index in w, then velocity with the cell's packed z
+// coordinate in w. k_cell_write gathers one record (one cache line) per
+// sorted object.
 __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  const int* __restrict__ scen, int n, const GridState* gs,
-                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rpos,
-                                                 float4* rvel) {
+                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
@@ -679,11 +694,11 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midp
                     (static_cast<uint64_t>(iy - g.imin[1]) << g.bits[2]) | pz;
                 if (g.sbits)
                     k |= static_cast<uint64_t>(scen[e]) << (g.bits[0] + g.bits[1] + g.bits[2]);
-                rpos[e] = make_float4(static_cast<float>(x - static_cast<double>(ix) * c),
-                                      static_cast<float>(y - static_cast<double>(iy) * c),
-                                      static_cast<float>(z - static_cast<double>(iz) * c), __int_as_float(e));
-                rvel[e] = make_float4(static_cast<float>(cur.vx[e]), static_cast<float>(cur.vy[e]),
-                                      static_cast<float>(cur.vz[e]), __int_as_float(static_cast<int>(pz)));
+                rec[2 * e] = make_float4(static_cast<float>(x - static_cast<double>(ix) * c),
+                                         static_cast<float>(y - static_cast<double>(iy) * c),
+                                         static_cast<float>(z - static_cast<double>(iz) * c), __int_as_float(e));
+                rec[2 * e + 1] = make_float4(static_cast<float>(cur.vx[e]), static_cast<float>(cur.vy[e]),
+                                             static_cast<float>(cur.vz[e]), __int_as_float(static_cast<int>(pz)));
             }
             keys[e] = k;
             idx[e] = e;
@@ -849,59 +864,65 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
     if (threadIdx.x == 0) gs->ncells = static_cast<unsigned int>(carry);
 }
 
-// Cell table and compact records in sorted order. Thread t of tile b takes
-// the ITEMS consecutive sorted positions b * TILE + t * ITEMS + r; one block
-// scan of the per-thread head counts numbers the cells.
+// Cell table and compact records in sorted order. Element e = tile +
+// r * BLOCK + thread (r < ITEMS), so every load and store is coalesced; cells
+// are numbered from wave ballots of the head flags and one block-wide
+// prefix over the tile's ITEMS x 4 wave segments (in element order).
 //   lpos: position relative to its cell's low corner, w = object index bits;
 //   lvel: velocity, w = the cell's packed z coordinate (int bits), from
 //         which pairs get their z cell offset.
 __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
-                                                       const float4* __restrict__ rpos,
-                                                       const float4* __restrict__ rvel, uint64_t* cell_key,
+                                                       const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lpos, float4* lvel) {
-    __shared__ int s_wave[WAVES_PER_BLOCK];
+    __shared__ int s_seg[ITEMS * WAVES_PER_BLOCK];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
     const int base = blockIdx.x * TILE;
     if (base >= A || g.npasses == 0) return;
     const uint64_t* keys = sorted_keys(g, k0, k1);
     const int* sidx = sorted_idx(g, i0, i1);
-    const int e0 = base + threadIdx.x * ITEMS;
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
+    const unsigned long long below = lanes_below();
     uint64_t key[ITEMS];
-    int id[ITEMS];
+    unsigned long long bal[ITEMS];
     bool head[ITEMS];
-    uint64_t prev = e0 > 0 && e0 - 1 < A ? keys[e0 - 1] : ~0ULL;
-    int cnt = 0;
-#pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = e0 + r;
-        const bool valid = e < A;
-        key[r] = valid ? keys[e] : 0ULL;
-        id[r] = valid ? sidx[e] : 0;
-        head[r] = valid && (e == 0 || key[r] != prev);
-        prev = key[r];
-        cnt += head[r] ? 1 : 0;
-    }
     float4 p[ITEMS], v[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        if (e0 + r < A) {
-            p[r] = rpos[id[r]];
-            v[r] = rvel[id[r]];
+        const int e = base + r * BLOCK + threadIdx.x;
+        const bool valid = e < A;
+        key[r] = valid ? keys[e] : 0ULL;
+        uint64_t prev = __shfl_up(key[r], 1, WAVE);
+        if (lane == 0) prev = valid && e > 0 ? keys[e - 1] : ~0ULL;
+        head[r] = valid && (e == 0 || key[r] != prev);
+        bal[r] = __ballot(head[r]);
+        if (lane == 0) s_seg[r * WAVES_PER_BLOCK + wave] = __popcll(bal[r]);
+        if (valid) {
+            const int id = sidx[e];
+            p[r] = rec[2 * id];
+            v[r] = rec[2 * id + 1];
         }
     }
-    int tot;
-    const int ex = block_exclusive_scan(cnt, tot, s_wave);
-    int c = boff[blockIdx.x] + ex - 1;
+    __syncthreads();
+    int run = boff[blockIdx.x];
+    int pre[ITEMS];
 #pragma unroll
     for (int r = 0; r < ITEMS; ++r) {
-        const int e = e0 + r;
+        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+            if (w == wave) pre[r] = run;
+            run += s_seg[r * WAVES_PER_BLOCK + w];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const int e = base + r * BLOCK + threadIdx.x;
         if (e >= A) break;
+        const int c = pre[r] + __popcll(bal[r] & below) + (head[r] ? 1 : 0) - 1;
         if (head[r]) {
-            ++c;
             cell_key[c] = key[r];
             cell_begin[c] = e;
         }
@@ -1235,8 +1256,16 @@ __device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, i
     append_event(gs, P, ev);
 }
 
+// One candidate, in both pair kernels: FP32 relative state from the compact
+// records (positions relative to the cell corner, cell offset (dx, dy, dz)
+// in whole cells, built as in the reference's GPU path so that DR_ERR_KM
+// bounds it), with the partner cell's (x, y) corner offset folded into the
+// own position: (ox + (lb.x - la.x)) regrouped as lb.x - (la.x - ox), the
+// same operands with one rounding each, inside DR_ERR_KM. Pairs the
+// pre-filter keeps go to the survivor queue for the FP64 kernel.
+
 // Appends a pre-filter survivor, wave-aggregated among the lanes that
-// reach this point together.
+// reach this point together (sparse cells, where survivors are few).
 __device__ __forceinline__ void push_survivor(GridState* gs, const PairParams& P, int a, int b) {
     const unsigned long long m = __ballot(1);
     const int leader = __ffsll(static_cast<long long>(m)) - 1;
@@ -1247,21 +1276,6 @@ __device__ __forceinline__ void push_survivor(GridState* gs, const PairParams& P
     base = __shfl(base, leader, WAVE);
     const unsigned long long slot = base + rank;
     if (slot < P.surv_cap) P.surv[slot] = make_uint2(static_cast<unsigned int>(a), static_cast<unsigned int>(b));
-}
-
-// One candidate: FP32 relative state from the compact records (positions
-// relative to the cell corner, cell offset (dx, dy, dz) in whole cells,
-// built as in the reference's GPU path so that DR_ERR_KM bounds it). Pairs
-// the pre-filter keeps go to the survivor queue for the FP64 kernel.
-__device__ __forceinline__ void candidate(GridState* gs, const PairParams& P, float4 la, float4 va,
-                                          float4 lb, float4 vb, float ox, float oy, float cellf,
-                                          unsigned int& cand) {
-    ++cand;
-    const float oz = static_cast<float>(__float_as_int(vb.w) - __float_as_int(va.w)) * cellf;
-    if (prefilter_keep_fast(ox + (lb.x - la.x), oy + (lb.y - la.y), oz + (lb.z - la.z),
-                            vb.x - va.x, vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
-                            P.vmin2_f))
-        push_survivor(gs, P, __float_as_int(la.w), __float_as_int(lb.w));
 }
 
 __device__ __forceinline__ void store_counters(PairPartial* parts, const PairCounters& cnt,
@@ -1289,58 +1303,162 @@ __device__ __forceinline__ void store_counters(PairPartial* parts, const PairCou
     }
 }
 
-// Sparse cells: one lane per sorted object of a cell with fewer than
-// DENSE_MIN objects, walking its cell's five ranges. The next partner's
-// record is loaded before the current one is tested.
+constexpr unsigned int SURV_INVALID = 0xffffffffu;
+
+// Per-wave survivor slots: reservations that grow geometrically (64 ..
+// 4096), so a fresh cloud whose every pair survives costs one counter
+// atomic per thousands of survivors; unused reserved slots are marked
+// SURV_INVALID. push() is called by the whole wave.
+struct SurvivorSlots {
+    unsigned long long base = 0;  // wave-uniform
+    unsigned int left = 0, size = 32;
+
+    __device__ __forceinline__ void push(GridState* gs, const PairParams& P, bool keep, int a, int b, int lane,
+                                         unsigned long long below) {
+        const unsigned long long m = __ballot(keep);
+        if (m == 0ULL) return;
+        const unsigned int cnt = static_cast<unsigned int>(__popcll(m));
+        if (left < cnt) {
+            if (static_cast<unsigned int>(lane) < left && base + lane < P.surv_cap)
+                P.surv[base + lane] = make_uint2(SURV_INVALID, SURV_INVALID);
+            size = size < 4096u ? 2u * size : 4096u;
+            const unsigned int R = size > cnt ? size : cnt;
+            unsigned long long r = 0;
+            if (lane == 0) r = atomicAdd(&gs->n_surv, static_cast<unsigned long long>(R));
+            base = __shfl(r, 0, WAVE);
+            left = R;
+        }
+        if (keep) {
+            const unsigned long long slot = base + __popcll(m & below);
+            if (slot < P.surv_cap)
+                P.surv[slot] = make_uint2(static_cast<unsigned int>(a), static_cast<unsigned int>(b));
+        }
+        base += cnt;
+        left -= cnt;
+    }
+
+    __device__ __forceinline__ void close(const PairParams& P, int lane) {
+        for (unsigned int k = static_cast<unsigned int>(lane); k < left; k += WAVE)
+            if (base + k < P.surv_cap) P.surv[base + k] = make_uint2(SURV_INVALID, SURV_INVALID);
+    }
+};
+
+// Sparse cells. A unit is one (own object, neighbour range) pair: the run
+// of partners the object is tested against (range 0 from the position after
+// its own). A block takes PAIR_GROUP consecutive sorted objects, ranks their
+// units by length in LDS (counting sort, longest first) and hands them to
+// its waves in rounds of 64, so the lanes of a round walk runs of nearly
+// equal length. A lane loads PAIR_BATCH partner records at a time into the
+// registers that test them (no copies of loads in flight, which would make
+// every iteration wait for its own loads). Survivors are few here: they are
+// appended one wave-aggregated atomic at a time (a per-wave reservation
+// would need wave-uniform loops and leave unused slots for k_exact).
+constexpr int PAIR_GROUP = BLOCK;   // sorted objects per block, one per thread
+constexpr int UNIT_BUCKETS = 64;    // unit lengths 1 .. 63, and >= 64
+constexpr int UNIT_OBJ_BITS = 8;    // log2(PAIR_GROUP)
+constexpr int PAIR_BATCH = 4;
+static_assert((1 << UNIT_OBJ_BITS) == PAIR_GROUP && UNIT_BUCKETS == WAVE, "unit packing");
+
 __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
-    const int p = blockIdx.x * BLOCK + threadIdx.x;
+    __shared__ int s_hist[UNIT_BUCKETS];
+    __shared__ int s_total;
+    __shared__ int2 s_unit[PAIR_GROUP * NRANGES];
     const int A = static_cast<int>(gs->alive);
+    const int g0 = blockIdx.x * PAIR_GROUP;
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
     unsigned int cand = 0;
-    if (p < A && gs->npasses > 0) {
-        const int c = P.cell_of[p];
-        const int long_range = gs->long_range;
-        if (P.cell_count[c] < gs->dense_min) {
-            const float cellf = static_cast<float>(gs->cell);
-            const float4 la = P.lpos[p], va = P.lvel[p];
-            for (int r = 0; r < NRANGES; ++r) {
-                const int2 run = P.ranges[c * NRANGES + r];
-                if (run.y - run.x > long_range) continue;  // a work item covers it
-                int dx, dy;
-                range_offset(r, dx, dy);
-                const float ox = static_cast<float>(dx) * cellf, oy = static_cast<float>(dy) * cellf;
-                int q = r == 0 ? p + 1 : run.x;
-                if (q >= run.y) continue;
-                float4 lb = P.lpos[q], vb = P.lvel[q];
-                for (; q < run.y; ++q) {
-                    const int nq = q + 1 < run.y ? q + 1 : q;
-                    const float4 nlb = P.lpos[nq], nvb = P.lvel[nq];
-                    candidate(gs, P, la, va, lb, vb, ox, oy, cellf, cand);
-                    lb = nlb;
-                    vb = nvb;
+    if (gs->npasses > 0 && g0 < A) {  // uniform over the block
+        if (threadIdx.x < UNIT_BUCKETS) s_hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int long_range = gs->long_range, dmin = gs->dense_min;
+        const int p = g0 + threadIdx.x;
+        const int c = p < A ? P.cell_of[p] : 0;
+        const bool own = p < A && P.cell_count[c] < dmin;
+        int q0[NRANGES], len[NRANGES], rank[NRANGES];
+#pragma unroll
+        for (int r = 0; r < NRANGES; ++r) {
+            int2 run = make_int2(0, 0);
+            if (own) run = P.ranges[c * NRANGES + r];
+            const int first = r == 0 ? p + 1 : run.x;
+            const int l = run.y - run.x > long_range ? 0 : run.y - first;  // long ranges: work items
+            q0[r] = first;
+            len[r] = l > 0 ? l : 0;
+            rank[r] = 0;
+            if (l > 0) rank[r] = atomicAdd(&s_hist[UNIT_BUCKETS - (l < UNIT_BUCKETS ? l : UNIT_BUCKETS)], 1);
+        }
+        __syncthreads();
+        // Exclusive scan of the bucket counts (one wave, one bucket per lane).
+        if (wave == 0) {
+            const int v = s_hist[lane];
+            int incl = v;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const int t = __shfl_up(incl, d, WAVE);
+                if (lane >= d) incl += t;
+            }
+            s_hist[lane] = incl - v;
+            if (lane == WAVE - 1) s_total = incl;
+        }
+        __syncthreads();
+        const int total = s_total;
+#pragma unroll
+        for (int r = 0; r < NRANGES; ++r) {
+            const int l = len[r];
+            if (l == 0) continue;
+            const int bucket = UNIT_BUCKETS - (l < UNIT_BUCKETS ? l : UNIT_BUCKETS);
+            s_unit[s_hist[bucket] + rank[r]] =
+                make_int2(q0[r], static_cast<int>(threadIdx.x) | (r << UNIT_OBJ_BITS) | (l << (UNIT_OBJ_BITS + 3)));
+        }
+        __syncthreads();
+        const float cellf = static_cast<float>(gs->cell);
+        for (int u = wave * WAVE + lane; u - lane < total; u += BLOCK) {
+            if (u >= total) continue;
+            const int2 d = s_unit[u];
+            const int po = g0 + (d.y & (PAIR_GROUP - 1));
+            const int r = (d.y >> UNIT_OBJ_BITS) & 7;
+            const int ql = d.x + (d.y >> (UNIT_OBJ_BITS + 3)) - 1;  // last partner
+            int dx, dy;
+            range_offset(r, dx, dy);
+            const float4 la = P.lpos[po], va = P.lvel[po];
+            const float ax = la.x - static_cast<float>(dx) * cellf;
+            const float ay = la.y - static_cast<float>(dy) * cellf;
+            for (int q = d.x; q <= ql; q += PAIR_BATCH) {
+                float4 b[PAIR_BATCH], w[PAIR_BATCH];
+#pragma unroll
+                for (int k = 0; k < PAIR_BATCH; ++k) {
+                    const int qq = q + k < ql ? q + k : ql;
+                    b[k] = P.lpos[qq];
+                    w[k] = P.lvel[qq];
+                }
+#pragma unroll
+                for (int k = 0; k < PAIR_BATCH; ++k) {
+                    if (q + k > ql) continue;
+                    ++cand;
+                    const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
+                    if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
+                                            w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f))
+                        push_survivor(gs, P, __float_as_int(la.w), __float_as_int(b[k].w));
                 }
             }
         }
+        __syncthreads();  // s_cnt is reused below
     }
     store_counters(parts, PairCounters{cand, 0, 0}, s_cnt);
 }
 
-__device__ __forceinline__ float4 lane_broadcast(float4 v, int j) {
-    return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), j)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), j)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.z), j)),
-                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.w), j)));
-}
+constexpr int PARTNER_BATCH = 4;
 
-constexpr unsigned int SURV_INVALID = 0xffffffffu;
-
-// Dense cells: one wave per work item (grid-stride over the items). The
-// wave loads 64 partner records at a time (one per lane, coalesced) and
-// broadcasts them one by one with v_readlane, so the inner loop touches no
-// memory. Survivor slots come from per-wave reservations that grow
-// geometrically (64 .. 4096), so a fresh cloud whose every pair survives
-// costs one counter atomic per thousands of survivors; unused reserved slots
-// are marked SURV_INVALID.
+// Dense cells and long ranges: one wave per work item (grid-stride over
+// the items), in one of two orientations, whichever takes fewer wave
+// iterations:
+//   own objects on the lanes: every lane tests its own object against the
+//     same partner, whose records are wave-uniform scalar loads
+//     (PARTNER_BATCH in flight);
+//   partners on the lanes (few own objects, a long run: the long ranges of
+//     sparse cells): 64 partners per lane load, the own objects' records
+//     scalar loads.
+// Both evaluate the same expression on the same operands, own object first.
 __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
                                                         const PairItem* __restrict__ items,
                                                         const float4* __restrict__ lpos,
@@ -1354,10 +1472,54 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
     const float cellf = static_cast<float>(gs->cell);
     const unsigned long long below = lanes_below();
     unsigned int cand = 0;
-    unsigned long long rbase = 0;  // wave-uniform survivor reservation
-    unsigned int rleft = 0, rsize = 32;
+    SurvivorSlots slots;
     for (unsigned int it = blockIdx.x * WAVES_PER_BLOCK + wave; it < n_items; it += total_waves) {
         const PairItem item = items[it];
+        int dx, dy;
+        range_offset(item.kind, dx, dy);
+        const float ox = static_cast<float>(dx) * cellf, oy = static_cast<float>(dy) * cellf;
+        const int q1 = item.part_begin + item.part_count;
+        const bool partners_on_lanes =
+            item.own_count * ((item.part_count + WAVE - 1) / WAVE) < item.part_count;
+        if (partners_on_lanes) {
+            for (int q0 = item.part_begin; q0 < q1; q0 += WAVE) {
+                const int q = q0 + lane;
+                const bool has = q < q1;
+                float4 lb = make_float4(0.f, 0.f, 0.f, 0.f), vb = lb;
+                if (has) {
+                    lb = lpos[q];
+                    vb = lvel[q];
+                }
+                for (int j0 = 0; j0 < item.own_count; j0 += PARTNER_BATCH) {
+                    float4 ta[PARTNER_BATCH], tv[PARTNER_BATCH];
+#pragma unroll
+                    for (int j = 0; j < PARTNER_BATCH; ++j) {
+                        const int pj = item.own_begin + (j0 + j < item.own_count ? j0 + j : item.own_count - 1);
+                        ta[j] = lpos[pj];
+                        tv[j] = lvel[pj];
+                    }
+#pragma unroll
+                    for (int j = 0; j < PARTNER_BATCH; ++j) {
+                        const float4 la = ta[j], va = tv[j];
+                        const int p = item.own_begin + j0 + j;
+                        // Range 0 pairs only with later positions.
+                        const bool test = has && j0 + j < item.own_count && (item.kind != 0 || q > p);
+                        bool keep = false;
+                        if (test) {
+                            ++cand;
+                            const float ax = la.x - ox, ay = la.y - oy;
+                            const float oz =
+                                static_cast<float>(__float_as_int(vb.w) - __float_as_int(va.w)) * cellf;
+                            keep = prefilter_keep_fast(lb.x - ax, lb.y - ay, oz + (lb.z - la.z), vb.x - va.x,
+                                                       vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
+                                                       P.vmin2_f);
+                        }
+                        slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
+                    }
+                }
+            }
+            continue;
+        }
         const int p = item.own_begin + lane;
         const bool active = lane < item.own_count;
         float4 la = make_float4(0.f, 0.f, 0.f, 0.f), va = la;
@@ -1365,26 +1527,25 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
             la = lpos[p];
             va = lvel[p];
         }
-        int dx, dy;
-        range_offset(item.kind, dx, dy);
         // Cell-corner offsets folded into the own position: dr = lb - (la - o).
-        const float ax = la.x - static_cast<float>(dx) * cellf;
-        const float ay = la.y - static_cast<float>(dy) * cellf;
+        const float ax = la.x - ox;
+        const float ay = la.y - oy;
         const int za = __float_as_int(va.w);
         // Range 0 pairs only with later positions.
         const int first = item.kind == 0 ? p + 1 : -1;
-        const int q1 = item.part_begin + item.part_count;
-        for (int q0 = item.part_begin; q0 < q1; q0 += WAVE) {
-            const int nb = min(WAVE, q1 - q0);
-            float4 tp = make_float4(0.f, 0.f, 0.f, 0.f), tv = tp;
-            if (lane < nb) {
-                tp = lpos[q0 + lane];
-                tv = lvel[q0 + lane];
+        for (int q0 = item.part_begin; q0 < q1; q0 += PARTNER_BATCH) {
+            float4 tb[PARTNER_BATCH], tw[PARTNER_BATCH];
+#pragma unroll
+            for (int j = 0; j < PARTNER_BATCH; ++j) {
+                const int q = q0 + j < q1 ? q0 + j : q1 - 1;
+                tb[j] = lpos[q];
+                tw[j] = lvel[q];
             }
-            for (int j = 0; j < nb; ++j) {
-                const float4 lb = lane_broadcast(tp, j);
-                const float4 vb = lane_broadcast(tv, j);
-                const bool test = active && q0 + j >= first;
+#pragma unroll
+            for (int j = 0; j < PARTNER_BATCH; ++j) {
+                const float4 lb = tb[j];
+                const float4 vb = tw[j];
+                const bool test = active && q0 + j < q1 && q0 + j >= first;
                 bool keep = false;
                 if (test) {
                     ++cand;
@@ -1395,32 +1556,11 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                                                vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
                                                P.vmin2_f);
                 }
-                const unsigned long long m = __ballot(keep);
-                if (m == 0ULL) continue;
-                const unsigned int cnt = static_cast<unsigned int>(__popcll(m));
-                if (rleft < cnt) {
-                    if (static_cast<unsigned int>(lane) < rleft && rbase + lane < P.surv_cap)
-                        P.surv[rbase + lane] = make_uint2(SURV_INVALID, SURV_INVALID);
-                    rsize = rsize < 4096u ? 2u * rsize : 4096u;
-                    const unsigned int R = rsize > cnt ? rsize : cnt;
-                    unsigned long long b = 0;
-                    if (lane == 0) b = atomicAdd(&gs->n_surv, static_cast<unsigned long long>(R));
-                    rbase = __shfl(b, 0, WAVE);
-                    rleft = R;
-                }
-                if (keep) {
-                    const unsigned long long slot = rbase + __popcll(m & below);
-                    if (slot < P.surv_cap)
-                        P.surv[slot] = make_uint2(static_cast<unsigned int>(__float_as_int(la.w)),
-                                                  static_cast<unsigned int>(__float_as_int(lb.w)));
-                }
-                rbase += cnt;
-                rleft -= cnt;
+                slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
             }
         }
     }
-    for (unsigned int k = static_cast<unsigned int>(lane); k < rleft; k += WAVE)
-        if (rbase + k < P.surv_cap) P.surv[rbase + k] = make_uint2(SURV_INVALID, SURV_INVALID);
+    slots.close(P, lane);
     store_counters(parts, PairCounters{cand, 0, 0}, s_cnt);
 }
 
@@ -1836,16 +1976,15 @@ __global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, 
 }
 
 __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
-                                                   const PairPartial* parts, int nparts,
+                                                   const PairPartial* parts, int nparts, int npair,
                                                    const PairPartial* dparts, int ndparts,
                                                    const PairPartial* eparts, int neparts) {
     __shared__ unsigned long long s_red[WAVES_PER_BLOCK][5];
     unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0;
     if (screened) {
-        for (int k = threadIdx.x; k < nparts; k += BLOCK) {
-            a += parts[k].cand;
-            m = parts[k].max_cell > m ? parts[k].max_cell : m;
-        }
+        // max_cell from the k_neighbours blocks, candidates from the k_pairs blocks.
+        for (int k = threadIdx.x; k < nparts; k += BLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
+        for (int k = threadIdx.x; k < npair; k += BLOCK) a += parts[k].cand;
         for (int k = threadIdx.x; k < ndparts; k += BLOCK) dn += dparts[k].cand;
         for (int k = threadIdx.x; k < neparts; k += BLOCK) {
             b += eparts[k].refined;
@@ -1967,8 +2106,7 @@ struct khip_ctx {
     int* scen = nullptr;     // scenario id per object (concatenated scenarios)
     int sbits = 0;
     std::vector<int64_t> scen_off;
-    float4* rpos = nullptr;  // the same records in object order (written by k_keys)
-    float4* rvel = nullptr;
+    float4* rec = nullptr;   // the same records in object order, 2 float4 each (written by k_keys)
     GridState* gs = nullptr;
     StepPartial* step_parts = nullptr;
     PairPartial* pair_parts = nullptr;
@@ -2099,7 +2237,7 @@ void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rpos, ctx->rvel, ctx->coltab, ctx->scen, ctx->steptab, ctx->mass, ctx->snap_mass,
+                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rec, ctx->coltab, ctx->scen, ctx->steptab, ctx->mass, ctx->snap_mass,
                     ctx->bstate, ctx->snap_bstate, ctx->jobs, ctx->samples, ctx->blogs,
                     ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
@@ -2143,7 +2281,7 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
            dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
-           dev_alloc(ctx, &ctx->rpos, cap) && dev_alloc(ctx, &ctx->rvel, cap) &&
+           dev_alloc(ctx, &ctx->rec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
            dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
@@ -2239,7 +2377,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint,
                                      ctx->sbits, tc);
     k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
-                                   ctx->idx[0], ctx->hist, tiles, ctx->rpos, ctx->rvel);
+                                   ctx->idx[0], ctx->hist, tiles, ctx->rec);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         if (pass > 0)
@@ -2252,7 +2390,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_cell_count<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
     k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
     k_cell_write<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
-                                         ctx->gs, ctx->bcount, ctx->rpos, ctx->rvel, ctx->cell_key,
+                                         ctx->gs, ctx->bcount, ctx->rec, ctx->cell_key,
                                          ctx->cell_begin, ctx->cell_of, ctx->lpos, ctx->lvel);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_columns<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->coltab);
@@ -2289,14 +2427,15 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.vmin2_f = static_cast<float>(P.vmin2);
     P.gdt3_f = static_cast<float>(GRAVITY_GRADIENT) * P.dt_f * P.dt_f * P.dt_f;
     P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
-    k_pairs<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    const int pair_blocks = std::max(1, (n + PAIR_GROUP - 1) / PAIR_GROUP);
+    k_pairs<<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     mark(ctx, 3);
     k_pairs_dense<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
                                                  ctx->dense_parts);
     mark(ctx, 4);
     k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
     mark(ctx, 5);
-    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n),
+    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n), pair_blocks,
                                  ctx->dense_parts, tc.dense_blocks, ctx->exact_parts, tc.exact_blocks);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
@@ -2565,7 +2704,7 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                                   ctx->moon + 2 * k, t, ctx->dt, bc);
                     }
                 } else {
-                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0,
+                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0,
                                                  ctx->dense_parts, 0, ctx->exact_parts, 0);
                     mark(ctx, 5);
                 }
@@ -2840,12 +2979,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             g_slot = static_cast<int>(k * sub + q);
                             mark(ctx, 1);
                             const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
-                            if (q < sub - 1)
-                                k_dense_output<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, n, ctx->dt,
-                                                                            static_cast<double>(q + 1) / sub,
-                                                                            dense_planes(ctx, q));
                             const Planes pa = q == sub - 1 ? pn : dense_planes(ctx, q);
-                            k_reduce_mid<<<grid_of(n), BLOCK, 0, s>>>(pb, pa, ctx->alive, n, ctx->step_parts);
+                            k_substep<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, sub, pa,
+                                                                   ctx->step_parts);
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q), h, thr, 1, info, q,
                                                 sub))
                                 return false;
@@ -2856,7 +2992,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                                   ctx->rstats);
                         mark(ctx, 0);
                     } else {
-                        k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, ctx->dense_parts,
+                        k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,
                                                      0, ctx->exact_parts, 0);
                     }
                     cur = nxt;
@@ -3633,7 +3769,7 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
                                                   ctx->step_parts);
             ok = enqueue_screen(ctx, ctx->planes(0), ctx->planes(1), t0, 0, 0, dt, ctx->cfg.threshold_km, 0);
         } else {
-            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, ctx->dense_parts, 0,
+            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, 0, ctx->dense_parts, 0,
                                          ctx->exact_parts, 0);
         }
         if (!ok) break;

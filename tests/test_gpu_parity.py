@@ -1,7 +1,7 @@
 """GPU tier: the HIP engine, called through the C ABI, against the oracle and
 the compiled reference's golden fixtures.
 
-Parity bar (north star): the same event set (pair, step), TCA within 1 ms,
+Parity bar: the same event set (pair, step), TCA within 1 ms,
 miss within 1 m, final states within 1e-9 relative. The engine's FP64 path
 keeps the reference's operation order without contraction, so every check
 here asks for more: bit-identical states, events and per-step statistics.

@@ -2,7 +2,7 @@
 term, FP32 J2 and Moon) against the FP64 path, which is bit-identical to the
 oracle (tests/test_gpu_parity.py).
 
-Bars (north star, configs[2]): 24 h position error <= 10 m; screening at
+Bars (1M objects, LEO shells densified, 24 h): 24 h position error <= 10 m; screening at
 2 km: recall of the FP64 event set, TCA within 1 ms and miss within 1 m on
 the matched events. The oracle anchors the FP64 side here too: a sample of
 objects propagated 24 h by the CPU oracle equals the engine's FP64 states.
@@ -13,7 +13,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 EPOCH = 2461253.9166666665
-MIX_DENSE = (0.7, 0.1, 0.1, 0.1, 1.5)  # LEO shells densified as in configs[2]
+MIX_DENSE = (0.7, 0.1, 0.1, 0.1, 1.5)  # LEO shells densified
 
 
 def _run(eng, pop, steps, precision, screen, thr=2.0):
