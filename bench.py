@@ -229,10 +229,13 @@ def recall_against(e_ref, e_new):
     return out
 
 
-# HBM bytes per k_pairs launch of the learned workload, measured by two
-# rocprofv3 PMC passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from copy
-# kernels of known size in the same runs).
-K_PAIRS_TRAFFIC_B = 136.5e6
+# HBM bytes per launch of the pair kernels in the learned workload (20 s
+# screening windows), measured by two rocprofv3 PMC passes (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, factors from copy kernels of known size in the same runs).
+PAIR_TRAFFIC_B = {"k_pairs": 148.6e6, "k_pairs_dense": 73.0e6}
+# Screening windows per 60 s learned step (20 s each; propagation and the
+# plain re-verification keep 10 s sub-steps).
+WINDOWS = 3
 
 # BF16 matrix-core flops of one corrector evaluation (10-128-128-128-6;
 # the input and output layers count once although they run hi/lo splits).
@@ -261,7 +264,7 @@ def learned_workload(kamd, args):
     def run(precision, dt, nsteps, nwarm):
         eng.set_precision(precision)
         if precision == kamd.PRECISION_LEARNED:
-            eng.set_learned()
+            eng.set_learned(windows=WINDOWS)
         for w in (nwarm, 0):
             eng.set_objects(pop["state"], alive0, pop["size_m"])
             eng.set_config(thr, 1.0, 0.01)
@@ -270,15 +273,28 @@ def learned_workload(kamd, args):
                 eng.run(w, screen=True)
                 eng.take_events()
                 eng.take_stats()
+        # Timed region: the whole run, no per-kernel event marks.
+        t1 = time.perf_counter()
+        eng.run(nsteps, screen=True)
+        wall = time.perf_counter() - t1
+        st = eng.take_stats()
+        out = dict(wall=wall, events=eng.take_events(), stats=st, states=eng.get_objects(),
+                   report=eng.learned_report() if precision == kamd.PRECISION_LEARNED else None)
+        # Per-kernel device times: the same run again from the same state with
+        # HIP event marks on the engine's stream (the marks cost ~5% of wall).
+        eng.set_objects(pop["state"], alive0, pop["size_m"])
+        eng.set_config(thr, 1.0, 0.01)
+        eng.set_time(EPOCH, 0.0, dt)
         eng.set_timing(True)
         eng.reset_timing()
         t1 = time.perf_counter()
         eng.run(nsteps, screen=True)
-        wall = time.perf_counter() - t1
+        out["wall_marked"] = time.perf_counter() - t1
         eng.set_timing(False)
-        st = eng.take_stats()
-        return dict(wall=wall, events=eng.take_events(), stats=st, timing=eng.timing(), states=eng.get_objects(),
-                    report=eng.learned_report() if precision == kamd.PRECISION_LEARNED else None)
+        out["timing"] = eng.timing()
+        eng.take_events()
+        eng.take_stats()
+        return out
 
     plain = run(kamd.PRECISION_FP64, DT, 6 * steps, 6 * warm)
     lrn = run(kamd.PRECISION_LEARNED, 60.0, steps, warm)
@@ -291,7 +307,7 @@ def learned_workload(kamd, args):
     v_l, v_p = obj_l / lrn["wall"], obj_p / plain["wall"]
     rec = recall_against(plain["events"], lrn["events"])
     tm = lrn["timing"]
-    launches_sub = max(1, int(tm["steps"]) * 6)
+    launches_sub = max(1, int(tm["steps"]) * WINDOWS)
     kernels = {"k_learned (propagation)": float(tm["propagate_ms"]), "grid (11-13 kernels)": float(tm["grid_ms"]),
                "k_pairs": float(tm["pairs_sparse_ms"]), "k_pairs_dense": float(tm["pairs_dense_ms"]),
                "k_exact": float(tm["exact_ms"]), "k_reverify": float(tm["reverify_ms"])}
@@ -305,10 +321,10 @@ def learned_workload(kamd, args):
         ach = FLOP_PER_CANDIDATE * units / (ms * 1e-3) / 1e12
         roof = {"kernel": dominant, "bound": "valu-fp32", "achieved": ach, "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
-                "traffic": K_PAIRS_TRAFFIC_B if dominant == "k_pairs" else None,
+                "traffic": PAIR_TRAFFIC_B.get(dominant),
                 "avg_launch_us": 1e3 * ms / launches_sub, "units_per_launch": units / launches_sub,
                 "unit_of_work": "candidate pair (60 FP32 flops)"}
-        if dominant == "k_pairs":
+        if dominant in PAIR_TRAFFIC_B:
             roof["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, "
                                       "calibrated on copy kernels: profiles/learned1M_pmc_traffic.md")
     else:
@@ -329,6 +345,7 @@ def learned_workload(kamd, args):
         "plain_fp64_value": v_p, "learned_over_plain_fp64": v_l / v_p,
         "object_seconds_per_s": v_l * DT,
         "ms_per_step_plain_fp64_10s": 1e3 * plain["wall"] / (6 * steps),
+        "ms_per_step_with_kernel_marks": 1e3 * lrn["wall_marked"] / steps,
         "position_error_m": {"max": float(dr.max()), "p99": float(np.percentile(dr, 99)),
                              "median": float(np.median(dr))},
         "alive_flags_equal": bool(np.array_equal(a64, al)),
@@ -336,7 +353,7 @@ def learned_workload(kamd, args):
         "learned_events_below_threshold": int((lrn["events"]["miss_km"] < thr).sum()),
         "methods": rep, "training": train_rep,
         "corrector_mfma_tflops_in_propagation": mf,
-        "candidate_pairs_per_substep": cand / launches_sub,
+        "candidate_pairs_per_window": cand / launches_sub,
         "candidate_pairs_per_step_plain_fp64": float(plain["stats"]["candidate_pairs"].mean()),
         "roofline": roof, "device": eng.device_name,
     }

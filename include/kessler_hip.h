@@ -80,6 +80,8 @@ typedef struct {
     int32_t cell_doublings; /* > 0 only where the grid extent needed a coarser cell (see khip_run) */
     int64_t propagated;     /* objects alive at the start of the step (0 for khip_screen) */
     int64_t dense_candidates; /* part of candidate_pairs screened by dense-cell work items */
+    int64_t survivors;      /* candidate pairs the FP32 pre-filter kept for the FP64 filter
+                               (no reference counterpart: the reference filters in FP64 only) */
 } khip_stats;
 
 /* Device time per kernel family, summed over the steps since the last
@@ -183,12 +185,15 @@ int khip_reset_timing(khip_ctx* ctx);
  * steps, and the first step after khip_set_objects / khip_set_precision /
  * khip_set_learned) runs the plain method for every object and keeps the
  * cheaper method for an object only if it stayed within tol_km of the plain
- * result in position and within tol_a_km in semi-major axis. Screening covers each sub-step of tau / substeps on the cubic
+ * result in position and within tol_a_km in semi-major axis. Screening covers each window of tau / windows on the cubic
  * Hermite dense output of the step, with a grid on midpoints and the
  * threshold raised by margin_km; every event found is then solved again on
  * plain FP64 trajectories from the step's start (filter_pair, Hermite +
- * golden section, radius test) and kept with the plain values. Event step
- * indices count sub-steps, as a plain run at dt = tau / substeps does. */
+ * golden section, radius test, in the sub-step of tau / substeps holding
+ * the TCA) and kept with the plain values. Event step indices count
+ * sub-steps, as a plain run at dt = tau / substeps does; statistics have one
+ * record per sub-step, a window's counts in the record of its first
+ * sub-step and zero counts in the others. */
 #define KHIP_PRECISION_LEARNED 2
 int khip_set_precision(khip_ctx* ctx, int mode);
 
@@ -201,6 +206,9 @@ typedef struct {
     double a_max_km;     /* corrector domain: semi-major axis below (default 12000) */
     double tol_a_km;     /* semi-major-axis deviation tolerated at a check (default 5e-8: a
                             systematic 5e-8 km per 60 s step drifts ~5 m along track in 24 h) */
+    int32_t windows;     /* screening windows per learned step (divides substeps; 0: substeps / 2
+                            when even, else substeps). Fewer, longer windows mean fewer grid
+                            builds and more candidate pairs per build. */
 } khip_learned_config;
 
 typedef struct {

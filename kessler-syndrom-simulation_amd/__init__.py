@@ -25,7 +25,8 @@ EVENT_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("step", "<i8"), ("t", "<f8"
 STATS_DTYPE = np.dtype([("candidate_pairs", "<i8"), ("refined_pairs", "<i8"),
                         ("co_orbiting", "<i8"), ("cell_km", "<f8"),
                         ("max_cell_objects", "<i4"), ("cell_doublings", "<i4"),
-                        ("propagated", "<i8"), ("dense_candidates", "<i8")])
+                        ("propagated", "<i8"), ("dense_candidates", "<i8"),
+                        ("survivors", "<i8")])
 OBJECT_DTYPE = np.dtype([("id", "<i4"), ("event", "<i4"), ("alive", "<i4"), ("type", "<i4"),
                          ("state", "<f8", (6,)), ("size_m", "<f8"), ("mass_kg", "<f8"),
                          ("area_to_mass", "<f8")])
@@ -83,7 +84,7 @@ class TrainConfig(C.Structure):
 
 class LearnedConfig(C.Structure):
     _fields_ = [("substeps", C.c_int32), ("check_every", C.c_int32), ("tol_km", _D), ("margin_km", _D),
-                ("e_max", _D), ("a_max_km", _D), ("tol_a_km", _D)]
+                ("e_max", _D), ("a_max_km", _D), ("tol_a_km", _D), ("windows", C.c_int32)]
 
 
 LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", "<i8"), ("rejected", "<i8"),
@@ -361,8 +362,8 @@ class Engine:
         return grad, loss.value
 
     def set_learned(self, substeps=6, check_every=10, tol_km=1e-5, margin_km=0.025, e_max=0.1,
-                    a_max_km=12000.0, tol_a_km=5e-8):
-        cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km, tol_a_km)
+                    a_max_km=12000.0, tol_a_km=5e-8, windows=0):
+        cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km, tol_a_km, windows)
         self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")
 
     def set_scenarios(self, offsets):

@@ -156,21 +156,27 @@ __device__ __forceinline__ double golden_min(const Hermite& h) {
     double fc = p.x * p.x + p.y * p.y + p.z * p.z;
     p = h.pos(d);
     double fd = p.x * p.x + p.y * p.y + p.z * p.z;
+    // Branch-free: the bracket and the new point are selected, then |p|^2 is
+    // evaluated once (lanes of a wave that keep different sides do not run
+    // both evaluations); the same operations on the same values as the
+    // two-branch form, so the result is bit for bit the same.
     for (int it = 0; it < 60; ++it) {
-        if (fc < fd) {
-            b = d;
+        const bool left = fc < fd;
+        if (left) b = d;
+        else a = c;
+        const double x = left ? b - g * (b - a) : a + g * (b - a);
+        p = h.pos(x);
+        const double fx = p.x * p.x + p.y * p.y + p.z * p.z;
+        if (left) {
             d = c;
             fd = fc;
-            c = b - g * (b - a);
-            p = h.pos(c);
-            fc = p.x * p.x + p.y * p.y + p.z * p.z;
+            c = x;
+            fc = fx;
         } else {
-            a = c;
             c = d;
             fc = fd;
-            d = a + g * (b - a);
-            p = h.pos(d);
-            fd = p.x * p.x + p.y * p.y + p.z * p.z;
+            d = x;
+            fd = fx;
         }
     }
     return 0.5 * (a + b);

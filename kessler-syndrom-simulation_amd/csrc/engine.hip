@@ -58,8 +58,14 @@ using namespace kessler::dev;
 
 namespace {
 
-constexpr int ITEMS = 8;               // elements per thread per tile
+constexpr int ITEMS = 8;               // elements per thread per tile (BLOCK threads)
 constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
+// The tile kernels of the grid build (keys, radix passes, cell table) take a
+// tile with TBLOCK threads, TITEMS elements each: 16 waves per tile keep
+// enough loads in flight with a few hundred tiles on 256 CUs.
+constexpr int TBLOCK = 1024;
+constexpr int TITEMS = TILE / TBLOCK;
+constexpr int TWAVES = TBLOCK / 64;
 constexpr int RADIX = 256;
 constexpr int KEY_BITS_MAX = 63;       // packed key width; the dead sentinel is all ones
 constexpr uint64_t DEAD_KEY = ~0ULL;
@@ -138,6 +144,7 @@ struct DevStats {
     int max_cell, doublings;
     long long propagated;
     long long dense_cand;
+    long long survivors;
 };
 static_assert(sizeof(DevStats) == sizeof(khip_stats), "stats layout");
 
@@ -208,13 +215,14 @@ __device__ __forceinline__ StepPartial wave_reduce(StepPartial p) {
 }
 
 // Block-wide combine; thread 0 returns the block's total.
+template <int NW = WAVES_PER_BLOCK>
 __device__ __forceinline__ StepPartial block_reduce(StepPartial p, StepPartial* s_rows) {
     p = wave_reduce(p);
     const int wave = threadIdx.x / WAVE;
     if (__lane_id() == 0) s_rows[wave] = p;
     __syncthreads();
     StepPartial t = s_rows[0];
-    for (int w = 1; w < WAVES_PER_BLOCK; ++w) combine(t, s_rows[w]);
+    for (int w = 1; w < NW; ++w) combine(t, s_rows[w]);
     return t;
 }
 
@@ -596,14 +604,14 @@ __device__ __forceinline__ int bits_for(long long range) {
 // then has midpoints closer than vmax dt + thr + G vmax dt^3 (each object is
 // within vmax dt / 2 of its midpoint over the step), so cells of that size
 // and the 13 forward neighbours enumerate every such pair.
-__global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
+__global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
                                                        TileCfg tc) {
-    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
+    __shared__ StepPartial s_rows[TWAVES];
     StepPartial acc = empty_step_partial();
-    for (int b = threadIdx.x; b < nparts; b += BLOCK) combine(acc, partials[b]);
-    const StepPartial tot = block_reduce(acc, s_rows);
+    for (int b = threadIdx.x; b < nparts; b += TBLOCK) combine(acc, partials[b]);
+    const StepPartial tot = block_reduce<TWAVES>(acc, s_rows);
     if (threadIdx.x != 0) return;
     gs->vmax_bits = tot.vmax_bits;
     for (int a = 0; a < 3; ++a) {
@@ -667,17 +675,17 @@ __global__ void __launch_bounds__(BLOCK) k_grid_setup(GridState* gs, const StepP
 // with the object index in w, then velocity with the cell's packed z
 // coordinate in w. k_cell_write gathers one record (one cache line) per
 // sorted object.
-__global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
+__global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  const int* __restrict__ scen, int n, const GridState* gs,
                                                  uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec) {
     __shared__ int s_hist[RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
-    s_hist[threadIdx.x] = 0;
+    if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * TILE;
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + threadIdx.x;
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + threadIdx.x;
         if (e < n) {
             uint64_t k = DEAD_KEY;
             if (alive[e]) {
@@ -706,26 +714,27 @@ __global__ void __launch_bounds__(BLOCK) k_keys(Planes cur, Planes end, int midp
         }
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+    if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_hist(const uint64_t* keys, int n, int pass,
+__global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int n, int pass,
                                                  const GridState* gs, int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
     if (pass >= gs->npasses) return;
-    s_hist[threadIdx.x] = 0;
+    if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
     const int shift = 8 * pass;
     const int base = blockIdx.x * TILE;
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + threadIdx.x;
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + threadIdx.x;
         if (e < n) atomicAdd(&s_hist[static_cast<int>((keys[e] >> shift) & 255u)], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+    if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-// Block-wide exclusive scan of one int per thread; returns the total.
+// Block-wide exclusive scan of one int per thread (NW waves); returns the total.
+template <int NW = WAVES_PER_BLOCK>
 __device__ __forceinline__ int block_exclusive_scan(int v, int& total, int* s_wave) {
     const int lane = __lane_id();
     const int wave = threadIdx.x / WAVE;
@@ -737,7 +746,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int& total, int* s_wa
     if (lane == WAVE - 1) s_wave[wave] = incl;
     __syncthreads();
     int before = 0, sum = 0;
-    for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+    for (int w = 0; w < NW; ++w) {
         if (w < wave) before += s_wave[w];
         sum += s_wave[w];
     }
@@ -765,35 +774,46 @@ __global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int
 }
 
 // Stable scatter of one radix pass. In-tile ranks: elements are taken in
-// index order, 256 per round; a wave finds the lanes holding the same digit
-// with 8 ballots, and per-wave digit counts in LDS give the cross-wave
-// offsets.
-__global__ void __launch_bounds__(BLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
-                                                    uint64_t* keys_out, int* idx_out, int n,
-                                                    int pass, const GridState* gs,
-                                                    const int* hist, const int* totals,
-                                                    int ntiles) {
+// index order, TBLOCK per round; a wave finds the lanes holding the same
+// digit with 8 ballots, and per-wave digit counts in LDS give the
+// cross-wave offsets.
+__global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
+                                                     uint64_t* keys_out, int* idx_out, int n,
+                                                     int pass, const GridState* gs,
+                                                     const int* hist, const int* totals,
+                                                     int ntiles) {
     __shared__ int s_off[RADIX];
-    __shared__ int s_wcnt[WAVES_PER_BLOCK][RADIX];
-    __shared__ int s_wave[WAVES_PER_BLOCK];
+    __shared__ int s_wcnt[TWAVES][RADIX];
+    __shared__ int s_wave[TWAVES];
     if (pass >= gs->npasses) return;
     const int t = threadIdx.x;
     {
         int tot;
-        const int digit_base = block_exclusive_scan(totals[t], tot, s_wave);
-        s_off[t] = digit_base + hist[t * ntiles + blockIdx.x];
-        for (int w = 0; w < WAVES_PER_BLOCK; ++w) s_wcnt[w][t] = 0;
+        const int digit_base = block_exclusive_scan<TWAVES>(t < RADIX ? totals[t] : 0, tot, s_wave);
+        if (t < RADIX) {
+            s_off[t] = digit_base + hist[t * ntiles + blockIdx.x];
+            for (int w = 0; w < TWAVES; ++w) s_wcnt[w][t] = 0;
+        }
     }
     __syncthreads();
     const int shift = 8 * pass;
     const int wave = t / WAVE;
     const unsigned long long below = lanes_below();
     const int base = blockIdx.x * TILE;
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + t;
+    uint64_t keys[TITEMS];
+    int ids[TITEMS];
+#pragma unroll
+    for (int r = 0; r < TITEMS; ++r) {  // all of the tile's loads in flight at once
+        const int e = base + r * TBLOCK + t;
+        keys[r] = e < n ? keys_in[e] : 0ULL;
+        ids[r] = e < n ? idx_in[e] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + t;
         const bool valid = e < n;
-        const uint64_t key = valid ? keys_in[e] : 0ULL;
-        const int id = valid ? idx_in[e] : 0;
+        const uint64_t key = keys[r];
+        const int id = ids[r];
         const int d = static_cast<int>((key >> shift) & 255u);
         unsigned long long peers = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
@@ -812,12 +832,14 @@ __global__ void __launch_bounds__(BLOCK) k_scatter(const uint64_t* keys_in, cons
             idx_out[pos] = id;
         }
         __syncthreads();
-        int add = 0;
-        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
-            add += s_wcnt[w][t];
-            s_wcnt[w][t] = 0;
+        if (t < RADIX) {
+            int add = 0;
+            for (int w = 0; w < TWAVES; ++w) {
+                add += s_wcnt[w][t];
+                s_wcnt[w][t] = 0;
+            }
+            s_off[t] += add;
         }
-        s_off[t] += add;
         __syncthreads();
     }
 }
@@ -831,22 +853,22 @@ __device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* 
 }
 
 // Cell heads per tile over the first `alive` sorted elements.
-__global__ void __launch_bounds__(BLOCK) k_cell_count(const uint64_t* k0, const uint64_t* k1,
-                                                       const GridState* gs, int* bcount) {
-    __shared__ int s_wave[WAVES_PER_BLOCK];
+__global__ void __launch_bounds__(TBLOCK) k_cell_count(const uint64_t* k0, const uint64_t* k1,
+                                                        const GridState* gs, int* bcount) {
+    __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
     const uint64_t* keys = sorted_keys(g, k0, k1);
     const int base = blockIdx.x * TILE;
     int cnt = 0;
     if (base < A && g.npasses > 0) {
-        for (int r = 0; r < ITEMS; ++r) {
-            const int e = base + r * BLOCK + threadIdx.x;
+        for (int r = 0; r < TITEMS; ++r) {
+            const int e = base + r * TBLOCK + threadIdx.x;
             if (e < A && (e == 0 || keys[e] != keys[e - 1])) ++cnt;
         }
     }
     int tot;
-    block_exclusive_scan(cnt, tot, s_wave);
+    block_exclusive_scan<TWAVES>(cnt, tot, s_wave);
     if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
 }
 
@@ -865,19 +887,19 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
 }
 
 // Cell table and compact records in sorted order. Element e = tile +
-// r * BLOCK + thread (r < ITEMS), so every load and store is coalesced; cells
-// are numbered from wave ballots of the head flags and one block-wide
-// prefix over the tile's ITEMS x 4 wave segments (in element order).
+// r * TBLOCK + thread (r < TITEMS), so every load and store is coalesced;
+// cells are numbered from wave ballots of the head flags and one block-wide
+// prefix over the tile's TITEMS x TWAVES wave segments (in element order).
 //   lpos: position relative to its cell's low corner, w = object index bits;
 //   lvel: velocity, w = the cell's packed z coordinate (int bits), from
 //         which pairs get their z cell offset.
-__global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
+__global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
                                                        const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lpos, float4* lvel) {
-    __shared__ int s_seg[ITEMS * WAVES_PER_BLOCK];
+    __shared__ int s_seg[TITEMS * TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
     const int base = blockIdx.x * TILE;
@@ -887,20 +909,20 @@ __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const 
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
     const unsigned long long below = lanes_below();
-    uint64_t key[ITEMS];
-    unsigned long long bal[ITEMS];
-    bool head[ITEMS];
-    float4 p[ITEMS], v[ITEMS];
+    uint64_t key[TITEMS];
+    unsigned long long bal[TITEMS];
+    bool head[TITEMS];
+    float4 p[TITEMS], v[TITEMS];
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + threadIdx.x;
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + threadIdx.x;
         const bool valid = e < A;
         key[r] = valid ? keys[e] : 0ULL;
         uint64_t prev = __shfl_up(key[r], 1, WAVE);
         if (lane == 0) prev = valid && e > 0 ? keys[e - 1] : ~0ULL;
         head[r] = valid && (e == 0 || key[r] != prev);
         bal[r] = __ballot(head[r]);
-        if (lane == 0) s_seg[r * WAVES_PER_BLOCK + wave] = __popcll(bal[r]);
+        if (lane == 0) s_seg[r * TWAVES + wave] = __popcll(bal[r]);
         if (valid) {
             const int id = sidx[e];
             p[r] = rec[2 * id];
@@ -909,17 +931,17 @@ __global__ void __launch_bounds__(BLOCK) k_cell_write(const uint64_t* k0, const 
     }
     __syncthreads();
     int run = boff[blockIdx.x];
-    int pre[ITEMS];
+    int pre[TITEMS];
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        for (int w = 0; w < WAVES_PER_BLOCK; ++w) {
+    for (int r = 0; r < TITEMS; ++r) {
+        for (int w = 0; w < TWAVES; ++w) {
             if (w == wave) pre[r] = run;
-            run += s_seg[r * WAVES_PER_BLOCK + w];
+            run += s_seg[r * TWAVES + w];
         }
     }
 #pragma unroll
-    for (int r = 0; r < ITEMS; ++r) {
-        const int e = base + r * BLOCK + threadIdx.x;
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + threadIdx.x;
         if (e >= A) break;
         const int c = pre[r] + __popcll(bal[r] & below) + (head[r] ? 1 : 0) - 1;
         if (head[r]) {
@@ -961,16 +983,6 @@ __device__ __forceinline__ bool axis_fits(long long v, int bits) {
     return v >= 0 && (bits >= 62 || v < (1LL << bits));
 }
 
-// First index in [0, n) whose key is >= key.
-__device__ __forceinline__ int lower_bound_u64(const uint64_t* ck, int n, uint64_t key) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ck[mid] < key) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ int segments(int n, int seg) { return (n + seg - 1) / seg; }
 
@@ -1053,32 +1065,70 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         rng[0] = make_int2(b, e0);
         const long long zlo = z > 0 ? z - 1 : 0;
         const long long zhi = axis_fits(z + 1, bz) ? z + 1 : z;
+        // Ranges 1-4: the cells (x+dx, y+dy, zlo..zhi), a run of at most three
+        // consecutive cells of that column. The four lower-bound searches run
+        // in lockstep (their loads are issued together); the run's end is
+        // among the next three cells.
+        uint64_t lo_key[NRANGES], hi_key[NRANGES];
+        int base[NRANGES], len[NRANGES];
+        bool valid[NRANGES];
+#pragma unroll
         for (int r = 1; r < NRANGES; ++r) {
             int dx, dy;
             range_offset(r, dx, dy);
             const long long nx = x + dx, ny = y + dy;
-            int2 run = make_int2(0, 0);
-            if (axis_fits(nx, bx) && axis_fits(ny, by)) {
+            base[r] = 0;
+            len[r] = 0;
+            lo_key[r] = hi_key[r] = 0ULL;
+            valid[r] = axis_fits(nx, bx) && axis_fits(ny, by);
+            if (valid[r]) {
                 const uint64_t row = scen_part | (static_cast<uint64_t>(nx) << (by + bz)) |
                                      (static_cast<uint64_t>(ny) << bz);
-                int lo = 0, hi = nc;
+                lo_key[r] = row | static_cast<uint64_t>(zlo);
+                hi_key[r] = row | static_cast<uint64_t>(zhi);
+                len[r] = nc;
                 if (use_tab) {
                     const int4 t = coltab[row >> bz];
-                    if (t.x == stamp) {
-                        lo = t.y;
-                        hi = t.z;
-                    } else {
-                        hi = 0;
-                    }
-                }
-                if (lo < hi) {
-                    const int i0 = lo + lower_bound_u64(cell_key + lo, hi - lo, row | static_cast<uint64_t>(zlo));
-                    const int i1 =
-                        lo + lower_bound_u64(cell_key + lo, hi - lo, (row | static_cast<uint64_t>(zhi)) + 1ULL);
-                    if (i0 < i1) run = make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A);
+                    valid[r] = t.x == stamp;  // the column has cells in this screening
+                    base[r] = valid[r] ? t.y : 0;
+                    len[r] = valid[r] ? t.z - t.y : 0;
                 }
             }
-            rng[r] = run;
+        }
+        for (;;) {
+            bool more = false;
+#pragma unroll
+            for (int r = 1; r < NRANGES; ++r) more |= len[r] > 0;
+            if (!more) break;
+            uint64_t probe[NRANGES];
+#pragma unroll
+            for (int r = 1; r < NRANGES; ++r) probe[r] = len[r] > 0 ? cell_key[base[r] + (len[r] >> 1)] : 0ULL;
+#pragma unroll
+            for (int r = 1; r < NRANGES; ++r) {
+                if (len[r] <= 0) continue;
+                const int half = len[r] >> 1;
+                if (probe[r] < lo_key[r]) {
+                    base[r] += half + 1;
+                    len[r] -= half + 1;
+                } else {
+                    len[r] = half;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 1; r < NRANGES; ++r) {
+            // base[r]: first cell >= (x+dx, y+dy, zlo) (or a cell of another column).
+            const int i0 = base[r];
+            int i1 = i0;
+            if (valid[r]) {
+                uint64_t k3[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) k3[m] = i0 + m < nc ? cell_key[i0 + m] : ~0ULL;
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+                    if (i1 == i0 + m && k3[m] >= lo_key[r] && k3[m] <= hi_key[r]) ++i1;
+            }
+            rng[r] = i0 < i1 ? make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A) : make_int2(0, 0);
         }
         for (int r = 0; r < NRANGES; ++r) ranges[c * NRANGES + r] = rng[r];
         items = static_cast<int>(cnt) >= gs->dense_min ? dense_items(b, static_cast<int>(cnt), rng, gs->seg)
@@ -1185,8 +1235,9 @@ struct PairParams {
     unsigned long long surv_cap;
     long long step;
     double t0, dt, thr, vmin2;
-    const StepInfo* info;   // learned form: step and t0 from here (sub-step q of nsub)
+    const StepInfo* info;   // learned form: step and t0 from here (window sub_q of sub_n)
     int sub_q, sub_n;
+    int sub_per;            // learned form: plain sub-steps per screening window
     float dt_f, thr_f, vmin2_f;
     float gdt3_f;   // GRAVITY_GRADIENT * dt^3
     float base_f;   // threshold + 1e-3 + DR_ERR_KM + DV_ERR_KM_S * dt
@@ -1244,7 +1295,9 @@ __device__ __forceinline__ void exact_pair(GridState* gs, const PairParams& P, i
     long long step = P.step;
     double t0 = P.t0;
     if (P.info) {
-        step = P.info->macro * P.sub_n + P.sub_q;
+        // The event's step is the plain sub-step holding the TCA.
+        const int j = min(P.sub_per - 1, static_cast<int>(s * P.sub_per));
+        step = (P.info->macro * P.sub_n + P.sub_q) * P.sub_per + j;
         t0 = P.info->t + static_cast<double>(P.sub_q) * dt;
     }
     ev.step = step;
@@ -1575,6 +1628,7 @@ __global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, Pa
          k += static_cast<unsigned long long>(gridDim.x) * BLOCK) {
         const uint2 pr = P.surv[k];
         if (pr.x == SURV_INVALID) continue;  // unused reserved slot
+        ++cnt.cand;                          // survivors
         exact_pair(gs, P, static_cast<int>(pr.x), static_cast<int>(pr.y), cnt);
     }
     store_counters(parts, cnt, s_cnt);
@@ -1975,20 +2029,21 @@ __global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, 
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, int screened,
+__global__ void __launch_bounds__(TBLOCK) k_finish(GridState* gs, DevStats* out, int screened,
                                                    const PairPartial* parts, int nparts, int npair,
                                                    const PairPartial* dparts, int ndparts,
-                                                   const PairPartial* eparts, int neparts) {
-    __shared__ unsigned long long s_red[WAVES_PER_BLOCK][5];
-    unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0;
+                                                   const PairPartial* eparts, int neparts, int fill) {
+    __shared__ unsigned long long s_red[TWAVES][6];
+    unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0, sv = 0;
     if (screened) {
         // max_cell from the k_neighbours blocks, candidates from the k_pairs blocks.
-        for (int k = threadIdx.x; k < nparts; k += BLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
-        for (int k = threadIdx.x; k < npair; k += BLOCK) a += parts[k].cand;
-        for (int k = threadIdx.x; k < ndparts; k += BLOCK) dn += dparts[k].cand;
-        for (int k = threadIdx.x; k < neparts; k += BLOCK) {
+        for (int k = threadIdx.x; k < nparts; k += TBLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
+        for (int k = threadIdx.x; k < npair; k += TBLOCK) a += parts[k].cand;
+        for (int k = threadIdx.x; k < ndparts; k += TBLOCK) dn += dparts[k].cand;
+        for (int k = threadIdx.x; k < neparts; k += TBLOCK) {
             b += eparts[k].refined;
             c += eparts[k].co;
+            sv += eparts[k].cand;
         }
     }
     a = wave_sum(a);
@@ -1996,6 +2051,7 @@ __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, 
     c = wave_sum(c);
     m = wave_max_u64(m);
     dn = wave_sum(dn);
+    sv = wave_sum(sv);
     const int wave = threadIdx.x / WAVE;
     if (__lane_id() == 0) {
         s_red[wave][0] = a;
@@ -2003,20 +2059,23 @@ __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, 
         s_red[wave][2] = c;
         s_red[wave][3] = m;
         s_red[wave][4] = dn;
+        s_red[wave][5] = sv;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < WAVES_PER_BLOCK; ++w) {
+    for (int w = 1; w < TWAVES; ++w) {
         s_red[0][0] += s_red[w][0];
         s_red[0][1] += s_red[w][1];
         s_red[0][2] += s_red[w][2];
         s_red[0][3] = s_red[w][3] > s_red[0][3] ? s_red[w][3] : s_red[0][3];
         s_red[0][4] += s_red[w][4];
+        s_red[0][5] += s_red[w][5];
     }
     if (screened) {
         DevStats s;
         s.cand = static_cast<long long>(s_red[0][0] + s_red[0][4]);
         s.dense_cand = static_cast<long long>(s_red[0][4]);
+        s.survivors = static_cast<long long>(s_red[0][5]);
         s.refined = static_cast<long long>(s_red[0][1]);
         s.co = static_cast<long long>(s_red[0][2]);
         gs->max_cell = static_cast<unsigned int>(s_red[0][3]);
@@ -2025,6 +2084,13 @@ __global__ void __launch_bounds__(BLOCK) k_finish(GridState* gs, DevStats* out, 
         s.doublings = gs->doublings;
         s.propagated = static_cast<long long>(gs->propagated);
         *out = s;
+        // Records of the window's later sub-steps: zero counts, objects counted.
+        for (int j = 1; j <= fill; ++j) {
+            DevStats f{};
+            f.cell_km = s.cell_km;
+            f.propagated = s.propagated;
+            out[j] = f;
+        }
     }
     gs->propagated = 0;
     gs->vmax_bits = 0ULL;
@@ -2148,7 +2214,7 @@ struct khip_ctx {
 
     // Learned form: per-object method, per-method lists, dense-output
     // states of the sub-step boundaries, check and re-verification tallies.
-    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8};
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0};
     int64_t lrn_cap = 0;
     uint8_t* mode = nullptr;
     uint8_t* snap_mode = nullptr;
@@ -2369,27 +2435,28 @@ void mark(khip_ctx* ctx, int k) {
 // Enqueues the screening kernels for the step whose start / end states are
 // `before` / `after` (stats record `stat_slot`).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
-                    double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1) {
+                    double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
+                    int sub_per = 1) {
     const int n = static_cast<int>(ctx->n_launch);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
-    k_grid_setup<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint,
+    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint,
                                      ctx->sbits, tc);
-    k_keys<<<tiles, BLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
+    k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
                                    ctx->idx[0], ctx->hist, tiles, ctx->rec);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         if (pass > 0)
-            k_hist<<<tiles, BLOCK, 0, s>>>(ctx->keys[src], n, pass, ctx->gs, ctx->hist, tiles);
+            k_hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], n, pass, ctx->gs, ctx->hist, tiles);
         k_digit_scan<<<RADIX, BLOCK, 0, s>>>(ctx->hist, tiles, pass, ctx->gs, ctx->totals);
-        k_scatter<<<tiles, BLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst],
+        k_scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst],
                                           ctx->idx[dst], n, pass, ctx->gs, ctx->hist, ctx->totals,
                                           tiles);
     }
-    k_cell_count<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
+    k_cell_count<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
     k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
-    k_cell_write<<<tiles, BLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
+    k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
                                          ctx->gs, ctx->bcount, ctx->rec, ctx->cell_key,
                                          ctx->cell_begin, ctx->cell_of, ctx->lpos, ctx->lvel);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
@@ -2419,6 +2486,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.info = info;
     P.sub_q = sub_q;
     P.sub_n = sub_n;
+    P.sub_per = sub_per;
     P.dt = dt;
     P.thr = thr;
     P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
@@ -2435,8 +2503,9 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     mark(ctx, 4);
     k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
     mark(ctx, 5);
-    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n), pair_blocks,
-                                 ctx->dense_parts, tc.dense_blocks, ctx->exact_parts, tc.exact_blocks);
+    k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n), pair_blocks,
+                                 ctx->dense_parts, tc.dense_blocks, ctx->exact_parts, tc.exact_blocks,
+                                 info ? sub_per - 1 : 0);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
 }
@@ -2468,18 +2537,19 @@ void accumulate_timing(khip_ctx* ctx, int64_t steps, bool screened) {
     }
 }
 
-void accumulate_timing_learned(khip_ctx* ctx, int64_t steps, int sub, bool screened) {
+void accumulate_timing_learned(khip_ctx* ctx, int64_t steps, int sub, int nw, bool screened) {
+    const int spw = sub / nw;
     for (int64_t k = 0; k < steps; ++k) {
         hipEvent_t* e0 = ctx->ev_pool.data() + MARKS * (k * sub);
-        hipEvent_t* el = ctx->ev_pool.data() + MARKS * (k * sub + sub - 1);
+        hipEvent_t* el = ctx->ev_pool.data() + MARKS * (k * sub + (nw - 1) * spw);
         float a = 0, d = 0;
         (void)hipEventElapsedTime(&a, e0[6], e0[7]);
         ctx->tm.propagate_ms += a;
         (void)hipEventElapsedTime(&d, e0[6], screened ? el[0] : e0[7]);
         ctx->tm.total_ms += d;
         if (screened) {
-            for (int q = 0; q < sub; ++q) {
-                hipEvent_t* e = ctx->ev_pool.data() + MARKS * (k * sub + q);
+            for (int q = 0; q < nw; ++q) {
+                hipEvent_t* e = ctx->ev_pool.data() + MARKS * (k * sub + q * spw);
                 float b = 0, c = 0, f = 0, g = 0;
                 (void)hipEventElapsedTime(&b, e[1], e[2]);
                 (void)hipEventElapsedTime(&c, e[2], e[3]);
@@ -2704,8 +2774,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                                   ctx->moon + 2 * k, t, ctx->dt, bc);
                     }
                 } else {
-                    k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0,
-                                                 ctx->dense_parts, 0, ctx->exact_parts, 0);
+                    k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0,
+                                                 ctx->dense_parts, 0, ctx->exact_parts, 0, 0);
                     mark(ctx, 5);
                 }
                 t += ctx->dt;
@@ -2786,6 +2856,13 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
 // ---------------------------------------------------------------------------
 
 constexpr size_t kGraphCache = 8;
+
+// Screening windows per learned step: the configured count, or by default
+// half the sub-steps (two sub-steps per window) when that divides evenly.
+int learned_windows(const khip_learned_config& L) {
+    if (L.windows > 0) return L.windows;
+    return L.substeps % 2 == 0 ? L.substeps / 2 : L.substeps;
+}
 
 bool ensure_learned(khip_ctx* ctx) {
     const int sub = ctx->lcfg.substeps;
@@ -2881,6 +2958,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         KHIP_TRY(ctx, hipMemsetAsync(ctx->cstats, 0, sizeof(CheckStats), s));
         ctx->reset_checks = false;
     }
+    const int nw = learned_windows(L);
+    const int spw = sub / nw;
+    const double hw = ctx->dt / nw;
     LearnedParams lp;
     lp.tau = ctx->dt;
     lp.substeps = sub;
@@ -2975,15 +3055,16 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     mark(ctx, 7);
                     if (screen) {
                         k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
-                        for (int q = 0; q < sub; ++q) {
-                            g_slot = static_cast<int>(k * sub + q);
+                        // Screening windows of tau / nw, each spw sub-steps long.
+                        for (int q = 0; q < nw; ++q) {
+                            g_slot = static_cast<int>(k * sub + q * spw);
                             mark(ctx, 1);
                             const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
-                            const Planes pa = q == sub - 1 ? pn : dense_planes(ctx, q);
-                            k_substep<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, sub, pa,
+                            const Planes pa = q == nw - 1 ? pn : dense_planes(ctx, q);
+                            k_substep<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
                                                                    ctx->step_parts);
-                            if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q), h, thr, 1, info, q,
-                                                sub))
+                            if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
+                                                info, q, nw, spw))
                                 return false;
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
@@ -2992,8 +3073,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                                   ctx->rstats);
                         mark(ctx, 0);
                     } else {
-                        k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,
-                                                     0, ctx->exact_parts, 0);
+                        k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,
+                                                     0, ctx->exact_parts, 0, 0);
                     }
                     cur = nxt;
                 }
@@ -3041,7 +3122,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             bool overflow = false;
             if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
             if (!overflow) {
-                if (ctx->timing) accumulate_timing_learned(ctx, len, sub, screen != 0);
+                if (ctx->timing) accumulate_timing_learned(ctx, len, sub, learned_windows(ctx->lcfg), screen != 0);
                 ctx->cur = cur;
                 ctx->t = t;
                 ctx->force_check = force;
@@ -3769,8 +3850,8 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
                                                   ctx->step_parts);
             ok = enqueue_screen(ctx, ctx->planes(0), ctx->planes(1), t0, 0, 0, dt, ctx->cfg.threshold_km, 0);
         } else {
-            k_finish<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, 0, ctx->dense_parts, 0,
-                                         ctx->exact_parts, 0);
+            k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 1, ctx->pair_parts, 0, 0, ctx->dense_parts, 0,
+                                         ctx->exact_parts, 0, 0);
         }
         if (!ok) break;
         std::vector<khip_event> saved_ev;
@@ -3853,7 +3934,8 @@ int khip_set_precision(khip_ctx* ctx, int mode) {
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
     if (!ctx || !cfg || cfg->substeps < 1 || cfg->substeps > 12 || 12 % cfg->substeps != 0 ||
         cfg->check_every < 1 || !(cfg->tol_km >= 0) || !(cfg->margin_km >= 0) || !(cfg->e_max >= 0) ||
-        !(cfg->a_max_km >= 0) || !(cfg->tol_a_km >= 0))
+        !(cfg->a_max_km >= 0) || !(cfg->tol_a_km >= 0) || cfg->windows < 0 ||
+        (cfg->windows > 0 && cfg->substeps % cfg->windows != 0))
         return KHIP_ERR_ARG;
     ctx->lcfg = *cfg;
     ctx->force_check = true;

@@ -178,3 +178,32 @@ def test_learned_screening_recall_at_2km(engine, kamd, trained):
     assert (el["miss_km"] <= 2.0 + 0.025).all()
     order = np.lexsort((el["j"], el["i"], el["t"], el["step"]))
     assert np.array_equal(order, np.arange(len(el)))
+
+
+def test_screening_windows_do_not_change_events(engine, kamd, trained):
+    """Every event is solved again on plain trajectories in the 10 s sub-step
+    holding its TCA, so the screening window length (60 s steps in six 10 s,
+    three 20 s or two 30 s windows) changes the work, not the events; the
+    statistics keep one record per 10 s sub-step (a window's counts in its
+    first sub-step's record)."""
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    pop = kamd.generate(60000, seed=8, mix=MIX_DENSE)
+    runs = {}
+    try:
+        for nw in (6, 3, 2):
+            _, _, ev, st = _run(engine, pop, 2, 30, 60.0, True, windows=nw)
+            runs[nw] = (np.sort(ev, order=["step", "t", "i", "j"]), st)
+    finally:
+        engine.set_learned()
+    ev6, st6 = runs[6]
+    print(f"windows 6/3/2: {[len(r[0]) for r in runs.values()]} events; candidates per window "
+          f"{[int(r[1]['candidate_pairs'].sum() / (30 * nw)) for nw, r in runs.items()]}")
+    assert len(ev6) > 50
+    for nw in (3, 2):
+        ev, st = runs[nw]
+        assert np.array_equal(ev, ev6), f"windows {nw}"
+        assert len(st) == len(st6) == 30 * 6
+        assert np.array_equal(st["propagated"], st6["propagated"])
+        first = np.arange(len(st)) % (6 // nw) == 0
+        assert (st["candidate_pairs"][~first] == 0).all() and (st["candidate_pairs"][first] > 0).all()
