@@ -279,8 +279,8 @@ int khip_take_breakups(khip_ctx* ctx, khip_breakup_event* out, int64_t cap, int6
  * and so changes the candidate counts in the statistics. */
 typedef struct {
     double cell_mult;     /* >= 1 (default 1: the reference's cell) */
-    int32_t dense_min;    /* cells with at least this many objects use work items (32) */
-    int32_t long_range;   /* sparse-cell ranges longer than this use work items (64) */
+    int32_t dense_min;    /* cells with at least this many objects use work items (128) */
+    int32_t long_range;   /* sparse-cell ranges longer than this use work items (256) */
     int32_t seg;          /* partners per work item (256) */
     int32_t dense_blocks; /* persistent grid of the dense-cell pair kernel (1024, <= 4096) */
     int32_t exact_blocks; /* grid of the exact FP64 kernel (512, <= 2048) */

@@ -73,10 +73,12 @@ constexpr int MAX_PASSES = 8;
 // Cells with at least DENSE_MIN objects (fresh fragment clouds, crowded
 // shells) are screened by work items: CHUNK own objects (one per lane)
 // against up to SEG partners read by the whole wave at once.
-constexpr int DENSE_MIN = 32;  // defaults; the tuned values live in GridState
+constexpr int DENSE_MIN = 128;  // defaults; the tuned values live in GridState
 // A sparse cell's range longer than LONG_RANGE (a neighbouring cloud) is
 // also handed to the work items instead of being walked lane by lane.
-constexpr int LONG_RANGE = 64;
+// (Measured at 1M objects, 2 km: the length-ranked lane walk of k_pairs
+// beats the work items for everything but real clouds.)
+constexpr int LONG_RANGE = 256;
 constexpr int CHUNK = 64;
 constexpr int SEG = 256;
 constexpr int DENSE_BLOCKS = 1024;
@@ -989,14 +991,22 @@ __device__ __forceinline__ int segments(int n, int seg) { return (n + seg - 1) /
 // Dense cell with k objects at sorted positions [b, b + k): per chunk of
 // CHUNK own objects, the range-0 tail from the chunk start plus ranges 1-4,
 // each cut into SEG-long segments.
-// Items of a sparse cell (one chunk, k < DENSE_MIN): its ranges longer than
-// LONG_RANGE objects (range 0 counted from the cell start).
-__device__ __forceinline__ int long_range_items(const int2* rng, int long_range, int seg) {
+// Items of a sparse cell (k < dense_min objects at [b, b + k)): its ranges
+// longer than long_range objects (range 0 counted from the cell start), per
+// chunk of CHUNK own objects (a wave's lanes hold one chunk), cut into
+// seg-long segments; range 0 of chunk i starts at the chunk.
+__device__ __forceinline__ int sparse_chunk_items(int b, int i, const int2* rng, int long_range, int seg) {
     int items = 0;
     for (int r = 0; r < NRANGES; ++r) {
         const int len = rng[r].y - rng[r].x;
-        if (len > long_range) items += segments(len, seg);
+        if (len > long_range) items += segments(r == 0 ? rng[0].y - (b + i * CHUNK) : len, seg);
     }
+    return items;
+}
+
+__device__ __forceinline__ int long_range_items(int b, int k, const int2* rng, int long_range, int seg) {
+    int items = 0;
+    for (int i = 0; i * CHUNK < k; ++i) items += sparse_chunk_items(b, i, rng, long_range, seg);
     return items;
 }
 
@@ -1131,8 +1141,9 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
             rng[r] = i0 < i1 ? make_int2(cell_begin[i0], i1 < nc ? cell_begin[i1] : A) : make_int2(0, 0);
         }
         for (int r = 0; r < NRANGES; ++r) ranges[c * NRANGES + r] = rng[r];
-        items = static_cast<int>(cnt) >= gs->dense_min ? dense_items(b, static_cast<int>(cnt), rng, gs->seg)
-                                                       : long_range_items(rng, gs->long_range, gs->seg);
+        items = static_cast<int>(cnt) >= gs->dense_min
+                    ? dense_items(b, static_cast<int>(cnt), rng, gs->seg)
+                    : long_range_items(b, static_cast<int>(cnt), rng, gs->long_range, gs->seg);
     }
     // Block-aggregated registration of dense cells: two atomics per block.
     int tot_items, tot_dense;
@@ -1175,15 +1186,21 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
         int2 rng[NRANGES];
         for (int r = 0; r < NRANGES; ++r) rng[r] = ranges[c * NRANGES + r];
         if (k < dmin) {
-            // Sparse cell: only its long ranges, own objects = the whole cell.
-            if (threadIdx.x != 0) continue;
-            int off = dc.item_base;
-            for (int r = 0; r < NRANGES; ++r) {
-                const int len = rng[r].y - rng[r].x;
-                if (len <= long_range) continue;
-                for (int sgi = 0; sgi * seg < len; ++sgi, ++off) {
-                    if (static_cast<unsigned int>(off) >= item_cap) continue;
-                    items[off] = PairItem{b, k, rng[r].x + sgi * seg, min(seg, len - sgi * seg), r, 0, 0, 0};
+            // Sparse cell: only its long ranges, one thread per chunk of own objects.
+            for (int i = threadIdx.x; i * CHUNK < k; i += BLOCK) {
+                int off = dc.item_base;
+                for (int j = 0; j < i; ++j) off += sparse_chunk_items(b, j, rng, long_range, seg);
+                const int own_begin = b + i * CHUNK;
+                const int own_count = min(CHUNK, k - i * CHUNK);
+                for (int r = 0; r < NRANGES; ++r) {
+                    if (rng[r].y - rng[r].x <= long_range) continue;
+                    const int r0 = r == 0 ? own_begin : rng[r].x;
+                    const int len = rng[r].y - r0;
+                    for (int sgi = 0; sgi * seg < len; ++sgi, ++off) {
+                        if (static_cast<unsigned int>(off) >= item_cap) continue;
+                        items[off] = PairItem{own_begin, own_count, r0 + sgi * seg, min(seg, len - sgi * seg), r,
+                                              0, 0, 0};
+                    }
                 }
             }
             continue;
@@ -1407,10 +1424,19 @@ struct SurvivorSlots {
 // appended one wave-aggregated atomic at a time (a per-wave reservation
 // would need wave-uniform loops and leave unused slots for k_exact).
 constexpr int PAIR_GROUP = BLOCK;   // sorted objects per block, one per thread
-constexpr int UNIT_BUCKETS = 64;    // unit lengths 1 .. 63, and >= 64
+constexpr int UNIT_BUCKETS = 64;    // unit length classes, longest first (unit_bucket)
 constexpr int UNIT_OBJ_BITS = 8;    // log2(PAIR_GROUP)
 constexpr int PAIR_BATCH = 4;
 static_assert((1 << UNIT_OBJ_BITS) == PAIR_GROUP && UNIT_BUCKETS == WAVE, "unit packing");
+
+// Length class of a unit, longest first: lengths 1-32 one class each
+// (buckets 63..32), then classes 8 long (33-40 -> 31, ..., 273-280 -> 1),
+// longer -> 0.
+__device__ __forceinline__ int unit_bucket(int l) {
+    if (l <= 32) return UNIT_BUCKETS - l;
+    const int c = 31 - ((l - 33) >> 3);
+    return c > 0 ? c : 0;
+}
 
 __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
@@ -1439,7 +1465,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
             q0[r] = first;
             len[r] = l > 0 ? l : 0;
             rank[r] = 0;
-            if (l > 0) rank[r] = atomicAdd(&s_hist[UNIT_BUCKETS - (l < UNIT_BUCKETS ? l : UNIT_BUCKETS)], 1);
+            if (l > 0) rank[r] = atomicAdd(&s_hist[unit_bucket(l)], 1);
         }
         __syncthreads();
         // Exclusive scan of the bucket counts (one wave, one bucket per lane).
@@ -1459,7 +1485,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
         for (int r = 0; r < NRANGES; ++r) {
             const int l = len[r];
             if (l == 0) continue;
-            const int bucket = UNIT_BUCKETS - (l < UNIT_BUCKETS ? l : UNIT_BUCKETS);
+            const int bucket = unit_bucket(l);
             s_unit[s_hist[bucket] + rank[r]] =
                 make_int2(q0[r], static_cast<int>(threadIdx.x) | (r << UNIT_OBJ_BITS) | (l << (UNIT_OBJ_BITS + 3)));
         }
@@ -3237,9 +3263,22 @@ bool autotune(khip_ctx* ctx, int64_t steps, bool allow_cell, double& base_ms, do
         int TileCfg::*field;
         std::vector<int> values;
     };
-    const Knob knobs[] = {{&TileCfg::dense_min, {16, 32, 64, 128}},
-                          {&TileCfg::long_range, {32, 64, 128, 256}},
-                          {&TileCfg::seg, {128, 256, 512}},
+    // The dense-cell threshold and the long-range cut act together (a large
+    // threshold pays only with a long cut): they are searched as pairs.
+    const std::pair<int, int> split[] = {{32, 64}, {64, 128}, {128, 192}, {128, 256}, {256, 256}, {256, 512}};
+    for (const auto& dl : split) {
+        if (!ok || (bestc.dense_min == dl.first && bestc.long_range == dl.second)) continue;
+        TileCfg cand = bestc;
+        cand.dense_min = dl.first;
+        cand.long_range = dl.second;
+        double t = 0.0;
+        ok = trial(cand, t);
+        if (ok && t < 0.98 * bt) {
+            bestc = cand;
+            bt = t;
+        }
+    }
+    const Knob knobs[] = {{&TileCfg::seg, {128, 256, 512}},
                           {&TileCfg::dense_blocks, {512, 1024, 2048, 4096}},
                           {&TileCfg::exact_blocks, {256, 512, 1024, 2048}}};
     for (const Knob& k : knobs) {
