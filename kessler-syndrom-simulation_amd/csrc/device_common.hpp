@@ -203,9 +203,11 @@ __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz
     const float dv2 = vx * vx + vy * vy + vz * vz;
     const float rv = rx * vx + ry * vy + rz * vz;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    float ts = dv2 > 0.0f ? -rv * __builtin_amdgcn_rcpf(dv2) : 0.0f;
-    ts = ts < 0.0f ? 0.0f : ts;
-    ts = ts > dt ? dt : ts;
+    // ts = -rv / dv2 clamped to [0, dt] in one v_max_f32 and one v_med3_f32:
+    // dv2 is floored at 1e-30 (km/s)^2 so that rcp stays finite; below that
+    // floor the relative motion over a step (< 1e-14 km) is far inside the
+    // margins and any ts in [0, dt] gives the same distance.
+    const float ts = __builtin_amdgcn_fmed3f(-rv * __builtin_amdgcn_rcpf(fmaxf(dv2, 1e-30f)), 0.0f, dt);
     const float cx = rx + vx * ts, cy = ry + vy * ts, cz = rz + vz * ts;
     const float d2 = cx * cx + cy * cy + cz * cz;
     const float dvn = fabsf(vx) + fabsf(vy) + fabsf(vz);

@@ -1502,6 +1502,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
             const float4 la = P.lpos[po], va = P.lvel[po];
             const float ax = la.x - static_cast<float>(dx) * cellf;
             const float ay = la.y - static_cast<float>(dy) * cellf;
+            cand += static_cast<unsigned int>(ql - d.x + 1);
             for (int q = d.x; q <= ql; q += PAIR_BATCH) {
                 float4 b[PAIR_BATCH], w[PAIR_BATCH];
 #pragma unroll
@@ -1513,7 +1514,6 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
 #pragma unroll
                 for (int k = 0; k < PAIR_BATCH; ++k) {
                     if (q + k > ql) continue;
-                    ++cand;
                     const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
                     if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
                                             w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f))
