@@ -251,10 +251,10 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
                                                       const MoonSample* __restrict__ moon3,
                                                       double dt, StepPartial* partials) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    bool on = false, was = false;
-    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-    if (i < n) {
+    StepPartial acc = empty_step_partial();
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        bool on = false, was = false;
+        double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
         D3 r{cur.x[i], cur.y[i], cur.z[i]};
         D3 v{cur.vx[i], cur.vy[i], cur.vz[i]};
         rx = r.x; ry = r.y; rz = r.z;
@@ -272,8 +272,9 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
         nxt.x[i] = r.x; nxt.y[i] = r.y; nxt.z[i] = r.z;
         nxt.vx[i] = v.x; nxt.vy[i] = v.y; nxt.vz[i] = v.z;
         on = a;
+        combine(acc, object_partial(on, was, vn, rx, ry, rz));
     }
-    const StepPartial t = block_reduce(object_partial(on, was, vn, rx, ry, rz), s_rows);
+    const StepPartial t = block_reduce(acc, s_rows);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -281,16 +282,13 @@ __global__ void __launch_bounds__(BLOCK) k_propagate(Planes cur, Planes nxt, uin
 __global__ void __launch_bounds__(BLOCK) k_reduce(Planes cur, const uint8_t* alive, int n,
                                                    StepPartial* partials) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    bool on = false;
-    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-    if (i < n && alive[i]) {
-        on = true;
-        rx = cur.x[i]; ry = cur.y[i]; rz = cur.z[i];
+    StepPartial acc = empty_step_partial();
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        if (!alive[i]) continue;
         const double vx = cur.vx[i], vy = cur.vy[i], vz = cur.vz[i];
-        vn = sqrt(vx * vx + vy * vy + vz * vz);
+        combine(acc, object_partial(true, false, sqrt(vx * vx + vy * vy + vz * vz), cur.x[i], cur.y[i], cur.z[i]));
     }
-    const StepPartial t = block_reduce(object_partial(on, false, vn, rx, ry, rz), s_rows);
+    const StepPartial t = block_reduce(acc, s_rows);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -360,8 +358,19 @@ __device__ __forceinline__ void net_wave(const float x[net::NIN], const uint16_t
         for (int k = 0; k < net::NIN; ++k) xt[k] = __shfl(x[k], 16 * t + j, WAVE);
         xin[t] = net::input_operand(xt, g);
     }
+    // Two passes of two 16-object tiles: the hidden activations of four tiles
+    // at once would hold one wave per SIMD (over 400 registers); per tile the
+    // arithmetic is the same, so the outputs are too.
     net::f32x4 out[4];
-    net::forward_tiles<4>(xin, frags, P, lane, out);
+    {
+        const net::u32x4 x0[2] = {xin[0], xin[1]};
+        net::forward_tiles<2>(x0, frags, P, lane, out);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the passes apart (register pressure)
+    {
+        const net::u32x4 x1[2] = {xin[2], xin[3]};
+        net::forward_tiles<2>(x1, frags, P, lane, out + 2);
+    }
     // Output 4g' + e of column j of tile t sits in lane j + 16 g', element e.
     const int mt = lane >> 4, mj = lane & 15;
 #pragma unroll
@@ -531,10 +540,10 @@ __device__ __forceinline__ void hermite_state(const double x0[3], const double v
 __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uint8_t* alive, int n, double dt,
                                                     int q, int sub, Planes out, StepPartial* partials) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    bool on = false;
-    double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
-    if (i < n) {
+    StepPartial acc = empty_step_partial();
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        bool on = false;
+        double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
         const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
         const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
         double ps[3], ws[3], pe[3], we[3];
@@ -559,9 +568,9 @@ __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uin
             const double vb = sqrt(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
             vn = va > vb ? va : vb;
         }
+        combine(acc, object_partial(on, on, vn, rx, ry, rz));
     }
-    StepPartial p = object_partial(on, on, vn, rx, ry, rz);
-    const StepPartial t = block_reduce(p, s_rows);
+    const StepPartial t = block_reduce(acc, s_rows);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
@@ -2434,6 +2443,11 @@ bool upload_moon(khip_ctx* ctx, double t_start, int64_t nsteps) {
 constexpr int64_t kInitialEvents = 1 << 22;
 
 inline int grid_of(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
+// Grid of the kernels that write per-block StepPartials (k_propagate,
+// k_reduce, k_substep; grid-stride): at most PART_BLOCKS partials for
+// k_grid_setup's single block to combine.
+constexpr int PART_BLOCKS = 1024;
+inline int part_grid(int64_t n) { return std::max(1, std::min(grid_of(n), PART_BLOCKS)); }
 
 // Timing marks of step slot `slot` in the current chunk: 0 before the
 // propagation, 1 after it, 2 before the pair kernels, 3 after k_pairs,
@@ -2467,7 +2481,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
-    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->step_parts, grid_of(n), dt, thr, ctx->passes, midpoint,
+    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->step_parts, part_grid(n), dt, thr, ctx->passes, midpoint,
                                      ctx->sbits, tc);
     k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
                                    ctx->idx[0], ctx->hist, tiles, ctx->rec);
@@ -2770,11 +2784,11 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 g_slot = static_cast<int>(k);
                 mark(ctx, 0);
                 if (ctx->precision == KHIP_PRECISION_MIXED)
-                    k_propagate<AccelMixed><<<grid_of(n), BLOCK, 0, s>>>(
+                    k_propagate<AccelMixed><<<part_grid(n), BLOCK, 0, s>>>(
                         ctx->planes(cur), ctx->planes(nxt), ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
                         ctx->step_parts);
                 else
-                    k_propagate<AccelFp64><<<grid_of(n), BLOCK, 0, s>>>(
+                    k_propagate<AccelFp64><<<part_grid(n), BLOCK, 0, s>>>(
                         ctx->planes(cur), ctx->planes(nxt), ctx->alive, n, ctx->moon + 2 * k, ctx->dt,
                         ctx->step_parts);
                 mark(ctx, 1);
@@ -3087,7 +3101,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             mark(ctx, 1);
                             const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
                             const Planes pa = q == nw - 1 ? pn : dense_planes(ctx, q);
-                            k_substep<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
+                            k_substep<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
                                                                    ctx->step_parts);
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw))
@@ -3885,7 +3899,7 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
         if (n > 0) {
             mark(ctx, 0);
             mark(ctx, 1);
-            k_reduce<<<grid_of(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n),
+            k_reduce<<<part_grid(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, static_cast<int>(n),
                                                   ctx->step_parts);
             ok = enqueue_screen(ctx, ctx->planes(0), ctx->planes(1), t0, 0, 0, dt, ctx->cfg.threshold_km, 0);
         } else {
