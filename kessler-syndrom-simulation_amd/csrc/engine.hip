@@ -901,15 +901,17 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
 // r * TBLOCK + thread (r < TITEMS), so every load and store is coalesced;
 // cells are numbered from wave ballots of the head flags and one block-wide
 // prefix over the tile's TITEMS x TWAVES wave segments (in element order).
-//   lpos: position relative to its cell's low corner, w = object index bits;
-//   lvel: velocity, w = the cell's packed z coordinate (int bits), from
-//         which pairs get their z cell offset.
+// Each sorted object's record is two float4 in lrec (32 bytes, one place):
+//   lrec[2e]:     position relative to its cell's low corner, w = object
+//                 index bits;
+//   lrec[2e + 1]: velocity, w = the cell's packed z coordinate (int bits),
+//                 from which pairs get their z cell offset.
 __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        const GridState* gs, const int* boff,
                                                        const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
-                                                       float4* lpos, float4* lvel) {
+                                                       float4* lrec) {
     __shared__ int s_seg[TITEMS * TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
@@ -960,8 +962,8 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
             cell_begin[c] = e;
         }
         cell_of[e] = c;
-        lpos[e] = p[r];
-        lvel[e] = v[r];
+        lrec[2 * e] = p[r];
+        lrec[2 * e + 1] = v[r];
     }
 }
 
@@ -1253,8 +1255,7 @@ struct PairParams {
     const int* cell_begin;
     const int* cell_count;
     const int2* ranges;
-    const float4* lpos;
-    const float4* lvel;
+    const float4* lrec;     // sorted compact records: position, velocity (2 float4 each)
     DevEvent* events;
     unsigned long long cap;
     uint2* surv;                    // pre-filter survivors (object index pairs)
@@ -1508,7 +1509,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
             const int ql = d.x + (d.y >> (UNIT_OBJ_BITS + 3)) - 1;  // last partner
             int dx, dy;
             range_offset(r, dx, dy);
-            const float4 la = P.lpos[po], va = P.lvel[po];
+            const float4 la = P.lrec[2 * po], va = P.lrec[2 * po + 1];
             const float ax = la.x - static_cast<float>(dx) * cellf;
             const float ay = la.y - static_cast<float>(dy) * cellf;
             cand += static_cast<unsigned int>(ql - d.x + 1);
@@ -1517,8 +1518,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
 #pragma unroll
                 for (int k = 0; k < PAIR_BATCH; ++k) {
                     const int qq = q + k < ql ? q + k : ql;
-                    b[k] = P.lpos[qq];
-                    w[k] = P.lvel[qq];
+                    b[k] = P.lrec[2 * qq];
+                    w[k] = P.lrec[2 * qq + 1];
                 }
 #pragma unroll
                 for (int k = 0; k < PAIR_BATCH; ++k) {
@@ -1549,8 +1550,7 @@ constexpr int PARTNER_BATCH = 4;
 // Both evaluate the same expression on the same operands, own object first.
 __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
                                                         const PairItem* __restrict__ items,
-                                                        const float4* __restrict__ lpos,
-                                                        const float4* __restrict__ lvel,
+                                                        const float4* __restrict__ lrec,
                                                         unsigned int item_cap, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     const int lane = static_cast<int>(__lane_id());
@@ -1575,16 +1575,16 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                 const bool has = q < q1;
                 float4 lb = make_float4(0.f, 0.f, 0.f, 0.f), vb = lb;
                 if (has) {
-                    lb = lpos[q];
-                    vb = lvel[q];
+                    lb = lrec[2 * q];
+                    vb = lrec[2 * q + 1];
                 }
                 for (int j0 = 0; j0 < item.own_count; j0 += PARTNER_BATCH) {
                     float4 ta[PARTNER_BATCH], tv[PARTNER_BATCH];
 #pragma unroll
                     for (int j = 0; j < PARTNER_BATCH; ++j) {
                         const int pj = item.own_begin + (j0 + j < item.own_count ? j0 + j : item.own_count - 1);
-                        ta[j] = lpos[pj];
-                        tv[j] = lvel[pj];
+                        ta[j] = lrec[2 * pj];
+                        tv[j] = lrec[2 * pj + 1];
                     }
 #pragma unroll
                     for (int j = 0; j < PARTNER_BATCH; ++j) {
@@ -1612,8 +1612,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
         const bool active = lane < item.own_count;
         float4 la = make_float4(0.f, 0.f, 0.f, 0.f), va = la;
         if (active) {
-            la = lpos[p];
-            va = lvel[p];
+            la = lrec[2 * p];
+            va = lrec[2 * p + 1];
         }
         // Cell-corner offsets folded into the own position: dr = lb - (la - o).
         const float ax = la.x - ox;
@@ -1626,8 +1626,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
 #pragma unroll
             for (int j = 0; j < PARTNER_BATCH; ++j) {
                 const int q = q0 + j < q1 ? q0 + j : q1 - 1;
-                tb[j] = lpos[q];
-                tw[j] = lvel[q];
+                tb[j] = lrec[2 * q];
+                tw[j] = lrec[2 * q + 1];
             }
 #pragma unroll
             for (int j = 0; j < PARTNER_BATCH; ++j) {
@@ -2183,8 +2183,7 @@ struct khip_ctx {
     int* cell_count = nullptr;
     int* cell_of = nullptr;
     int2* ranges = nullptr;
-    float4* lpos = nullptr;
-    float4* lvel = nullptr;
+    float4* lrec = nullptr;  // compact records in sorted order, 2 float4 each
     TileCfg tile{1.0, DENSE_MIN, LONG_RANGE, SEG, DENSE_BLOCKS, EXACT_BLOCKS, 0};
 
     // In-run breakups (plain and mixed forms).
@@ -2338,7 +2337,7 @@ void free_all(khip_ctx* ctx) {
     void* ptrs[] = {ctx->state[0], ctx->state[1], ctx->alive, ctx->radius, ctx->size_m,
                     ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->hist,
                     ctx->totals, ctx->bcount, ctx->cell_key, ctx->cell_begin, ctx->cell_count,
-                    ctx->cell_of, ctx->ranges, ctx->lpos, ctx->lvel, ctx->rec, ctx->coltab, ctx->scen, ctx->steptab, ctx->mass, ctx->snap_mass,
+                    ctx->cell_of, ctx->ranges, ctx->lrec, ctx->rec, ctx->coltab, ctx->scen, ctx->steptab, ctx->mass, ctx->snap_mass,
                     ctx->bstate, ctx->snap_bstate, ctx->jobs, ctx->samples, ctx->blogs,
                     ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
@@ -2381,7 +2380,7 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
            dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
            dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
-           dev_alloc(ctx, &ctx->lpos, cap) && dev_alloc(ctx, &ctx->lvel, cap) &&
+           dev_alloc(ctx, &ctx->lrec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->rec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
@@ -2498,7 +2497,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
     k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
                                          ctx->gs, ctx->bcount, ctx->rec, ctx->cell_key,
-                                         ctx->cell_begin, ctx->cell_of, ctx->lpos, ctx->lvel);
+                                         ctx->cell_begin, ctx->cell_of, ctx->lrec);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_columns<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->coltab);
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
@@ -2515,8 +2514,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.cell_begin = ctx->cell_begin;
     P.cell_count = ctx->cell_count;
     P.ranges = ctx->ranges;
-    P.lpos = ctx->lpos;
-    P.lvel = ctx->lvel;
+    P.lrec = ctx->lrec;
     P.events = ctx->events;
     P.cap = static_cast<unsigned long long>(ctx->ev_cap);
     P.surv = ctx->surv;
@@ -2538,7 +2536,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     const int pair_blocks = std::max(1, (n + PAIR_GROUP - 1) / PAIR_GROUP);
     k_pairs<<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     mark(ctx, 3);
-    k_pairs_dense<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lpos, ctx->lvel, icap,
+    k_pairs_dense<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
                                                  ctx->dense_parts);
     mark(ctx, 4);
     k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
