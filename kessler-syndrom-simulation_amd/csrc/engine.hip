@@ -485,20 +485,30 @@ __global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8
     }
 }
 
-// Per-method index lists of the objects alive after a check.
+// Per-method index lists of the objects alive after a check. Same-address
+// atomics serialize (about 12 ns each on MI355X), so each block takes one
+// contiguous chunk of objects (LIST_BLOCKS blocks at most): a first pass
+// counts the chunk's objects per method, one atomic per method reserves the
+// chunk's stretch of each list, and a second pass writes the indices in
+// object order (wave ballots, running offsets).
+constexpr int LIST_BLOCKS = 256;
+
 __global__ void __launch_bounds__(BLOCK) k_build_lists(const uint8_t* alive, const uint8_t* mode, int n,
                                                         int* lists, int cap, int* counts) {
     __shared__ int s_base[3];
     __shared__ int s_wave[3][WAVES_PER_BLOCK];
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    const int m = i < n && alive[i] ? mode[i] : -1;
+    const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
+    const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     const int wave = threadIdx.x / WAVE;
     const unsigned long long below = lanes_below();
-    int rank[3], tot[3];
+    int cnt[3] = {0, 0, 0};
+    for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+        const int m = alive[i] ? mode[i] : -1;
+        for (int q = 0; q < 3; ++q) cnt[q] += m == q ? 1 : 0;
+    }
     for (int q = 0; q < 3; ++q) {
-        const unsigned long long b = __ballot(m == q);
-        rank[q] = __popcll(b & below);
-        if (__lane_id() == 0) s_wave[q][wave] = __popcll(b);
+        const int c = static_cast<int>(wave_sum<unsigned int>(static_cast<unsigned int>(cnt[q])));
+        if (__lane_id() == 0) s_wave[q][wave] = c;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -509,12 +519,27 @@ __global__ void __launch_bounds__(BLOCK) k_build_lists(const uint8_t* alive, con
         }
     }
     __syncthreads();
-    if (m >= 0) {
-        int off = s_base[m] + rank[m];
-        for (int w = 0; w < wave; ++w) off += s_wave[m][w];
-        lists[m * cap + off] = i;
+    int run[3] = {s_base[0], s_base[1], s_base[2]};
+    __syncthreads();  // s_wave is reused below
+    for (int b = i0; b < i1; b += BLOCK) {
+        const int i = b + threadIdx.x;
+        const int m = i < i1 && alive[i] ? mode[i] : -1;
+        int rank[3];
+        for (int q = 0; q < 3; ++q) {
+            const unsigned long long bq = __ballot(m == q);
+            rank[q] = __popcll(bq & below);
+            if (__lane_id() == 0) s_wave[q][wave] = __popcll(bq);
+        }
+        __syncthreads();
+        if (m >= 0) {
+            int off = run[m] + rank[m];
+            for (int w = 0; w < wave; ++w) off += s_wave[m][w];
+            lists[m * cap + off] = i;
+        }
+        for (int q = 0; q < 3; ++q)
+            for (int w = 0; w < WAVES_PER_BLOCK; ++w) run[q] += s_wave[q][w];
+        __syncthreads();
     }
-    (void)tot;
 }
 
 // Dense output: state at fraction s of the step from the cubic Hermite of an
@@ -3077,7 +3102,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
                             pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
                             ctx->net_norm, lp, ctx->cstats);
-                        k_build_lists<<<grid_of(n), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
+                        k_build_lists<<<std::max(1, std::min(grid_of(n), LIST_BLOCKS)), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
                                                                    static_cast<int>(c), ctx->counts);
                     } else {
                         k_learned<net::CLASS_LEARNED><<<g_net, BLOCK, 0, s>>>(
