@@ -66,6 +66,7 @@ constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
 constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
 constexpr int TWAVES = TBLOCK / 64;
+constexpr int COL_BITS = 24;  // column table index width (see k_cell_write)
 constexpr int RADIX = 256;
 constexpr int KEY_BITS_MAX = 63;       // packed key width; the dead sentinel is all ones
 constexpr uint64_t DEAD_KEY = ~0ULL;
@@ -936,7 +937,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
                                                        const GridState* gs, const int* boff,
                                                        const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
-                                                       float4* lrec) {
+                                                       float4* lrec, int4* __restrict__ coltab) {
     __shared__ int s_seg[TITEMS * TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
@@ -947,7 +948,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
     const unsigned long long below = lanes_below();
-    uint64_t key[TITEMS];
+    uint64_t key[TITEMS], prv[TITEMS];
     unsigned long long bal[TITEMS];
     bool head[TITEMS];
     float4 p[TITEMS], v[TITEMS];
@@ -958,6 +959,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
         key[r] = valid ? keys[e] : 0ULL;
         uint64_t prev = __shfl_up(key[r], 1, WAVE);
         if (lane == 0) prev = valid && e > 0 ? keys[e - 1] : ~0ULL;
+        prv[r] = prev;
         head[r] = valid && (e == 0 || key[r] != prev);
         bal[r] = __ballot(head[r]);
         if (lane == 0) s_seg[r * TWAVES + wave] = __popcll(bal[r]);
@@ -977,6 +979,11 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
             run += s_seg[r * TWAVES + w];
         }
     }
+    // Column table (when the column index fits COL_BITS): the first cell of
+    // a column opens its entry and closes the previous column's; the last
+    // element closes its own column.
+    const bool use_tab = g.sbits + g.bits[0] + g.bits[1] <= COL_BITS;
+    const int bz = g.bits[2];
 #pragma unroll
     for (int r = 0; r < TITEMS; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
@@ -985,7 +992,14 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
         if (head[r]) {
             cell_key[c] = key[r];
             cell_begin[c] = e;
+            const uint64_t col = key[r] >> bz;
+            if (use_tab && (e == 0 || (prv[r] >> bz) != col)) {
+                coltab[col].x = g.stamp;
+                coltab[col].y = c;
+                if (e > 0) coltab[prv[r] >> bz].z = c;
+            }
         }
+        if (use_tab && e == A - 1) coltab[key[r] >> bz].z = c + 1;
         cell_of[e] = c;
         lrec[2 * e] = p[r];
         lrec[2 * e + 1] = v[r];
@@ -1054,27 +1068,12 @@ __device__ __forceinline__ int dense_items(int b, int k, const int2* rng, int se
     return items;
 }
 
-// Column table: for every occupied (x, y) column of the packed key space,
-// the run [begin, end) of its cells in the sorted cell table, tagged with
-// the screening's stamp (entries of earlier screenings read as empty, so
-// the table is never cleared). Used when the column index fits COL_BITS.
-constexpr int COL_BITS = 24;
+// Column table (written by k_cell_write): for every occupied (x, y) column
+// of the packed key space, the run [begin, end) of its cells in the sorted
+// cell table, tagged with the screening's stamp (entries of earlier
+// screenings read as empty, so the table is never cleared). Used when the
+// column index fits COL_BITS.
 
-__global__ void __launch_bounds__(BLOCK) k_columns(const GridState* gs, const uint64_t* cell_key, int4* coltab) {
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    const int nc = static_cast<int>(gs->ncells);
-    if (c >= nc || gs->sbits + gs->bits[0] + gs->bits[1] > COL_BITS) return;
-    const int bz = gs->bits[2];
-    const uint64_t col = cell_key[c] >> bz;
-    const bool first = c == 0 || (cell_key[c - 1] >> bz) != col;
-    const bool last = c + 1 == nc || (cell_key[c + 1] >> bz) != col;
-    int4* e = coltab + col;
-    if (first) {
-        e->x = gs->stamp;
-        e->y = c;
-    }
-    if (last) e->z = c + 1;
-}
 
 __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint64_t* cell_key,
                                                        const int* cell_begin, int* cell_count,
@@ -2522,9 +2521,8 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
     k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
                                          ctx->gs, ctx->bcount, ctx->rec, ctx->cell_key,
-                                         ctx->cell_begin, ctx->cell_of, ctx->lrec);
+                                         ctx->cell_begin, ctx->cell_of, ctx->lrec, ctx->coltab);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
-    k_columns<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->coltab);
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
                                               ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
                                               icap, ctx->coltab);
