@@ -196,7 +196,7 @@ __device__ __forceinline__ double golden_min(const Hermite& h) {
 // DR_ERR_KM + DV_ERR_KM_S * dt, folded by the caller.
 __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz, float vx, float vy,
                                                     float vz, float dt, float gdt3, float base,
-                                                    float vmin2) {
+                                                    float vmin2, float lim_max2) {
     // FP32 only: fused multiply-adds round once instead of twice, which can
     // only shrink the error the margins cover.
 #pragma clang fp contract(fast)
@@ -210,6 +210,10 @@ __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz
     const float ts = __builtin_amdgcn_fmed3f(-rv * __builtin_amdgcn_rcpf(fmaxf(dv2, 1e-30f)), 0.0f, dt);
     const float cx = rx + vx * ts, cy = ry + vy * ts, cz = rz + vz * ts;
     const float d2 = cx * cx + cy * cy + cz * cz;
+    // lim_max2 bounds lim^2 below from above for every candidate of the
+    // screening (the caller's pair_lim_max2): most candidates leave here,
+    // before the margins are computed.
+    if (d2 > lim_max2) return false;
     const float dvn = fabsf(vx) + fabsf(vy) + fabsf(vz);
     const float rn = fabsf(rx) + fabsf(ry) + fabsf(rz);
     const float lim = base + gdt3 * dvn + 1e-5f * (rn + dvn * dt);

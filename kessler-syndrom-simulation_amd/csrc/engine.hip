@@ -1294,6 +1294,20 @@ struct PairParams {
     float base_f;   // threshold + 1e-3 + DR_ERR_KM + DV_ERR_KM_S * dt
 };
 
+// Upper bound of the pre-filter's lim^2 over every candidate pair of a
+// screening: partners lie within two cell edges of the own object per axis
+// (neighbour cells), so |dr|_1 <= 6 cell, and |dv|_1 <= sqrt(3) |dv| <=
+// 2 sqrt(3) vmax (vmax: the largest speed of the velocities in the records);
+// lim grows with both. Safety factors cover FP32 rounding.
+__device__ __forceinline__ float pair_lim_max2(const GridState* gs, const PairParams& P) {
+    const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
+    const double dvn = 2.0 * 1.7320508075688772 * vmax * 1.001;
+    const double rn = 6.01 * gs->cell;
+    const double lim = static_cast<double>(P.base_f) + static_cast<double>(P.gdt3_f) * dvn +
+                       1e-5 * (rn + dvn * static_cast<double>(P.dt_f));
+    return static_cast<float>(lim * lim * 1.001);
+}
+
 struct PairCounters {
     unsigned int cand, refined, co;
 };
@@ -1482,6 +1496,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
     unsigned int cand = 0;
+    const float lim_max2 = pair_lim_max2(gs, P);
     if (gs->npasses > 0 && g0 < A) {  // uniform over the block
         if (threadIdx.x < UNIT_BUCKETS) s_hist[threadIdx.x] = 0;
         __syncthreads();
@@ -1550,7 +1565,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
                     if (q + k > ql) continue;
                     const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
                     if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
-                                            w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f))
+                                            w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f, lim_max2))
                         push_survivor(gs, P, __float_as_int(la.w), __float_as_int(b[k].w));
                 }
             }
@@ -1584,6 +1599,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
     const float cellf = static_cast<float>(gs->cell);
     const unsigned long long below = lanes_below();
     unsigned int cand = 0;
+    const float lim_max2 = pair_lim_max2(gs, P);
     SurvivorSlots slots;
     for (unsigned int it = blockIdx.x * WAVES_PER_BLOCK + wave; it < n_items; it += total_waves) {
         const PairItem item = items[it];
@@ -1624,7 +1640,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                                 static_cast<float>(__float_as_int(vb.w) - __float_as_int(va.w)) * cellf;
                             keep = prefilter_keep_fast(lb.x - ax, lb.y - ay, oz + (lb.z - la.z), vb.x - va.x,
                                                        vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
-                                                       P.vmin2_f);
+                                                       P.vmin2_f, lim_max2);
                         }
                         slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
                     }
@@ -1666,7 +1682,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                     // same operands, one rounding each, inside DR_ERR_KM.
                     keep = prefilter_keep_fast(lb.x - ax, lb.y - ay, oz + (lb.z - la.z), vb.x - va.x,
                                                vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
-                                               P.vmin2_f);
+                                               P.vmin2_f, lim_max2);
                 }
                 slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
             }

@@ -308,3 +308,53 @@ def test_fast_prefilter_is_conservative(orc, rcp_err):
     lost = (exact != 0) & ~keep
     assert not lost.any(), np.flatnonzero(lost)[:10]
     assert (exact != 0).sum() > 1000  # the sample exercises the boundary
+
+
+def _lim_max2(base, gdt3, dt, vmax, cell):
+    """pair_lim_max2 of the device (engine.hip): lim over the extreme
+    candidate (|dr|_1 = 6 cells, |dv|_1 = 2 sqrt(3) vmax) with its safety
+    factors, squared and rounded to float32."""
+    dvn = 2.0 * np.sqrt(3.0) * vmax * 1.001
+    rn = 6.01 * cell
+    lim = float(base) + float(gdt3) * dvn + 1e-5 * (rn + dvn * float(dt))
+    return np.float32(lim * lim * 1.001)
+
+
+@pytest.mark.parametrize("dt,cell,vmax,thr", [(10.0, 207.5, 10.27, 2.0), (20.0, 207.5, 10.27, 2.025),
+                                              (10.0, 223.0, 11.0, 5.0), (60.0, 900.0, 8.0, 2.0)])
+def test_prefilter_lim_max_bounds_every_candidate(dt, cell, vmax, thr):
+    """The pair kernels reject a candidate at once when d^2 > lim_max2; that
+    is the full test's decision only if lim^2 <= lim_max2 for every candidate
+    a screening can produce: partners in a neighbouring cell (offset in
+    {-1, 0, 1} per axis, both positions anywhere in their cells), speeds up
+    to vmax. Random candidates plus the extreme corners and opposite
+    diagonal velocities, lim in float32 as the device computes it."""
+    f = np.float32
+    rng = np.random.default_rng(11)
+    n = 200000
+    la = rng.uniform(0, cell, (n, 3))
+    lb = rng.uniform(0, cell, (n, 3))
+    off = rng.integers(-1, 2, (n, 3))
+    ua = rng.normal(size=(n, 3))
+    ub = rng.normal(size=(n, 3))
+    ua *= (vmax * rng.uniform(0, 1, n) ** 0.2 / np.linalg.norm(ua, axis=1))[:, None]
+    ub *= (vmax * rng.uniform(0, 1, n) ** 0.2 / np.linalg.norm(ub, axis=1))[:, None]
+    # extreme corners: |dr| = 2 cells per axis, opposite velocities along a diagonal
+    k = n // 10
+    sgn = rng.choice([-1.0, 1.0], (k, 3))
+    la[:k] = np.where(sgn > 0, 0.0, cell)
+    lb[:k] = np.where(sgn > 0, cell, 0.0)
+    off[:k] = sgn.astype(int)
+    ua[:k] = -sgn * vmax / np.sqrt(3.0)
+    ub[:k] = sgn * vmax / np.sqrt(3.0)
+    r = (off * cell + lb - la).astype(f)
+    v = (ub - ua).astype(f)
+    base = f(thr) + f(1e-3) + f(5e-4) + f(5e-6) * f(dt)
+    gdt3 = f(2e-6) * f(dt) * f(dt) * f(dt)
+    dvn = np.abs(v).sum(1, dtype=f)
+    rn = np.abs(r).sum(1, dtype=f)
+    lim = base + gdt3 * dvn + f(1e-5) * (rn + dvn * f(dt))
+    bound = _lim_max2(base, gdt3, dt, vmax, cell)
+    assert (lim * lim <= bound).all(), float((lim * lim).max()) / float(bound)
+    # the bound is tight enough to matter: within 10% of the worst candidate
+    assert float(bound) < 1.1 * float((lim * lim).max())
