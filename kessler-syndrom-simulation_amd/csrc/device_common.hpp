@@ -203,6 +203,18 @@ __device__ __forceinline__ bool prefilter_keep_fast(float rx, float ry, float rz
     const float dv2 = vx * vx + vy * vy + vz * vz;
     const float rv = rx * vx + ry * vy + rz * vz;
     const float r2 = rx * rx + ry * ry + rz * rz;
+    // Reciprocal-free first rejection. r2 dv2 - rv^2 = dv2 * (squared
+    // distance of the line r + v t from the origin, over all t), a lower
+    // bound of the distance over [0, dt]. The test
+    //   fma(r2, K, -L) * dv2 > fma(rv, rv, 1e-30),  L = 1.00001 lim_max2,
+    // K = 1 - 2e-6, holds only if that exact distance exceeds L: the three
+    // dot products (relative error <= 3 ulp each, Cauchy-Schwarz for rv),
+    // the fma and the two products err by less than 16 ulp of r2 dv2, which
+    // K's 2e-6 covers, and the 1e-30 keeps every product above the
+    // subnormal range. The d2 computed below (fma-rounded, >= (1 - 6 ulp)
+    // of the exact distance at any t) then exceeds lim_max2, so this test
+    // rejects only candidates that the d2 > lim_max2 test rejects too.
+    if (fmaf(r2, 0.999998f, -1.00001f * lim_max2) * dv2 > fmaf(rv, rv, 1e-30f)) return false;
     // ts = -rv / dv2 clamped to [0, dt] in one v_max_f32 and one v_med3_f32:
     // dv2 is floored at 1e-30 (km/s)^2 so that rcp stays finite; below that
     // floor the relative motion over a step (< 1e-14 km) is far inside the

@@ -15,13 +15,15 @@
 //                 wave).
 //   k_grid_setup  cell size (collision.cpp:339-356) and a dense cell key:
 //                 per-axis cell offsets from the bounding box, packed into
-//                 at most 31 bits, so the radix sort runs 1-4 byte passes.
+//                 as few bits as the box needs (27 at 1M objects), so the
+//                 radix sort runs that many bits / 9 passes.
 //   k_keys        packed cell key per object (dead objects get a sentinel
 //                 that sorts last) + first-pass digit histogram.
 //   k_hist / k_digit_scan / k_scatter
-//                 stable LSD radix sort of (key, index): per-tile digit
-//                 histograms, per-digit scans, then a scatter whose in-tile
-//                 ranks come from wave64 ballots (match on the 8 digit bits).
+//                 stable LSD radix sort of (key, index), 9-bit digits:
+//                 per-tile digit histograms, per-digit scans, then a scatter
+//                 whose in-tile ranks come from wave64 ballots (match on the
+//                 9 digit bits).
 //   k_cell_count / k_cell_scan / k_cell_write
 //                 cell boundaries by a wavefront scan of the head flags;
 //                 writes the cell table and the compact FP32 copy of each
@@ -67,7 +69,12 @@ constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
 constexpr int TWAVES = TBLOCK / 64;
 constexpr int COL_BITS = 24;  // column table index width (see k_cell_write)
-constexpr int RADIX = 256;
+// LSD radix sort of the packed cell keys, 9-bit digits: at 1M objects the
+// key has 27 bits (about 400 cells of 200 km per axis), which three passes
+// cover.
+constexpr int RADIX_BITS = 9;
+constexpr int RADIX = 1 << RADIX_BITS;
+constexpr uint64_t DIGIT_MASK = RADIX - 1;
 constexpr int KEY_BITS_MAX = 63;       // packed key width; the dead sentinel is all ones
 constexpr uint64_t DEAD_KEY = ~0ULL;
 constexpr int MAX_PASSES = 8;
@@ -694,7 +701,15 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
             total += gs->bits[a];
         }
         if (total <= KEY_BITS_MAX) {
-            gs->npasses = (total + 8) / 8;
+            // Dead objects carry the all-ones key and must sort after every
+            // live one (the sort is stable, so equal keys would keep index
+            // order). A live key is all ones in its low `total` bits only
+            // if its top cell field is; while that field's largest value
+            // stays below all ones, `total` sorted bits keep dead keys
+            // last, otherwise one more bit is sorted.
+            const long long top = static_cast<long long>(floor(hi[0] / cell)) - gs->imin[0];
+            const bool top_full = top == (1LL << gs->bits[0]) - 1;
+            gs->npasses = (total + (top_full ? 1 : 0) + RADIX_BITS - 1) / RADIX_BITS;
             break;
         }
         cell = cell * 2.0;
@@ -747,7 +762,7 @@ __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int mid
             }
             keys[e] = k;
             idx[e] = e;
-            atomicAdd(&s_hist[static_cast<int>(k & 255u)], 1);
+            atomicAdd(&s_hist[static_cast<int>(k & DIGIT_MASK)], 1);
         }
     }
     __syncthreads();
@@ -760,11 +775,11 @@ __global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int n, in
     if (pass >= gs->npasses) return;
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
-    const int shift = 8 * pass;
+    const int shift = RADIX_BITS * pass;
     const int base = blockIdx.x * TILE;
     for (int r = 0; r < TITEMS; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
-        if (e < n) atomicAdd(&s_hist[static_cast<int>((keys[e] >> shift) & 255u)], 1);
+        if (e < n) atomicAdd(&s_hist[static_cast<int>((keys[e] >> shift) & DIGIT_MASK)], 1);
     }
     __syncthreads();
     if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
@@ -812,7 +827,7 @@ __global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int
 
 // Stable scatter of one radix pass. In-tile ranks: elements are taken in
 // index order, TBLOCK per round; a wave finds the lanes holding the same
-// digit with 8 ballots, and per-wave digit counts in LDS give the
+// digit with RADIX_BITS ballots, and per-wave digit counts in LDS give the
 // cross-wave offsets.
 __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
                                                      uint64_t* keys_out, int* idx_out, int n,
@@ -833,7 +848,7 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
         }
     }
     __syncthreads();
-    const int shift = 8 * pass;
+    const int shift = RADIX_BITS * pass;
     const int wave = t / WAVE;
     const unsigned long long below = lanes_below();
     const int base = blockIdx.x * TILE;
@@ -851,9 +866,9 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
         const bool valid = e < n;
         const uint64_t key = keys[r];
         const int id = ids[r];
-        const int d = static_cast<int>((key >> shift) & 255u);
+        const int d = static_cast<int>((key >> shift) & DIGIT_MASK);
         unsigned long long peers = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < RADIX_BITS; ++b) {
             const bool bit = (d >> b) & 1;
             const unsigned long long m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -2273,7 +2288,7 @@ struct khip_ctx {
     double epoch_jd = JD_DEFAULT_EPOCH, t = 0.0, dt = 10.0;
     int64_t step = 0;
     int chunk = 256;
-    int passes = 4;   // radix passes launched per step (adapted per chunk)
+    int passes = 3;   // radix passes launched per step (adapted per chunk)
     int precision = KHIP_PRECISION_FP64;
 
     // Learned corrector: FP32 master parameters, Adam moments, BF16

@@ -358,3 +358,80 @@ def test_prefilter_lim_max_bounds_every_candidate(dt, cell, vmax, thr):
     assert (lim * lim <= bound).all(), float((lim * lim).max()) / float(bound)
     # the bound is tight enough to matter: within 10% of the worst candidate
     assert float(bound) < 1.1 * float((lim * lim).max())
+
+
+def _fma32(a, b, c):
+    """float32 fused multiply-add (the float64 product of two float32 is exact)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64)).astype(np.float32)
+
+
+def _dot32(a, b):
+    """fma chain of a 3-vector dot product, as the device contracts it."""
+    return _fma32(a[:, 2], b[:, 2], _fma32(a[:, 1], b[:, 1], (a[:, 0] * b[:, 0]).astype(np.float32)))
+
+
+def _early_rejection_pairs(rng, lim_max2, dt, n):
+    """Relative positions and velocities for the early-rejection checks:
+    lines passing at distance^2 within 0.3% of lim_max2 with the closest
+    approach before, inside or after the window and partners up to 300 km
+    apart (where cancellation in r2 dv2 - rv^2 is largest), random pairs,
+    and nearly co-moving pairs (speeds down to 1e-20 km/s)."""
+    k = n // 3
+    u = rng.normal(size=(k, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = rng.normal(size=(k, 3))
+    w -= (w * u).sum(1, keepdims=True) * u
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    speed = 10.0 ** rng.uniform(-1, 1.3, k)
+    t_star = rng.uniform(-dt, 2 * dt, k)
+    t_star[: k // 2] = np.sign(t_star[: k // 2]) * np.minimum(np.abs(t_star[: k // 2]), 300.0 / speed[: k // 2])
+    miss = np.sqrt(float(lim_max2) * (1 + rng.uniform(-3e-3, 3e-3, k)))
+    r1 = u * miss[:, None] - w * (speed * t_star)[:, None]
+    v1 = w * speed[:, None]
+    r2 = rng.uniform(-500, 500, (k, 3))
+    v2 = rng.normal(size=(k, 3)) * 10.0
+    m = n - 2 * k
+    r3 = rng.normal(size=(m, 3))
+    r3 *= (np.sqrt(float(lim_max2)) * rng.uniform(0.9, 1.1, m) / np.linalg.norm(r3, axis=1))[:, None]
+    v3 = rng.normal(size=(m, 3)) * 10.0 ** rng.uniform(-20, -8, m)[:, None]
+    return (np.concatenate([r1, r2, r3]).astype(np.float32), np.concatenate([v1, v2, v3]).astype(np.float32))
+
+
+def _early_reject(r, v, lim_max2, K):
+    f = np.float32
+    dv2, rv, r2 = _dot32(v, v), _dot32(r, v), _dot32(r, r)
+    lq = f(f(1.00001) * f(lim_max2))
+    return (_fma32(r2, f(K), -lq) * dv2).astype(f) > _fma32(rv, rv, f(1e-30))
+
+
+def _clamped_d2(r, v, dt, rcp_err):
+    """d2 of the device's prefilter_keep_fast: reciprocal (perturbed by
+    rcp_err relative), clamp to [0, dt], fma-rounded closest point."""
+    f = np.float32
+    dv2, rv = _dot32(v, v), _dot32(r, v)
+    inv = (f(1) / np.maximum(dv2, f(1e-30))).astype(f) * f(1 + rcp_err)
+    ts = np.clip((-rv * inv).astype(f), f(0), f(dt)).astype(f)
+    c = np.stack([_fma32(v[:, i], ts, r[:, i]) for i in range(3)], 1)
+    return _dot32(c, c)
+
+
+@pytest.mark.parametrize("lim_max2,dt", [(4.41, 10.0), (4.52, 20.0), (26.8, 10.0), (4.9, 60.0)])
+def test_prefilter_early_rejection_implies_d2_rejection(lim_max2, dt):
+    """The pre-filter's reciprocal-free first test rejects a candidate only
+    if the d^2 > lim_max2 test after it would: its outcome never changes the
+    pre-filter's decision, so candidates, survivors and events are those of
+    the test without it. Checked on 300k pairs at the boundary, with the
+    reciprocal error pushed to +-4 ulp; the same check with the slack factor
+    K = 1 (no allowance for the cancellation in r2 dv2 - rv^2) fails on the
+    same pairs, so they reach the cases the factor is for."""
+    rng = np.random.default_rng(21)
+    r, v = _early_rejection_pairs(rng, lim_max2, dt, 300000)
+    lm = np.float32(lim_max2)
+    rej = _early_reject(r, v, lm, 0.999998)
+    assert rej[100000:200000].mean() > 0.99  # random pairs in neighbouring cells: nearly all leave here
+    for rcp_err in (-2.0 ** -21, 0.0, 2.0 ** -21):
+        d2 = _clamped_d2(r, v, dt, rcp_err)
+        bad = rej & ~(d2 > lm)
+        assert not bad.any(), (rcp_err, np.flatnonzero(bad)[:5])
+    planted = _early_reject(r, v, lm, 1.0) & ~(_clamped_d2(r, v, dt, 0.0) > lm)
+    assert planted.any()
