@@ -50,6 +50,28 @@ FLOP_PER_CANDIDATE = 60.0
 BYTES_PER_OBJECT_STEP = 97.0
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+# Step model: HBM bytes per alive object per screened step (FP64 state in
+# and out 96, key and index 12, one sort pass 24, compact record write 32
+# and read 32, cell begin and count 8) and FP64 flops per RK4 step.
+B_OBJ_SCREENED = 204.0
+F64_RK4 = 326.0
+
+
+def step_model(objects, windows, rk4_steps, candidates, corrector_evals, t_measured_s):
+    """Roofline of one whole step: the largest of its HBM, FP64, FP32 and
+    BF16-MFMA terms at the MI355X peaks, against the measured step time
+    (all counts per step; `windows` = screenings per step)."""
+    terms = {"hbm": objects * windows * B_OBJ_SCREENED / (PEAK_HBM_GBS * 1e9),
+             "fp64_rk4": rk4_steps * F64_RK4 / (PEAK_FP64_TFLOPS * 1e12),
+             "fp32_prefilter": candidates * FLOP_PER_CANDIDATE / (PEAK_FP32_TFLOPS * 1e12),
+             "mfma_bf16_corrector": corrector_evals * FLOP_PER_INFERENCE / (PEAK_BF16_TFLOPS * 1e12)}
+    binding = max(terms, key=terms.get)
+    return {"model": "T_roof = max(N x screenings x 204 B / 8 TB/s, RK4 steps x 326 / 78.6 TFLOP/s, "
+                     "candidates x 60 / 157.3 TFLOP/s, corrector evaluations x 69,632 / 2.5 PFLOP/s)",
+            "terms_us": {k: v * 1e6 for k, v in terms.items()}, "binding": binding,
+            "t_roof_us": terms[binding] * 1e6, "t_measured_us": t_measured_s * 1e6,
+            "achieved": terms[binding] / t_measured_s}
 
 
 def cascade_population(kamd):
@@ -346,6 +368,16 @@ def learned_workload(kamd, args):
     roof["kernel_ms_per_learned_step"] = {k: v / max(1, int(tm["steps"])) for k, v in kernels.items()}
     rep = lrn["report"]
     n_net = int(rep["learned"]) * int(tm["steps"])
+    # Per 60 s step: three screenings; RK4 steps of the three methods, with
+    # check steps (every check_every-th) running every object's plain method
+    # beside its own.
+    n_obj = obj_l / (6 * steps)
+    f_chk = int(rep["checks"]) / max(1, int(rep["steps"]))
+    rk4_l = ((1 - f_chk) * (int(rep["learned"]) + int(rep["base"]) + 6 * int(rep["plain"]))
+             + f_chk * 7 * n_obj)
+    model_l = step_model(n_obj, WINDOWS, rk4_l, cand / steps, int(rep["learned"]), lrn["wall"] / steps)
+    model_p = step_model(obj_p / (6 * steps), 1, obj_p / (6 * steps),
+                         float(plain["stats"]["candidate_pairs"].mean()), 0, plain["wall"] / (6 * steps))
     mf = FLOP_PER_INFERENCE * n_net / (tm["propagate_ms"] * 1e-3) / 1e12 if tm["propagate_ms"] > 0 else None
     out = {
         "metric": METRIC, "value": v_l, "unit": "object-steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
@@ -368,10 +400,11 @@ def learned_workload(kamd, args):
         "corrector_mfma_tflops_in_propagation": mf,
         "candidate_pairs_per_window": cand / launches_sub,
         "candidate_pairs_per_step_plain_fp64": float(plain["stats"]["candidate_pairs"].mean()),
-        "roofline": roof, "device": eng.device_name,
+        "roofline": roof, "step_roofline": model_l, "step_roofline_plain_fp64": model_p,
+        "device": eng.device_name,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_window(kamd, pop, thr, budget_s=20.0)
+        out["cpu_baseline"] = cpu_baseline_window(kamd, pop, thr)
         out["gpu_over_cpu"] = v_l / out["cpu_baseline"]["value"]
     eng.close()
     print(json.dumps(out))
@@ -469,7 +502,7 @@ def ensemble_workload(kamd, args):
     print(json.dumps(out))
 
 
-def cpu_baseline_window(kamd, pop, thr, budget_s=20.0):
+def cpu_baseline_window(kamd, pop, thr, budget_s=12.0, max_steps=400):
     """The reference's CPU path (plain FP64, 10 s) on the same catalogue,
     stepped from the start until budget_s of loop time."""
     from oracle.bind import oracle, reference
@@ -482,7 +515,7 @@ def cpu_baseline_window(kamd, pop, thr, budget_s=20.0):
     states = pop["state"].copy()
     alive = pop["alive"].astype(np.uint8)
     spent, k, obj = 0.0, 0, 0
-    while spent < budget_s and k < 20:
+    while spent < budget_s and k < max_steps:
         n_alive = int(alive.sum())
         states, alive, _, _, loop_s = ref.run(states, alive, pop["size_m"], EPOCH, DT, 1, screen=True, thr=thr,
                                               t_start=k * DT, per_step=False)
@@ -597,6 +630,8 @@ def main():
         "candidate_pairs": cand,
         "roofline": roof,
         "roofline_propagate": roof_prop,
+        "step_roofline": step_model(obj_steps / args.steps, 1, obj_steps / args.steps, cand / args.steps, 0,
+                                    wall / args.steps),
         "device": eng.device_name,
     }
     if not args.no_cpu:
