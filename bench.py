@@ -267,7 +267,7 @@ def recall_against(e_ref, e_new):
 # HBM bytes per launch of the pair kernels in the learned workload (20 s
 # screening windows), measured by two rocprofv3 PMC passes (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, factors from copy kernels of known size in the same runs).
-PAIR_TRAFFIC_B = {"k_pairs": 174.5e6, "k_pairs_dense": 0.08e6}
+PAIR_TRAFFIC_B = {"k_pairs": 160.9e6, "k_pairs_dense": 0.1e6}
 # Screening windows per 60 s learned step (20 s each; propagation and the
 # plain re-verification keep 10 s sub-steps).
 WINDOWS = 3
