@@ -358,13 +358,17 @@ __device__ __forceinline__ void net_wave(const float x[net::NIN], const uint16_t
                                          float o[net::NOUT]) {
     const int lane = static_cast<int>(__lane_id());
     const int j = lane & 15, g = lane >> 4;
+    // Packed BF16 words of the lane's own object, then per tile the words
+    // of column j (object 16 t + j).
+    uint32_t w[10];
+    net::input_words(x, w);
     net::u32x4 xin[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        float xt[net::NIN];
+        uint32_t wt[10];
 #pragma unroll
-        for (int k = 0; k < net::NIN; ++k) xt[k] = __shfl(x[k], 16 * t + j, WAVE);
-        xin[t] = net::input_operand(xt, g);
+        for (int k = 0; k < 10; ++k) wt[k] = static_cast<uint32_t>(__shfl(static_cast<int>(w[k]), 16 * t + j, WAVE));
+        xin[t] = net::input_operand_words(wt, g);
     }
     // Two passes of two 16-object tiles: the hidden activations of four tiles
     // at once would hold one wave per SIMD (over 400 registers); per tile the

@@ -120,17 +120,39 @@ __device__ __forceinline__ u32x4 pack8(uint32_t h0, uint32_t h1, uint32_t h2, ui
     return r;
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to BF16 (nearest even, as bf16_rne for finite
+// values, FP32 subnormals kept) in one v_cvt_pk_bf16_f32: a in the low half.
+__device__ __forceinline__ uint32_t cvt2(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+// Low BF16 parts of a and b, given w = cvt2(a, b): the rounding residues
+// a - hi(a), b - hi(b) are exact in FP32.
+__device__ __forceinline__ uint32_t cvt2_lo(float a, float b, uint32_t w) {
+    return cvt2(a - bits_f32(w << 16), b - bits_f32(w & 0xffff0000u));
+}
+
 // B operand of K step ks from output tiles 2ks (a) and 2ks+1 (b) of the
 // previous layer, rounded to BF16.
 __device__ __forceinline__ u32x4 pack_tiles(f32x4 a, f32x4 b) {
-    return pack8(bf16_rne(a.x), bf16_rne(a.y), bf16_rne(a.z), bf16_rne(a.w), bf16_rne(b.x), bf16_rne(b.y),
-                 bf16_rne(b.z), bf16_rne(b.w));
+    u32x4 r;
+    r.x = cvt2(a.x, a.y);
+    r.y = cvt2(a.z, a.w);
+    r.z = cvt2(b.x, b.y);
+    r.w = cvt2(b.z, b.w);
+    return r;
 }
-// Low BF16 parts of the same values.
-__device__ __forceinline__ u32x4 pack_tiles_lo(f32x4 a, f32x4 b) {
-    return pack8(bf16_lo(a.x, bf16_rne(a.x)), bf16_lo(a.y, bf16_rne(a.y)), bf16_lo(a.z, bf16_rne(a.z)),
-                 bf16_lo(a.w, bf16_rne(a.w)), bf16_lo(b.x, bf16_rne(b.x)), bf16_lo(b.y, bf16_rne(b.y)),
-                 bf16_lo(b.z, bf16_rne(b.z)), bf16_lo(b.w, bf16_rne(b.w)));
+// Low BF16 parts of the same values, given hi = pack_tiles(a, b).
+__device__ __forceinline__ u32x4 pack_tiles_lo(f32x4 a, f32x4 b, u32x4 hi) {
+    u32x4 r;
+    r.x = cvt2_lo(a.x, a.y, hi.x);
+    r.y = cvt2_lo(a.z, a.w, hi.y);
+    r.z = cvt2_lo(b.x, b.y, hi.z);
+    r.w = cvt2_lo(b.z, b.w, hi.w);
+    return r;
 }
 
 __device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
@@ -251,18 +273,37 @@ __device__ __forceinline__ void to_frame(const Frame& fr, D3 dr, D3 dv, double o
 
 // Input-layer B operand for lane group g from the standardised inputs x of
 // the lane's column: the extended vector u = (x_hi, x_lo, x_hi, 0, 0)
-// pairs with the weight row (W_hi, W_hi, W_lo, 0, 0).
-__device__ __forceinline__ uint32_t u_elem(const float x[NIN], int k) {
-    if (k < 10) return bf16_rne(x[k]);
-    if (k < 20) return bf16_lo(x[k - 10], bf16_rne(x[k - 10]));
-    if (k < 30) return bf16_rne(x[k - 20]);
-    return 0u;
+// pairs with the weight row (W_hi, W_hi, W_lo, 0, 0): K rows 0-9 x_hi,
+// 10-19 x_lo, 20-29 x_hi, 30-31 zero. It is built from packed words of the
+// column's inputs (input_words): w[i] = the BF16 pair (x_2i, x_2i+1),
+// w[5 + i] their low parts. With kmap(0, g, e) the lane group g holds K rows
+// 4g..4g+3 and 16+4g..16+4g+3 of u, i.e. the pairs
+//   g = 0: hi(0,1) hi(2,3) lo(6,7) lo(8,9)     g = 1: hi(4,5) hi(6,7) hi(0,1) hi(2,3)
+//   g = 2: hi(8,9) lo(0,1) hi(4,5) hi(6,7)     g = 3: lo(2,3) lo(4,5) hi(8,9) 0
+// selected by g (uniform per 16 lanes) instead of indexing x by a lane-
+// dependent K row (compare-and-select chains over all ten inputs).
+__device__ __forceinline__ void input_words(const float x[NIN], uint32_t w[10]) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        w[i] = cvt2(x[2 * i], x[2 * i + 1]);
+        w[5 + i] = cvt2_lo(x[2 * i], x[2 * i + 1], w[i]);
+    }
+}
+__device__ __forceinline__ uint32_t sel4(int g, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    return g == 0 ? a : (g == 1 ? b : (g == 2 ? c : d));
+}
+__device__ __forceinline__ u32x4 input_operand_words(const uint32_t w[10], int g) {
+    u32x4 r;
+    r.x = sel4(g, w[0], w[2], w[4], w[6]);
+    r.y = sel4(g, w[1], w[3], w[5], w[7]);
+    r.z = sel4(g, w[8], w[0], w[2], w[4]);
+    r.w = sel4(g, w[9], w[1], w[3], 0u);
+    return r;
 }
 __device__ __forceinline__ u32x4 input_operand(const float x[NIN], int g) {
-    uint32_t h[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) h[e] = u_elem(x, kmap(0, g, e));
-    return pack8(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7]);
+    uint32_t w[10];
+    input_words(x, w);
+    return input_operand_words(w, g);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,7 +378,7 @@ __device__ __forceinline__ void forward_tiles(const u32x4 xin[NT], const uint16_
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
                     const f32x4 h0 = silu4(z0[t]), h1 = silu4(z1[t]);
-                    const u32x4 hi = pack_tiles(h0, h1), lo = pack_tiles_lo(h0, h1);
+                    const u32x4 hi = pack_tiles(h0, h1), lo = pack_tiles_lo(h0, h1, hi);
                     out[t] = mfma(ah, hi, out[t]);
                     out[t] = mfma(ah, lo, out[t]);
                     out[t] = mfma(al, hi, out[t]);
@@ -480,7 +521,7 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
     for (int p = 0; p < 4; ++p) {
         const f32x4 h0 = silu4(z3[2 * p]), h1 = silu4(z3[2 * p + 1]);
         hb3[p] = pack_tiles(h0, h1);
-        hb3lo[p] = pack_tiles_lo(h0, h1);
+        hb3lo[p] = pack_tiles_lo(h0, h1, hb3[p]);
     }
     f32x4 o;
     o.x = o.y = o.z = o.w = 0.0f;
