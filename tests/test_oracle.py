@@ -320,21 +320,25 @@ def _lim_max2(base, gdt3, dt, vmax, cell):
     return np.float32(lim * lim * 1.001)
 
 
-@pytest.mark.parametrize("dt,cell,vmax,thr", [(10.0, 207.5, 10.27, 2.0), (20.0, 207.5, 10.27, 2.025),
-                                              (10.0, 223.0, 11.0, 5.0), (60.0, 900.0, 8.0, 2.0)])
-def test_prefilter_lim_max_bounds_every_candidate(dt, cell, vmax, thr):
+@pytest.mark.parametrize("dt,cell,vmax,thr,zsub", [(10.0, 207.5, 10.27, 2.0, 1), (20.0, 207.5, 10.27, 2.025, 1),
+                                                   (10.0, 223.0, 11.0, 5.0, 1), (60.0, 900.0, 8.0, 2.0, 1),
+                                                   (20.0, 207.5, 10.27, 2.025, 2), (60.0, 900.0, 8.0, 2.0, 4)])
+def test_prefilter_lim_max_bounds_every_candidate(dt, cell, vmax, thr, zsub):
     """The pair kernels reject a candidate at once when d^2 > lim_max2; that
     is the full test's decision only if lim^2 <= lim_max2 for every candidate
     a screening can produce: partners in a neighbouring cell (offset in
-    {-1, 0, 1} per axis, both positions anywhere in their cells), speeds up
-    to vmax. Random candidates plus the extreme corners and opposite
-    diagonal velocities, lim in float32 as the device computes it."""
+    {-1, 0, 1} per axis; along z, with cells split into zsub, in
+    {-zsub .. zsub} cells of cell / zsub), both positions anywhere in their
+    cells, speeds up to vmax. Random candidates plus the extreme corners and
+    opposite diagonal velocities, lim in float32 as the device computes it."""
     f = np.float32
     rng = np.random.default_rng(11)
     n = 200000
-    la = rng.uniform(0, cell, (n, 3))
-    lb = rng.uniform(0, cell, (n, 3))
-    off = rng.integers(-1, 2, (n, 3))
+    edge = np.array([cell, cell, cell / zsub])
+    reach = np.array([1, 1, zsub])
+    la = rng.uniform(0, 1, (n, 3)) * edge
+    lb = rng.uniform(0, 1, (n, 3)) * edge
+    off = rng.integers(-reach, reach + 1, (n, 3))
     ua = rng.normal(size=(n, 3))
     ub = rng.normal(size=(n, 3))
     ua *= (vmax * rng.uniform(0, 1, n) ** 0.2 / np.linalg.norm(ua, axis=1))[:, None]
@@ -342,12 +346,12 @@ def test_prefilter_lim_max_bounds_every_candidate(dt, cell, vmax, thr):
     # extreme corners: |dr| = 2 cells per axis, opposite velocities along a diagonal
     k = n // 10
     sgn = rng.choice([-1.0, 1.0], (k, 3))
-    la[:k] = np.where(sgn > 0, 0.0, cell)
-    lb[:k] = np.where(sgn > 0, cell, 0.0)
-    off[:k] = sgn.astype(int)
+    la[:k] = np.where(sgn > 0, 0.0, edge)
+    lb[:k] = np.where(sgn > 0, edge, 0.0)
+    off[:k] = (sgn * reach).astype(int)
     ua[:k] = -sgn * vmax / np.sqrt(3.0)
     ub[:k] = sgn * vmax / np.sqrt(3.0)
-    r = (off * cell + lb - la).astype(f)
+    r = (off * edge + lb - la).astype(f)
     v = (ub - ua).astype(f)
     base = f(thr) + f(1e-3) + f(5e-4) + f(5e-6) * f(dt)
     gdt3 = f(2e-6) * f(dt) * f(dt) * f(dt)
