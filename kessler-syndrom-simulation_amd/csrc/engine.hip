@@ -24,8 +24,9 @@
 //                 per-tile digit histograms, per-digit scans, then a scatter
 //                 whose in-tile ranks come from wave64 ballots (match on the
 //                 9 digit bits).
-//   k_cell_count / k_cell_scan / k_cell_write
-//                 cell boundaries by a wavefront scan of the head flags;
+//   k_cell_count / k_cell_write
+//                 cell boundaries by a wavefront scan of the head flags
+//                 (per-tile counts, summed by each tile of k_cell_write);
 //                 writes the cell table and the compact FP32 copy of each
 //                 sorted object (cell-relative position, velocity, index).
 //   k_neighbours  per cell: population and the 13 forward neighbours
@@ -928,20 +929,6 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_count(const uint64_t* k0, const
     if (threadIdx.x == 0) bcount[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, GridState* gs) {
-    __shared__ int s_wave[WAVES_PER_BLOCK];
-    int carry = 0;
-    for (int b0 = 0; b0 < ntiles; b0 += BLOCK) {
-        const int b = b0 + threadIdx.x;
-        const int v = b < ntiles ? bcount[b] : 0;
-        int tot;
-        const int ex = block_exclusive_scan(v, tot, s_wave);
-        if (b < ntiles) bcount[b] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) gs->ncells = static_cast<unsigned int>(carry);
-}
-
 // Cell table and compact records in sorted order. Element e = tile +
 // r * TBLOCK + thread (r < TITEMS), so every load and store is coalesced;
 // cells are numbered from wave ballots of the head flags and one block-wide
@@ -953,15 +940,28 @@ __global__ void __launch_bounds__(BLOCK) k_cell_scan(int* bcount, int ntiles, Gr
 //                 from which pairs get their z cell offset.
 __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
-                                                       const GridState* gs, const int* boff,
+                                                       GridState* gs, const int* bcount, int ntiles,
                                                        const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lrec, int4* __restrict__ coltab) {
     __shared__ int s_seg[TITEMS * TWAVES];
+    __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.alive);
     const int base = blockIdx.x * TILE;
     if (base >= A || g.npasses == 0) return;
+    // Cells of the earlier tiles: the sum of their head counts (k_cell_count),
+    // a few loads per thread; block 0 sums every tile and writes the total.
+    int run;
+    {
+        const int upto = blockIdx.x == 0 ? ntiles : static_cast<int>(blockIdx.x);
+        int part = 0;
+        for (int k = threadIdx.x; k < upto; k += TBLOCK) part += bcount[k];
+        int tot;
+        block_exclusive_scan<TWAVES>(part, tot, s_wave);
+        run = blockIdx.x == 0 ? 0 : tot;
+        if (blockIdx.x == 0 && threadIdx.x == 0) gs->ncells = static_cast<unsigned int>(tot);
+    }
     const uint64_t* keys = sorted_keys(g, k0, k1);
     const int* sidx = sorted_idx(g, i0, i1);
     const int lane = static_cast<int>(__lane_id());
@@ -989,7 +989,6 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
         }
     }
     __syncthreads();
-    int run = boff[blockIdx.x];
     int pre[TITEMS];
 #pragma unroll
     for (int r = 0; r < TITEMS; ++r) {
@@ -2553,9 +2552,8 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
                                           tiles);
     }
     k_cell_count<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
-    k_cell_scan<<<1, BLOCK, 0, s>>>(ctx->bcount, tiles, ctx->gs);
     k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
-                                         ctx->gs, ctx->bcount, ctx->rec, ctx->cell_key,
+                                         ctx->gs, ctx->bcount, tiles, ctx->rec, ctx->cell_key,
                                          ctx->cell_begin, ctx->cell_of, ctx->lrec, ctx->coltab);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
