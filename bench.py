@@ -79,10 +79,12 @@ def cascade_population(kamd):
 
 
 def run_gpu(kamd, pop, steps, warmup):
-    """Plain FP64 run of `pop`: the tiling is autotuned on a few steps first
-    (results do not depend on it), then the timed run carries no per-kernel
-    event marks; per-kernel device times come from the same run repeated
-    from the same state with marks."""
+    """Plain FP64 run of `pop`: the warm-up steps run first, then the tiling
+    is autotuned on a few steps of the warmed state (results do not depend
+    on it; at t = 0 every breakup's fragments still share one point, which
+    is no guide to the rest of the day), then the timed run from t = 0
+    carries no per-kernel event marks; per-kernel device times come from
+    the same run repeated from the same state with marks."""
     eng = kamd.Engine(0)
     alive = pop["alive"].astype(np.uint8)
 
@@ -92,12 +94,11 @@ def run_gpu(kamd, pop, steps, warmup):
         eng.set_time(EPOCH, 0.0, DT)
 
     reset()
-    tune = eng.autotune(steps=4)
     if warmup > 0:
-        reset()
         eng.run(warmup, screen=True)
         eng.take_events()
         eng.take_stats()
+    tune = eng.autotune(steps=4)
     reset()
     t0 = time.perf_counter()
     eng.run(steps, screen=True)
@@ -447,14 +448,24 @@ def ensemble_workload(kamd, args):
     eng = kamd.Engine(0)
     train = train_corrector(eng, kamd, args.train_iters)
 
-    def run(precision, dt, nsteps, piece):
+    def setup(precision, dt):
         eng.set_precision(precision)
         eng.set_graph(precision == kamd.PRECISION_LEARNED)
         eng.set_objects(pop["state"], alive0, pop["size_m"])
         eng.set_scenarios(off)
         eng.set_config(thr, 1.0, 0.01)
         eng.set_time(EPOCH, 0.0, dt)
+
+    def run(precision, dt, nsteps, piece, warm):
+        # Untimed warm-up of 10 simulated minutes, then the tiling is tuned
+        # on that state (at t = 0 every breakup's fragments still share one
+        # point); the timed run starts again from t = 0.
+        setup(precision, dt)
+        eng.run(warm, screen=True)
+        eng.take_stats()
+        eng.take_events()
         tune = eng.autotune(steps=4)
+        setup(precision, dt)
         conj = np.zeros(64, np.int64)
         coll = np.zeros(64, np.int64)
         objs, done, wall = 0, 0, 0.0
@@ -473,16 +484,16 @@ def ensemble_workload(kamd, args):
         s, a = eng.get_objects()
         return dict(wall=wall, conj=conj, coll=coll, alive=a, tune=tune)
 
-    lrn = run(kamd.PRECISION_LEARNED, 60.0, steps, 120)
+    lrn = run(kamd.PRECISION_LEARNED, 60.0, steps, 120, 10)
     captures, launches = eng.graph_counts()
-    plain = run(kamd.PRECISION_FP64, DT, 6 * steps, 720)
+    plain = run(kamd.PRECISION_FP64, DT, 6 * steps, 720, 60)
     # object-steps in 10 s steps: the alive population times the plain steps
     # (the population changes only by re-entries, which the two forms share).
     n_alive = int(alive0.sum())
     obj = n_alive * 6 * steps
     v_l, v_p = obj / lrn["wall"], obj / plain["wall"]
     out = {
-        "metric": METRIC, "value": v_l, "unit": "object-steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "metric": METRIC, "value": v_l, "unit": "object-steps/s", "n_gpus": 1, "steps": steps, "warmup": 10,
         "ms_per_step": 1e3 * lrn["wall"] / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 corrector + f64 state + f32 perturbations", "data": "synthetic",
         "config": {"workload": (f"ensemble: 64 cascade scenarios, {len(pop):,} objects concatenated "
