@@ -1,10 +1,12 @@
-"""Full-scale parity check of the cascade configuration (not collected by
-pytest: about three minutes of CPU on 16 threads).
+"""Full-scale parity check of the plain FP64 path (not collected by pytest:
+three minutes of CPU on 16 threads for the cascade, about fourteen for the
+1M-object catalogue).
 
-The cascade population (the 28,340-object synthetic catalogue of seed 0
-plus 40 seeded catastrophic breakups at t = 0: 255,220 objects) is stepped
-for 24 h at dt = 10 s with screening at 5 km (radius scale 1, v_min
-10 m/s) twice: by the engine's plain FP64 path on the GPU, and by the
+Populations: `cascade`, the 28,340-object synthetic catalogue of seed 0
+plus 40 seeded catastrophic breakups at t = 0 (255,220 objects), screened
+at 5 km; `million`, 1,000,000 synthetic objects (seed 0, LEO shells x1.5),
+screened at 2 km. Either is stepped for 24 h at dt = 10 s (radius scale 1,
+v_min 10 m/s) twice: by the engine's plain FP64 path on the GPU, and by the
 reference's own compiled CPU path (oracle/_ref, the main.cpp step loop:
 Simulation::step + Screener). Both runs go in ten-minute chunks, the reference
 chained through its returned states with t_start = chunk start (its CPU
@@ -13,7 +15,7 @@ field and bit for bit (pair, step, TCA, miss distance, relative speed,
 collision flag, in (step, t, i, j) order), the final states bit for bit and
 the alive flags. Prints one JSON line.
 
-    python tests/check_cascade_24h.py [--hours H]
+    python tests/check_full_scale.py [--population cascade|million] [--hours H]
 """
 import argparse
 import json
@@ -31,7 +33,6 @@ from oracle.bind import reference  # noqa: E402
 
 EPOCH = 2461253.916666667
 DT = 10.0
-THR = 5.0
 CHUNK = 60  # steps per chunk (ten minutes)
 FIELDS = ("i", "j", "step", "t", "miss_km", "v_rel_km_s", "collision")
 
@@ -43,18 +44,22 @@ def ordered(ev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--hours", type=int, default=24)
+    ap.add_argument("--population", choices=("cascade", "million"), default="cascade")
     args = ap.parse_args()
     nchunks = args.hours * 3600 // int(CHUNK * DT)
     nsteps = nchunks * CHUNK
     kamd = load_package()
-    pop = kamd.cascade(28340, 0, 40, 0.045)
+    if args.population == "cascade":
+        pop, thr = kamd.cascade(28340, 0, 40, 0.045), 5.0
+    else:
+        pop, thr = kamd.generate(1_000_000, seed=0, mix=(0.7, 0.1, 0.1, 0.1, 1.5)), 2.0
     alive0 = pop["alive"].astype(np.uint8)
     print(f"{len(pop)} objects, {nsteps} steps", flush=True)
 
     eng = kamd.Engine(0)
     eng.set_precision(0)
     eng.set_objects(pop["state"], alive0, pop["size_m"])
-    eng.set_config(THR, 1.0, 0.01)
+    eng.set_config(thr, 1.0, 0.01)
     eng.set_time(EPOCH, 0.0, DT)
     t1 = time.perf_counter()
     gev = []
@@ -75,7 +80,7 @@ def main():
     states, alive = pop["state"].copy(), alive0.copy()
     rev, cpu_s = [], 0.0
     for c in range(nchunks):
-        states, alive, ev, _, loop_s = ref.run(states, alive, pop["size_m"], EPOCH, DT, CHUNK, screen=True, thr=THR,
+        states, alive, ev, _, loop_s = ref.run(states, alive, pop["size_m"], EPOCH, DT, CHUNK, screen=True, thr=thr,
                                                t_start=c * CHUNK * DT, cap=1 << 23, per_step=False)
         ev = ev.copy()
         ev["step"] += c * CHUNK
@@ -87,7 +92,8 @@ def main():
     same_len = len(gev) == len(rev)
     fields = {f: bool(same_len and np.array_equal(gev[f], rev[f])) for f in FIELDS}
     out = {
-        "objects": int(len(pop)), "steps": nsteps, "events_engine": int(len(gev)), "events_reference": int(len(rev)),
+        "population": args.population, "objects": int(len(pop)), "threshold_km": thr, "steps": nsteps,
+        "events_engine": int(len(gev)), "events_reference": int(len(rev)),
         "collisions": int(gev["collision"].sum()), "fields_equal": fields,
         "states_equal": bool(np.array_equal(gs, states)), "alive_equal": bool(np.array_equal(ga, alive)),
         "engine_s": gpu_s, "reference_loop_s": cpu_s, "reference_threads": cores,
