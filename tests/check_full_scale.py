@@ -95,9 +95,14 @@ def main():
         "population": args.population, "objects": int(len(pop)), "threshold_km": thr, "steps": nsteps,
         "events_engine": int(len(gev)), "events_reference": int(len(rev)),
         "collisions": int(gev["collision"].sum()), "fields_equal": fields,
-        "states_equal": bool(np.array_equal(gs, states)), "alive_equal": bool(np.array_equal(ga, alive)),
+        "states_equal": bool(np.array_equal(gs, states, equal_nan=True)),
+        "alive_equal": bool(np.array_equal(ga, alive)),
         "engine_s": gpu_s, "reference_loop_s": cpu_s, "reference_threads": cores,
     }
+    # Rows that differ (NaN in the same place counts as equal), by alive flag.
+    diff = ~((gs == states) | (np.isnan(gs) & np.isnan(states))).all(axis=1)
+    out["rows_differing"] = {"alive": int((diff & (alive == 1)).sum()), "dead": int((diff & (alive == 0)).sum())}
+    out["rows_with_nan"] = {"engine": int(np.isnan(gs).any(axis=1).sum()), "reference": int(np.isnan(states).any(axis=1).sum())}
     print(json.dumps(out), flush=True)
     ok = same_len and all(fields.values()) and out["states_equal"] and out["alive_equal"]
     raise SystemExit(0 if ok else 1)

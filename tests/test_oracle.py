@@ -439,3 +439,20 @@ def test_prefilter_early_rejection_implies_d2_rejection(lim_max2, dt):
         assert not bad.any(), (rcp_err, np.flatnonzero(bad)[:5])
     planted = _early_reject(r, v, lm, 1.0) & ~(_clamped_d2(r, v, dt, 0.0) > lm)
     assert planted.any()
+
+
+def test_engine_moon_matches_oracle_over_a_day(kamd, orc):
+    """The engine's host Moon series (its per-step samples go to the device)
+    equals the oracle's, which the reference's golden vectors pin, bit for
+    bit at every step start and midpoint of a day at 10 s. A build of the
+    host sources that changes libm calls (a compiler merging sin / cos
+    pairs) differs in the last bit at a few of these times, enough to move a
+    near-equatorial GEO object's tiny z velocity by one ulp."""
+    epoch = 2461253.916666667
+    bad = []
+    for step in range(8640):
+        for t in (step * 10.0, step * 10.0 + 5.0):
+            jd = epoch + t / 86400.0
+            if not np.array_equal(kamd.moon_position(jd), orc.moon_position(jd)):
+                bad.append(t)
+    assert not bad, bad[:5]
