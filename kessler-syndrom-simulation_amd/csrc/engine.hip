@@ -1767,17 +1767,36 @@ __device__ __forceinline__ int plain_refine(D3 ri, D3 vi, D3 rj, D3 vj, D3 ai, D
     return 0;
 }
 
-// States of one object at the start of sub-steps k and k + 1.
-__device__ __forceinline__ void plain_at(Planes s0, int i, const MoonSample* m, double h, int sub, int k, D3& ra,
-                                         D3& va, D3& rb, D3& vb) {
+// States of objects i and j at the start of sub-steps k and k + 1. Both are
+// stepped in the same loop body, so their two independent RK4 chains
+// interleave (a lane's work here is one long dependent chain per object,
+// and few waves run).
+__device__ __forceinline__ void plain_at2(Planes s0, int i, int j, const MoonSample* m, double h, int sub, int k,
+                                          D3& rai, D3& vai, D3& rbi, D3& vbi, D3& raj, D3& vaj, D3& rbj,
+                                          D3& vbj) {
     const int stride = 12 / sub;
-    D3 r{s0.x[i], s0.y[i], s0.z[i]}, v{s0.vx[i], s0.vy[i], s0.vz[i]};
-    for (int q = 0; q < k; ++q) rk4(r, v, h, m[stride * q], m[stride * q + stride / 2], m[stride * (q + 1)]);
-    ra = r;
-    va = v;
-    rk4(r, v, h, m[stride * k], m[stride * k + stride / 2], m[stride * (k + 1)]);
-    rb = r;
-    vb = v;
+    D3 ri{s0.x[i], s0.y[i], s0.z[i]}, vi{s0.vx[i], s0.vy[i], s0.vz[i]};
+    D3 rj{s0.x[j], s0.y[j], s0.z[j]}, vj{s0.vx[j], s0.vy[j], s0.vz[j]};
+    for (int q = 0; q < k; ++q) {
+        const MoonSample& m0 = m[stride * q];
+        const MoonSample& m1 = m[stride * q + stride / 2];
+        const MoonSample& m2 = m[stride * (q + 1)];
+        rk4(ri, vi, h, m0, m1, m2);
+        rk4(rj, vj, h, m0, m1, m2);
+    }
+    rai = ri;
+    vai = vi;
+    raj = rj;
+    vaj = vj;
+    const MoonSample& m0 = m[stride * k];
+    const MoonSample& m1 = m[stride * k + stride / 2];
+    const MoonSample& m2 = m[stride * (k + 1)];
+    rk4(ri, vi, h, m0, m1, m2);
+    rk4(rj, vj, h, m0, m1, m2);
+    rbi = ri;
+    vbi = vi;
+    rbj = rj;
+    vbj = vj;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* events, unsigned long long cap,
@@ -1799,8 +1818,7 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
         double s = 0.0, miss = 0.0, vrel = 0.0;
         while (tries < 2 && k >= 0 && k < sub) {
             D3 rai, vai, rbi, vbi, raj, vaj, rbj, vbj;
-            plain_at(s0, i, moon13, h, sub, k, rai, vai, rbi, vbi);
-            plain_at(s0, j, moon13, h, sub, k, raj, vaj, rbj, vbj);
+            plain_at2(s0, i, j, moon13, h, sub, k, rai, vai, rbi, vbi, raj, vaj, rbj, vbj);
             code = plain_refine(rai, vai, raj, vaj, rbi, vbi, rbj, vbj, h, thr, vmin2, s, miss, vrel);
             ++tries;
             if (code == 2 && dir <= 0) {
