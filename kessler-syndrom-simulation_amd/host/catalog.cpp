@@ -7,10 +7,13 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <random>
 #include <sstream>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "../../include/kessler_host.h"
 #include "breakup.hpp"
@@ -245,29 +248,76 @@ int khost_write_catalog(const char* path, int64_t n, const khost_object* objs, c
     std::fprintf(f, "# Colonnes : NORAD_ID | NOM | OBJECT_TYPE | INTL_DESIGNATOR | COUNTRY | RCS_SIZE | "
                     "RCS_VALUE_m2 | TLE_SOURCE_EPOCH | LAT_deg | LON_deg | ALT_km | X_km | Y_km | Z_km | "
                     "VX_km_s | VY_km_s | VZ_km_s | size_m\n");
-    for (int64_t i = 0; i < n; ++i) {
-        const khost_object& o = objs[i];
+    // Lines are formatted in parallel, 4,096 per chunk, and written in order.
+    auto format = [&](const khost_object& o, char* buf, size_t cap) {
         const double* s = o.state;
         const double alt = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]) - R_EARTH;
-        std::fprintf(f, "%d | %s %d | %s | - | - | %s | 0 | - | 0 | 0 | %.3f | %.17g | %.17g | %.17g | "
-                        "%.17g | %.17g | %.17g | size_m=%.17g\n",
-                     o.id, o.event >= 0 ? "FRAG" : "OBJ", o.id,
-                     object_type_name(static_cast<ObjectType>(o.type)), rcs_name(o.size_m), alt, s[0],
-                     s[1], s[2], s[3], s[4], s[5], o.size_m);
+        return std::snprintf(buf, cap,
+                             "%d | %s %d | %s | - | - | %s | 0 | - | 0 | 0 | %.3f | %.17g | %.17g | %.17g | "
+                             "%.17g | %.17g | %.17g | size_m=%.17g\n",
+                             o.id, o.event >= 0 ? "FRAG" : "OBJ", o.id,
+                             object_type_name(static_cast<ObjectType>(o.type)), rcs_name(o.size_m), alt, s[0],
+                             s[1], s[2], s[3], s[4], s[5], o.size_m);
+    };
+    constexpr int64_t CHUNK_LINES = 4096;
+    const int64_t nchunks = (n + CHUNK_LINES - 1) / CHUNK_LINES;
+    std::vector<std::string> text(static_cast<size_t>(nchunks > 0 ? nchunks : 0));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t c = 0; c < nchunks; ++c) {
+        std::string& out = text[static_cast<size_t>(c)];
+        out.reserve(static_cast<size_t>(CHUNK_LINES) * 240);
+        char line[512];
+        const int64_t end = std::min(n, (c + 1) * CHUNK_LINES);
+        for (int64_t i = c * CHUNK_LINES; i < end; ++i) {
+            const int len = format(objs[i], line, sizeof line);
+            if (len < 0) continue;
+            if (static_cast<size_t>(len) < sizeof line) {
+                out.append(line, static_cast<size_t>(len));
+            } else {
+                std::string big(static_cast<size_t>(len) + 1, '\0');
+                format(objs[i], &big[0], big.size());
+                out.append(big.data(), static_cast<size_t>(len));
+            }
+        }
     }
-    return std::fclose(f) == 0 ? 0 : -3;
+    bool ok = true;
+    for (const std::string& t : text) ok = ok && std::fwrite(t.data(), 1, t.size(), f) == t.size();
+    return std::fclose(f) == 0 && ok ? 0 : -3;
 }
 
 int64_t khost_load_catalog(const char* path, int64_t cap, khost_object* out, double* epoch_jd,
                            int* dropped) {
-    int bad = 0;
-    std::vector<Object> objs = load_catalog(path, &bad);
-    if (dropped) *dropped = bad;
-    if (epoch_jd) *epoch_jd = read_epoch_jd(path);
-    const MassTable masses;
-    assign_masses(objs, masses);
-    const int64_t n = static_cast<int64_t>(objs.size());
-    for (int64_t i = 0; i < n && i < cap && out; ++i) out[i] = from_object(objs[i]);
+    // The two-call protocol (count, then fill) parses the file once: the
+    // last result is kept while the file's size and modification time are
+    // unchanged.
+    static std::mutex mu;
+    static std::string last_path;
+    static off_t last_size = -1;
+    static struct timespec last_mtime {};
+    static std::vector<khost_object> last;
+    static double last_epoch = 0.0;
+    static int last_bad = 0;
+    struct stat st {};
+    const bool have = ::stat(path, &st) == 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!(have && last_path == path && last_size == st.st_size && last_mtime.tv_sec == st.st_mtim.tv_sec &&
+          last_mtime.tv_nsec == st.st_mtim.tv_nsec)) {
+        int bad = 0;
+        std::vector<Object> objs = load_catalog(path, &bad);
+        const MassTable masses;
+        assign_masses(objs, masses);
+        last.resize(objs.size());
+        for (size_t i = 0; i < objs.size(); ++i) last[i] = from_object(objs[i]);
+        last_epoch = read_epoch_jd(path);
+        last_bad = bad;
+        last_path = have ? path : std::string();
+        last_size = have ? st.st_size : -1;
+        last_mtime = have ? st.st_mtim : timespec{};
+    }
+    if (dropped) *dropped = last_bad;
+    if (epoch_jd) *epoch_jd = last_epoch;
+    const int64_t n = static_cast<int64_t>(last.size());
+    for (int64_t i = 0; i < n && i < cap && out; ++i) out[i] = last[static_cast<size_t>(i)];
     return n;
 }
 

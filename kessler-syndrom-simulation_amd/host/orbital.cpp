@@ -9,7 +9,9 @@
 #include "orbital.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <iostream>
 #include <sstream>
@@ -122,48 +124,106 @@ Elements elements(const State& s) {
     return {a, e, std::acos(h.z / norm(h)) / DEG, a * (1 - e) - R_EARTH, a * (1 + e) - R_EARTH, period};
 }
 
+namespace {
+
+// One catalogue line: 0 skipped (empty or comment), 1 object, 2 malformed.
+struct ParsedLine {
+    int status = 0;
+    Object o;
+};
+
+// The fields of one line split as std::getline(stream, token, '|') splits
+// them (no empty token after a final '|'), each stripped of blanks.
+void parse_line(const char* b, const char* e, ParsedLine& out) {
+    const std::string line(b, e);
+    if (line.empty() || line[0] == '#') {
+        out.status = 0;
+        return;
+    }
+    std::vector<std::string> field;
+    field.reserve(20);
+    size_t start = 0;
+    while (start < line.size()) {
+        const size_t bar = line.find('|', start);
+        const size_t stop = bar == std::string::npos ? line.size() : bar;
+        field.push_back(strip(line.substr(start, stop - start)));
+        if (bar == std::string::npos) break;
+        start = bar + 1;
+    }
+    if (field.size() < 17) {
+        out.status = 2;
+        return;
+    }
+    try {
+        Object& o = out.o;
+        o.id = std::stoi(field[0]);
+        o.name = field[1];
+        o.type = parse_object_type(field[2]);
+        o.country = field[4];
+        o.rcs = field[5];
+        // Size from the radar cross-section class: collision radii
+        // 0.1 / 0.4 / 2 m for SMALL / MEDIUM (and unknown) / LARGE.
+        o.size_m = o.rcs == "SMALL" ? 0.2 : (o.rcs == "LARGE" ? 4.0 : 0.8);
+        o.s.r = {std::stod(field[11]), std::stod(field[12]), std::stod(field[13])};
+        o.s.v = {std::stod(field[14]), std::stod(field[15]), std::stod(field[16])};
+        // Extension: an 18th field "size_m=<L>" (written for fragments,
+        // whose size is not an RCS class) overrides the class size.
+        if (field.size() > 17 && field[17].rfind("size_m=", 0) == 0) o.size_m = std::stod(field[17].substr(7));
+        out.status = 1;
+    } catch (...) {
+        out.status = 2;
+    }
+}
+
+}  // namespace
+
+// load_catalog (orbital.cpp:150-170): the whole file is read at once, lines
+// are parsed in parallel (each independently, as the reference parses them
+// one by one) and the objects kept in file order.
 std::vector<Object> load_catalog(const std::string& path, int* dropped) {
     std::vector<Object> objs;
-    std::ifstream in(path);
-    if (!in) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
         std::cerr << "cannot open " << path << "\n";
         if (dropped) *dropped = 0;
         return objs;
     }
-    int bad = 0;
-    std::string line;
-    std::vector<std::string> field;
-    while (std::getline(in, line)) {
-        if (line.empty() || line[0] == '#') continue;
-        field.clear();
-        std::stringstream ss(line);
-        std::string tok;
-        while (std::getline(ss, tok, '|')) field.push_back(strip(tok));
-        if (field.size() < 17) {
-            ++bad;
-            continue;
-        }
-        try {
-            Object o;
-            o.id = std::stoi(field[0]);
-            o.name = field[1];
-            o.type = parse_object_type(field[2]);
-            o.country = field[4];
-            o.rcs = field[5];
-            // Size from the radar cross-section class: collision radii
-            // 0.1 / 0.4 / 2 m for SMALL / MEDIUM (and unknown) / LARGE.
-            o.size_m = o.rcs == "SMALL" ? 0.2 : (o.rcs == "LARGE" ? 4.0 : 0.8);
-            o.s.r = {std::stod(field[11]), std::stod(field[12]), std::stod(field[13])};
-            o.s.v = {std::stod(field[14]), std::stod(field[15]), std::stod(field[16])};
-            // Extension: an 18th field "size_m=<L>" (written for fragments,
-            // whose size is not an RCS class) overrides the class size.
-            if (field.size() > 17 && field[17].rfind("size_m=", 0) == 0)
-                o.size_m = std::stod(field[17].substr(7));
-            objs.push_back(std::move(o));
-        } catch (...) {
-            ++bad;
-        }
+    std::string buf;
+    std::fseek(f, 0, SEEK_END);
+    const long size = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    if (size > 0) {
+        buf.resize(static_cast<size_t>(size));
+        buf.resize(std::fread(&buf[0], 1, buf.size(), f));
     }
+    std::fclose(f);
+    // Line boundaries ('\n' only, as std::getline); a last line without
+    // '\n' counts, an empty tail after a final '\n' does not.
+    std::vector<size_t> begin;
+    begin.reserve(buf.size() / 64 + 1);
+    for (size_t p = 0; p < buf.size();) {
+        begin.push_back(p);
+        const void* nl = std::memchr(buf.data() + p, '\n', buf.size() - p);
+        p = nl ? static_cast<size_t>(static_cast<const char*>(nl) - buf.data()) + 1 : buf.size();
+    }
+    const int64_t nlines = static_cast<int64_t>(begin.size());
+    std::vector<ParsedLine> parsed(static_cast<size_t>(nlines));
+#pragma omp parallel for schedule(dynamic, 2048)
+    for (int64_t k = 0; k < nlines; ++k) {
+        const size_t b = begin[static_cast<size_t>(k)];
+        size_t e = k + 1 < nlines ? begin[static_cast<size_t>(k + 1)] : buf.size();
+        if (e > b && buf[e - 1] == '\n') --e;
+        parse_line(buf.data() + b, buf.data() + e, parsed[static_cast<size_t>(k)]);
+    }
+    int bad = 0;
+    size_t keep = 0;
+    for (const ParsedLine& pl : parsed) {
+        keep += pl.status == 1 ? 1 : 0;
+        bad += pl.status == 2 ? 1 : 0;
+    }
+    objs.reserve(keep);
+    for (ParsedLine& pl : parsed)
+        if (pl.status == 1) objs.push_back(std::move(pl.o));
     if (dropped) *dropped = bad;
     return objs;
 }
