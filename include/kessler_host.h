@@ -83,6 +83,18 @@ int64_t khost_load_catalog(const char* path, int64_t cap, khost_object* out, dou
 
 void khost_moon_position(double jd, double* out3);
 
+/* write_snapshot (orbital.hpp:125, orbital.cpp:190-197): the snapshot CSV of
+ * objs at time t (s after the epoch); exact != 0 writes 17 significant
+ * digits instead of the reference's 9. 0 on success. */
+int khost_write_snapshot(const char* path, int64_t n, const khost_object* objs, double t, int exact);
+
+/* read_snapshot (orbital.hpp:133, orbital.cpp:199-249): replays `path` into
+ * objs[0..n), in and out. Returns the object count after the replay
+ * (fragments appended or dropped; the first `cap` are written back), or -1
+ * where the reference's function returns false (unreadable, or not matching
+ * the list). *t receives the snapshot time. */
+int64_t khost_read_snapshot(const char* path, int64_t n, int64_t cap, khost_object* objs, double* t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,7 +61,7 @@ PRECISION_MIXED = 1
 # Every symbol include/kessler_host.h declares.
 HOST_SYMBOLS = (
     "khost_generate", "khost_impacts", "khost_cascade", "khost_write_catalog",
-    "khost_load_catalog", "khost_moon_position",
+    "khost_load_catalog", "khost_moon_position", "khost_write_snapshot", "khost_read_snapshot",
 )
 
 _P = C.c_void_p
@@ -174,6 +174,8 @@ def library() -> C.CDLL:
         "khost_write_catalog": (_I, [C.c_char_p, _L, _P, C.c_char_p]),
         "khost_load_catalog": (_L, [C.c_char_p, _L, _P, C.POINTER(_D), C.POINTER(_I)]),
         "khost_moon_position": (None, [_D, _P]),
+        "khost_write_snapshot": (_I, [C.c_char_p, _L, _P, _D, _I]),
+        "khost_read_snapshot": (_L, [C.c_char_p, _L, _L, _P, C.POINTER(_D)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -506,6 +508,31 @@ def load_catalog(path: str):
     out = np.zeros(max(n, 0), dtype=OBJECT_DTYPE)
     lib.khost_load_catalog(path.encode(), n, _ptr(out), C.byref(ep), C.byref(dr))
     return out, ep.value, dr.value
+
+
+def write_snapshot(path: str, objs, t: float, exact: bool = False):
+    """write_snapshot / write_snapshot_exact of the host core (khost_write_snapshot)."""
+    objs = np.ascontiguousarray(objs, dtype=OBJECT_DTYPE)
+    rc = library().khost_write_snapshot(path.encode(), len(objs), _ptr(objs), float(t), 1 if exact else 0)
+    if rc != 0:
+        raise RuntimeError(f"khost_write_snapshot failed ({rc})")
+
+
+def read_snapshot(path: str, objs):
+    """read_snapshot of the host core (khost_read_snapshot): returns (objects
+    after the replay, t), or None where the snapshot does not match `objs`."""
+    objs = np.array(objs, dtype=OBJECT_DTYPE, copy=True)
+    t = _D()
+    cap = len(objs)
+    while True:
+        buf = np.zeros(max(cap, 1), dtype=OBJECT_DTYPE)
+        buf[: len(objs)] = objs
+        n = library().khost_read_snapshot(path.encode(), len(objs), cap, _ptr(buf), C.byref(t))
+        if n < 0:
+            return None
+        if n <= cap:
+            return buf[:n], t.value
+        cap = n
 
 
 def moon_position(jd: float) -> np.ndarray:

@@ -321,6 +321,29 @@ int64_t khost_load_catalog(const char* path, int64_t cap, khost_object* out, dou
     return n;
 }
 
+int khost_write_snapshot(const char* path, int64_t n, const khost_object* objs, double t, int exact) {
+    if (!path || n < 0 || (n > 0 && !objs)) return -3;
+    std::vector<Object> v;
+    v.reserve(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) v.push_back(to_object(objs[i]));
+    if (exact) write_snapshot_exact(path, v, t);
+    else write_snapshot(path, v, t);
+    return 0;
+}
+
+int64_t khost_read_snapshot(const char* path, int64_t n, int64_t cap, khost_object* objs, double* t) {
+    if (!path || n < 0 || (n > 0 && !objs)) return -3;
+    std::vector<Object> v;
+    v.reserve(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) v.push_back(to_object(objs[i]));
+    double ts = 0.0;
+    if (!read_snapshot(path, v, &ts)) return -1;
+    if (t) *t = ts;
+    const int64_t total = static_cast<int64_t>(v.size());
+    for (int64_t i = 0; i < total && i < cap && objs; ++i) objs[i] = from_object(v[static_cast<size_t>(i)]);
+    return total;
+}
+
 void khost_moon_position(double jd, double* out3) {
     const Vec3 m = moon_position(jd);
     out3[0] = m.x;

@@ -262,6 +262,73 @@ void write_snapshot_exact(const std::string& path, const std::vector<Object>& ob
     write_snapshot_impl(path, objs, t, 17);
 }
 
+// read_snapshot (orbital.hpp:133, orbital.cpp:199-249): comment lines carry
+// "t = <seconds>", the first other line names the columns, then one row per
+// object in the order written. A row must match the loaded object at its
+// position by id; rows past the list are fragments born during the run
+// (ids from FRAGMENT_ID_BASE), appended with generic properties; loaded
+// fragments past the last row are dropped. A catalogue object without a row,
+// a row out of order or a malformed row make the file not match (false).
+bool read_snapshot(const std::string& path, std::vector<Object>& objs, double* t) {
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string line;
+    size_t row = 0;
+    bool columns_seen = false;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        if (line[0] == '#') {
+            const size_t at = line.find("t =");
+            if (t && at != std::string::npos) *t = std::atof(line.c_str() + at + 3);
+            continue;
+        }
+        if (!columns_seen) {
+            columns_seen = true;
+            continue;
+        }
+        // Comma-separated fields, split as std::getline(stream, token, ',').
+        std::vector<std::string> f;
+        size_t start = 0;
+        while (start < line.size()) {
+            const size_t comma = line.find(',', start);
+            const size_t stop = comma == std::string::npos ? line.size() : comma;
+            f.push_back(line.substr(start, stop - start));
+            if (comma == std::string::npos) break;
+            start = comma + 1;
+        }
+        if (f.size() < 8) return false;
+        try {
+            const int id = std::stoi(f[0]);
+            if (row >= objs.size()) {
+                if (id < FRAGMENT_ID_BASE) return false;
+                Object o;
+                o.id = id;
+                o.name = "FRAG " + std::to_string(id - FRAGMENT_ID_BASE);
+                o.type = ObjectType::Debris;
+                o.rcs = "SMALL";
+                o.size_m = 0.1;
+                o.event = 0;
+                objs.push_back(std::move(o));
+            } else if (objs[row].id != id) {
+                return false;
+            }
+            Object& o = objs[row];
+            o.s.r = {std::stod(f[1]), std::stod(f[2]), std::stod(f[3])};
+            o.s.v = {std::stod(f[4]), std::stod(f[5]), std::stod(f[6])};
+            o.alive = std::stoi(f[7]) != 0;
+        } catch (...) {
+            return false;
+        }
+        ++row;
+    }
+    if (row < objs.size()) {
+        for (size_t k = row; k < objs.size(); ++k)
+            if (!objs[k].is_fragment()) return false;
+        objs.resize(row);
+    }
+    return true;
+}
+
 std::string jd_to_utc_string(double jd) {
     const double z = std::floor(jd + 0.5);
     const double frac = jd + 0.5 - z;

@@ -88,6 +88,9 @@ double read_epoch_jd(const std::string& path);
 void write_snapshot(const std::string& path, const std::vector<Object>& objs, double t);
 // Full-precision variant (%.17g): exact round trip of every double.
 void write_snapshot_exact(const std::string& path, const std::vector<Object>& objs, double t);
+// Replays a snapshot into `objs` (orbital.hpp:133); false if unreadable or not
+// matching the loaded list. `t` receives the snapshot time (s).
+bool read_snapshot(const std::string& path, std::vector<Object>& objs, double* t = nullptr);
 std::string jd_to_utc_string(double jd);
 
 class DeviceEngine;

@@ -163,6 +163,33 @@ class Reference(Checker):
         f = self._f
         f("rng", None, [C.c_uint64, _I, _D, _L, _P])
         f("impacts", _L, [_I, _P, _P, _P, _D, _D, _I, _P, _P, _P, C.c_uint64, _D, _L, _P, _P])
+        f("write_snapshot", None, [C.c_char_p, _I, _P, _P, _P, _D])
+        f("read_snapshot", _L, [C.c_char_p, _I, _P, _P, _L, _P, _P, C.POINTER(_D)])
+
+    def write_snapshot(self, path, ids, states, alive, t):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        states = np.ascontiguousarray(states, dtype=np.float64)
+        alive = np.ascontiguousarray(alive, dtype=np.uint8)
+        self.write_snapshot_(path.encode(), len(ids), _ptr(ids), _ptr(states), _ptr(alive), float(t))
+
+    def read_snapshot(self, path, ids, events):
+        """(ids, states, alive, t) after the reference's read_snapshot into a
+        list with these ids and fragment flags, or None where it returns false."""
+        n = len(ids)
+        cap = n
+        t = _D()
+        while True:
+            ib = np.zeros(max(cap, 1), dtype=np.int32)
+            ib[:n] = ids
+            ev = np.ascontiguousarray(events, dtype=np.int32)
+            st = np.zeros((max(cap, 1), 6))
+            al = np.zeros(max(cap, 1), dtype=np.uint8)
+            got = self.read_snapshot_(path.encode(), n, _ptr(ib), _ptr(ev), cap, _ptr(st), _ptr(al), C.byref(t))
+            if got < 0:
+                return None
+            if got <= cap:
+                return ib[:got], st[:got], al[:got], t.value
+            cap = got
 
     def rng(self, seed, kind, n, mean=0.0):
         out = np.zeros(n)

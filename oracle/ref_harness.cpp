@@ -334,4 +334,35 @@ int64_t ref_apply(int n, const double* before, const double* after, const double
     return total;
 }
 
+// write_snapshot of n objects (ids, states, alive flags) at time t.
+void ref_write_snapshot(const char* path, int n, const int* ids, const double* states, const uint8_t* alive,
+                        double t) {
+    std::vector<Object> objs;
+    fill_objects(objs, n, states, alive, nullptr);
+    for (int i = 0; i < n; ++i) objs[i].id = ids[i];
+    write_snapshot(path, objs, t);
+}
+
+// read_snapshot into a list of n objects (ids, fragment flags as event >= 0):
+// the count after the replay (the first cap objects' ids, states and alive
+// flags written back), or -1 where read_snapshot returns false.
+int64_t ref_read_snapshot(const char* path, int n, int* ids, const int* events, int64_t cap, double* states,
+                          uint8_t* alive, double* t) {
+    std::vector<Object> objs(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+        objs[i].id = ids[i];
+        objs[i].event = events[i];
+    }
+    if (!read_snapshot(path, objs, t)) return -1;
+    const int64_t total = static_cast<int64_t>(objs.size());
+    for (int64_t i = 0; i < total && i < cap; ++i) {
+        const Object& o = objs[static_cast<size_t>(i)];
+        ids[i] = o.id;
+        double* d = states + 6 * static_cast<size_t>(i);
+        d[0] = o.s.r.x; d[1] = o.s.r.y; d[2] = o.s.r.z; d[3] = o.s.v.x; d[4] = o.s.v.y; d[5] = o.s.v.z;
+        alive[i] = o.alive ? 1 : 0;
+    }
+    return total;
+}
+
 }  // extern "C"
