@@ -63,3 +63,40 @@ def test_cli_forced_impact_cascade_matches_reference_binary(catalog_path, tmp_pa
     out, ref = _compare_outputs(args, tmp_path, ["conjunctions.csv", "final_state.csv",
                                                  "breakups.csv", "population.csv"])
     assert "Evenements : 84 dont 41 catastrophiques" in out
+
+
+def _run_both(args, tmp_path):
+    if not os.path.exists(REF_CLI):
+        pytest.skip("compiled reference binary (oracle/_ref) not present")
+    mine, theirs = tmp_path / "mine", tmp_path / "ref"
+    mine.mkdir(parents=True)
+    theirs.mkdir(parents=True)
+    r1 = _run(CLI, args + ["--gpu"], str(mine))
+    assert r1.returncode == 0, r1.stderr
+    r2 = _run(REF_CLI, args, str(theirs))
+    assert r2.returncode == 0, r2.stderr
+    return mine / "output", theirs / "output"
+
+
+@pytest.mark.gpu
+def test_cli_snapshots_trajectories_and_profile_match_reference_binary(catalog_path, tmp_path):
+    # 1 h at 10 s with a snapshot every 15 min; then the same run tracking two
+    # NORAD ids every 10 min (trajectories.csv instead of snapshots).
+    mine, theirs = _run_both([catalog_path, "1", "10", "900", "--conj-km", "5"], tmp_path / "snap")
+    snaps = sorted(p.name for p in theirs.glob("snapshot_*.csv"))
+    assert len(snaps) == 4 and sorted(p.name for p in mine.glob("snapshot_*.csv")) == snaps
+    for name in snaps + ["final_state.csv", "conjunctions.csv"]:
+        assert (mine / name).read_bytes() == (theirs / name).read_bytes(), f"{name} differs"
+    # profile.csv: the same header and rows; hour, object count and largest
+    # cell population agree (the other columns are timings).
+    a = (mine / "profile.csv").read_text().splitlines()
+    b = (theirs / "profile.csv").read_text().splitlines()
+    assert a[0] == b[0] and len(a) == len(b) > 1
+    cols = a[0].split(",")
+    keep = [cols.index(c) for c in ("t_h", "objects", "max_cell_objects")]
+    for ra, rb in zip(a[1:], b[1:]):
+        fa, fb = ra.split(","), rb.split(",")
+        assert [fa[k] for k in keep] == [fb[k] for k in keep]
+    mine, theirs = _run_both([catalog_path, "1", "10", "600", "25544,43013", "--conj-km", "5"], tmp_path / "track")
+    assert (mine / "trajectories.csv").read_bytes() == (theirs / "trajectories.csv").read_bytes()
+    assert not list(mine.glob("snapshot_*.csv"))
