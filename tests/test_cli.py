@@ -98,5 +98,6 @@ def test_cli_snapshots_trajectories_and_profile_match_reference_binary(catalog_p
         fa, fb = ra.split(","), rb.split(",")
         assert [fa[k] for k in keep] == [fb[k] for k in keep]
     mine, theirs = _run_both([catalog_path, "1", "10", "600", "25544,43013", "--conj-km", "5"], tmp_path / "track")
-    assert (mine / "trajectories.csv").read_bytes() == (theirs / "trajectories.csv").read_bytes()
+    for name in ("trajectories.csv", "moon.csv"):
+        assert (mine / name).read_bytes() == (theirs / name).read_bytes(), f"{name} differs"
     assert not list(mine.glob("snapshot_*.csv"))
