@@ -481,6 +481,9 @@ def ensemble_workload(kamd, args):
             conj += np.bincount(sc[below], minlength=64)
             coll += np.bincount(sc[ev["collision"] == 1], minlength=64)
             done += k
+            # Progress on stderr (stdout carries the one JSON line).
+            print(f"ensemble {'learned' if precision == kamd.PRECISION_LEARNED else 'plain FP64'}: "
+                  f"{done}/{nsteps} steps, {wall:.1f} s", file=sys.stderr, flush=True)
         s, a = eng.get_objects()
         return dict(wall=wall, conj=conj, coll=coll, alive=a, tune=tune)
 
