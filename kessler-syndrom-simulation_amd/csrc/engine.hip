@@ -612,6 +612,59 @@ __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uin
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// Every screening window of a learned step in one pass over its start (a)
+// and end (b) states: window q's end state (the dense output at (q + 1) / nw,
+// written to the q-th dense planes of `dense`, or b itself for the last
+// window) and its midpoint-grid reductions (partials + q * gridDim.x). Window
+// q starts at window q - 1's end, the same dense output k_substep recomputes,
+// so states and reductions equal k_substep's window by window.
+constexpr int FUSED_WINDOWS_MAX = 4;
+
+__global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uint8_t* alive, int n, double dt,
+                                                    int nw, double* dense, size_t cap, StepPartial* partials) {
+    __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
+    StepPartial acc[FUSED_WINDOWS_MAX];
+#pragma unroll
+    for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) acc[q] = empty_step_partial();
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
+        const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
+        const bool on = alive[i] != 0;
+        double ps[3] = {x0[0], x0[1], x0[2]}, ws[3] = {v0[0], v0[1], v0[2]};
+#pragma unroll
+        for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) {
+            if (q >= nw) break;
+            double pe[3], we[3];
+            if (q == nw - 1) {
+                for (int k = 0; k < 3; ++k) pe[k] = x1[k], we[k] = v1[k];
+            } else {
+                hermite_state(x0, v0, x1, v1, dt, static_cast<double>(q + 1) / nw, pe, we);
+                double* o = dense + static_cast<size_t>(q) * 6 * cap;
+                o[i] = pe[0]; o[cap + i] = pe[1]; o[2 * cap + i] = pe[2];
+                o[3 * cap + i] = we[0]; o[4 * cap + i] = we[1]; o[5 * cap + i] = we[2];
+            }
+            double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+            if (on) {
+                rx = 0.5 * (ps[0] + pe[0]);
+                ry = 0.5 * (ps[1] + pe[1]);
+                rz = 0.5 * (ps[2] + pe[2]);
+                const double va = sqrt(ws[0] * ws[0] + ws[1] * ws[1] + ws[2] * ws[2]);
+                const double vb = sqrt(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
+                vn = va > vb ? va : vb;
+            }
+            combine(acc[q], object_partial(on, on, vn, rx, ry, rz));
+            for (int k = 0; k < 3; ++k) ps[k] = pe[k], ws[k] = we[k];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) {
+        if (q >= nw) break;
+        const StepPartial t = block_reduce(acc[q], s_rows);
+        if (threadIdx.x == 0) partials[q * gridDim.x + blockIdx.x] = t;
+        __syncthreads();  // s_rows is reused by the next window
+    }
+}
+
 __global__ void k_reset(GridState* gs) {
     if (threadIdx.x != 0) return;
     gs->passes_max = 0;
@@ -2285,6 +2338,7 @@ struct khip_ctx {
     float4* rec = nullptr;   // the same records in object order, 2 float4 each (written by k_keys)
     GridState* gs = nullptr;
     StepPartial* step_parts = nullptr;
+    StepPartial* win_parts = nullptr;  // learned form: FUSED_WINDOWS_MAX * PART_BLOCKS window reductions
     PairPartial* pair_parts = nullptr;
     PairPartial* dense_parts = nullptr;
     PairPartial* exact_parts = nullptr;
@@ -2417,7 +2471,7 @@ void free_all(khip_ctx* ctx) {
                     ctx->bstate, ctx->snap_bstate, ctx->jobs, ctx->samples, ctx->blogs,
                     ctx->snap_state,
                     ctx->snap_alive, ctx->events, ctx->stats, ctx->moon, ctx->gs,
-                    ctx->step_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
+                    ctx->step_parts, ctx->win_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
                     ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
                     ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
                     ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates};
@@ -2551,12 +2605,12 @@ void mark(khip_ctx* ctx, int k) {
 // `before` / `after` (stats record `stat_slot`).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
-                    int sub_per = 1) {
+                    int sub_per = 1, const StepPartial* parts = nullptr) {
     const int n = static_cast<int>(ctx->n_launch);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
-    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->step_parts, part_grid(n), dt, thr, ctx->passes, midpoint,
+    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n), dt, thr, ctx->passes, midpoint,
                                      ctx->sbits, tc);
     k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
                                    ctx->idx[0], ctx->hist, tiles, ctx->rec);
@@ -2983,7 +3037,8 @@ bool ensure_learned(khip_ctx* ctx) {
     if (!dev_alloc(ctx, &ctx->mode, c) || !dev_alloc(ctx, &ctx->snap_mode, c) ||
         !dev_alloc(ctx, &ctx->lists, 3 * c) || !dev_alloc(ctx, &ctx->snap_lists, 3 * c + 3) ||
         !dev_alloc(ctx, &ctx->counts, 3) || !dev_alloc(ctx, &ctx->cstats, 1) || !dev_alloc(ctx, &ctx->rstats, 1) ||
-        !dev_alloc(ctx, &ctx->substates, static_cast<size_t>(std::max(1, sub - 1)) * 6 * c))
+        !dev_alloc(ctx, &ctx->substates, static_cast<size_t>(std::max(1, sub - 1)) * 6 * c) ||
+        !dev_alloc(ctx, &ctx->win_parts, static_cast<size_t>(FUSED_WINDOWS_MAX) * PART_BLOCKS))
         return false;
     KHIP_TRY(ctx, hipMemset(ctx->mode, net::CLASS_PLAIN, c));
     KHIP_TRY(ctx, hipMemset(ctx->counts, 0, 3 * sizeof(int)));
@@ -3167,16 +3222,24 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     mark(ctx, 7);
                     if (screen) {
                         k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
-                        // Screening windows of tau / nw, each spw sub-steps long.
+                        // Screening windows of tau / nw, each spw sub-steps long;
+                        // up to FUSED_WINDOWS_MAX windows' states and reductions
+                        // come from one pass over the step's start and end.
+                        const bool fused = nw <= FUSED_WINDOWS_MAX;
                         for (int q = 0; q < nw; ++q) {
                             g_slot = static_cast<int>(k * sub + q * spw);
                             mark(ctx, 1);
                             const Planes pb = q == 0 ? pc : dense_planes(ctx, q - 1);
                             const Planes pa = q == nw - 1 ? pn : dense_planes(ctx, q);
-                            k_substep<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
-                                                                   ctx->step_parts);
+                            if (!fused)
+                                k_substep<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
+                                                                       ctx->step_parts);
+                            else if (q == 0)
+                                k_windows<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, nw,
+                                                                       ctx->substates, c, ctx->win_parts);
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
-                                                info, q, nw, spw))
+                                                info, q, nw, spw,
+                                                fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr))
                                 return false;
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
