@@ -617,12 +617,16 @@ __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uin
 // written to the q-th dense planes of `dense`, or b itself for the last
 // window) and its midpoint-grid reductions (partials + q * gridDim.x). Window
 // q starts at window q - 1's end, the same dense output k_substep recomputes,
-// so states and reductions equal k_substep's window by window.
+// so states and reductions equal k_substep's window by window. Also marks
+// the events before the step's screening (no kernel appends events while
+// this one runs), as k_mark_events does for the per-window path.
 constexpr int FUSED_WINDOWS_MAX = 4;
 
 __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uint8_t* alive, int n, double dt,
-                                                    int nw, double* dense, size_t cap, StepPartial* partials) {
+                                                    int nw, double* dense, size_t cap, StepPartial* partials,
+                                                    GridState* gs, unsigned long long ev_cap) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
+    if (blockIdx.x == 0 && threadIdx.x == 0) gs->ev_mark = gs->events < ev_cap ? gs->events : ev_cap;
     StepPartial acc[FUSED_WINDOWS_MAX];
 #pragma unroll
     for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) acc[q] = empty_step_partial();
@@ -3221,11 +3225,12 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     }
                     mark(ctx, 7);
                     if (screen) {
-                        k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                         // Screening windows of tau / nw, each spw sub-steps long;
                         // up to FUSED_WINDOWS_MAX windows' states and reductions
                         // come from one pass over the step's start and end.
                         const bool fused = nw <= FUSED_WINDOWS_MAX;
+                        if (!fused)
+                            k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                         for (int q = 0; q < nw; ++q) {
                             g_slot = static_cast<int>(k * sub + q * spw);
                             mark(ctx, 1);
@@ -3235,8 +3240,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                 k_substep<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
                                                                        ctx->step_parts);
                             else if (q == 0)
-                                k_windows<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, nw,
-                                                                       ctx->substates, c, ctx->win_parts);
+                                k_windows<<<part_grid(n), BLOCK, 0, s>>>(
+                                    pc, pn, ctx->alive, n, ctx->dt, nw, ctx->substates, c, ctx->win_parts, ctx->gs,
+                                    static_cast<unsigned long long>(ctx->ev_cap));
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw,
                                                 fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr))
