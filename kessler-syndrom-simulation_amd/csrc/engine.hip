@@ -701,6 +701,98 @@ __global__ void k_zero_events(GridState* gs) {
 // Grid
 // ---------------------------------------------------------------------------
 
+// A screening's statistics record (k_finish), passed to the next screening's
+// k_grid_setup to be written there, one launch fewer (out == nullptr: none).
+struct FinishArgs {
+    DevStats* out;
+    const PairPartial* parts;
+    int nparts, npair;
+    const PairPartial* dparts;
+    int ndparts;
+    const PairPartial* eparts;
+    int neparts, fill;
+};
+
+__device__ __forceinline__ void finish_window(GridState* gs, DevStats* out, int screened,
+                                                   const PairPartial* parts, int nparts, int npair,
+                                                   const PairPartial* dparts, int ndparts,
+                                                   const PairPartial* eparts, int neparts, int fill) {
+    __shared__ unsigned long long s_red[TWAVES][6];
+    unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0, sv = 0;
+    if (screened) {
+        // max_cell from the k_neighbours blocks, candidates from the k_pairs blocks.
+        for (int k = threadIdx.x; k < nparts; k += TBLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
+        for (int k = threadIdx.x; k < npair; k += TBLOCK) a += parts[k].cand;
+        for (int k = threadIdx.x; k < ndparts; k += TBLOCK) dn += dparts[k].cand;
+        for (int k = threadIdx.x; k < neparts; k += TBLOCK) {
+            b += eparts[k].refined;
+            c += eparts[k].co;
+            sv += eparts[k].cand;
+        }
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    m = wave_max_u64(m);
+    dn = wave_sum(dn);
+    sv = wave_sum(sv);
+    const int wave = threadIdx.x / WAVE;
+    if (__lane_id() == 0) {
+        s_red[wave][0] = a;
+        s_red[wave][1] = b;
+        s_red[wave][2] = c;
+        s_red[wave][3] = m;
+        s_red[wave][4] = dn;
+        s_red[wave][5] = sv;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < TWAVES; ++w) {
+        s_red[0][0] += s_red[w][0];
+        s_red[0][1] += s_red[w][1];
+        s_red[0][2] += s_red[w][2];
+        s_red[0][3] = s_red[w][3] > s_red[0][3] ? s_red[w][3] : s_red[0][3];
+        s_red[0][4] += s_red[w][4];
+        s_red[0][5] += s_red[w][5];
+    }
+    if (screened) {
+        DevStats s;
+        s.cand = static_cast<long long>(s_red[0][0] + s_red[0][4]);
+        s.dense_cand = static_cast<long long>(s_red[0][4]);
+        s.survivors = static_cast<long long>(s_red[0][5]);
+        s.refined = static_cast<long long>(s_red[0][1]);
+        s.co = static_cast<long long>(s_red[0][2]);
+        gs->max_cell = static_cast<unsigned int>(s_red[0][3]);
+        s.cell_km = gs->cell_ref;
+        s.max_cell = static_cast<int>(gs->max_cell);
+        s.doublings = gs->doublings;
+        s.propagated = static_cast<long long>(gs->propagated);
+        *out = s;
+        // Records of the window's later sub-steps: zero counts, objects counted.
+        for (int j = 1; j <= fill; ++j) {
+            DevStats f{};
+            f.cell_km = s.cell_km;
+            f.propagated = s.propagated;
+            out[j] = f;
+        }
+    }
+    gs->propagated = 0;
+    gs->vmax_bits = 0ULL;
+    for (int a = 0; a < 3; ++a) {
+        gs->lo[a] = ~0ULL;
+        gs->hi[a] = 0ULL;
+    }
+    gs->alive = 0;
+    gs->cand = gs->refined = gs->co = 0ULL;
+    gs->max_cell = 0;
+    gs->ncells = 0;
+    gs->npasses = 0;
+    gs->n_items = 0;
+    gs->n_dense = 0;
+    if (gs->n_surv > gs->surv_max) gs->surv_max = gs->n_surv;
+    gs->n_surv = 0;
+}
+
 __device__ __forceinline__ int bits_for(long long range) {
     return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
 }
@@ -713,8 +805,13 @@ __device__ __forceinline__ int bits_for(long long range) {
 __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
-                                                       TileCfg tc) {
+                                                       TileCfg tc, FinishArgs fin) {
     __shared__ StepPartial s_rows[TWAVES];
+    // The previous screening's record first: it reads that screening's grid
+    // fields, which thread 0 then overwrites below.
+    if (fin.out)
+        finish_window(gs, fin.out, 1, fin.parts, fin.nparts, fin.npair, fin.dparts, fin.ndparts, fin.eparts,
+                      fin.neparts, fin.fill);
     StepPartial acc = empty_step_partial();
     for (int b = threadIdx.x; b < nparts; b += TBLOCK) combine(acc, partials[b]);
     const StepPartial tot = block_reduce<TWAVES>(acc, s_rows);
@@ -2201,80 +2298,7 @@ __global__ void __launch_bounds__(TBLOCK) k_finish(GridState* gs, DevStats* out,
                                                    const PairPartial* parts, int nparts, int npair,
                                                    const PairPartial* dparts, int ndparts,
                                                    const PairPartial* eparts, int neparts, int fill) {
-    __shared__ unsigned long long s_red[TWAVES][6];
-    unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0, sv = 0;
-    if (screened) {
-        // max_cell from the k_neighbours blocks, candidates from the k_pairs blocks.
-        for (int k = threadIdx.x; k < nparts; k += TBLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
-        for (int k = threadIdx.x; k < npair; k += TBLOCK) a += parts[k].cand;
-        for (int k = threadIdx.x; k < ndparts; k += TBLOCK) dn += dparts[k].cand;
-        for (int k = threadIdx.x; k < neparts; k += TBLOCK) {
-            b += eparts[k].refined;
-            c += eparts[k].co;
-            sv += eparts[k].cand;
-        }
-    }
-    a = wave_sum(a);
-    b = wave_sum(b);
-    c = wave_sum(c);
-    m = wave_max_u64(m);
-    dn = wave_sum(dn);
-    sv = wave_sum(sv);
-    const int wave = threadIdx.x / WAVE;
-    if (__lane_id() == 0) {
-        s_red[wave][0] = a;
-        s_red[wave][1] = b;
-        s_red[wave][2] = c;
-        s_red[wave][3] = m;
-        s_red[wave][4] = dn;
-        s_red[wave][5] = sv;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < TWAVES; ++w) {
-        s_red[0][0] += s_red[w][0];
-        s_red[0][1] += s_red[w][1];
-        s_red[0][2] += s_red[w][2];
-        s_red[0][3] = s_red[w][3] > s_red[0][3] ? s_red[w][3] : s_red[0][3];
-        s_red[0][4] += s_red[w][4];
-        s_red[0][5] += s_red[w][5];
-    }
-    if (screened) {
-        DevStats s;
-        s.cand = static_cast<long long>(s_red[0][0] + s_red[0][4]);
-        s.dense_cand = static_cast<long long>(s_red[0][4]);
-        s.survivors = static_cast<long long>(s_red[0][5]);
-        s.refined = static_cast<long long>(s_red[0][1]);
-        s.co = static_cast<long long>(s_red[0][2]);
-        gs->max_cell = static_cast<unsigned int>(s_red[0][3]);
-        s.cell_km = gs->cell_ref;
-        s.max_cell = static_cast<int>(gs->max_cell);
-        s.doublings = gs->doublings;
-        s.propagated = static_cast<long long>(gs->propagated);
-        *out = s;
-        // Records of the window's later sub-steps: zero counts, objects counted.
-        for (int j = 1; j <= fill; ++j) {
-            DevStats f{};
-            f.cell_km = s.cell_km;
-            f.propagated = s.propagated;
-            out[j] = f;
-        }
-    }
-    gs->propagated = 0;
-    gs->vmax_bits = 0ULL;
-    for (int a = 0; a < 3; ++a) {
-        gs->lo[a] = ~0ULL;
-        gs->hi[a] = 0ULL;
-    }
-    gs->alive = 0;
-    gs->cand = gs->refined = gs->co = 0ULL;
-    gs->max_cell = 0;
-    gs->ncells = 0;
-    gs->npasses = 0;
-    gs->n_items = 0;
-    gs->n_dense = 0;
-    if (gs->n_surv > gs->surv_max) gs->surv_max = gs->n_surv;
-    gs->n_surv = 0;
+    finish_window(gs, out, screened, parts, nparts, npair, dparts, ndparts, eparts, neparts, fill);
 }
 
 __global__ void k_fp64_check(int n, const double* a, const double* b, double* out) {
@@ -2606,16 +2630,22 @@ void mark(khip_ctx* ctx, int k) {
 }
 
 // Enqueues the screening kernels for the step whose start / end states are
-// `before` / `after` (stats record `stat_slot`).
+// `before` / `after` (stats record `stat_slot`). `prev` (if set) is the
+// previous screening's record, written by this screening's k_grid_setup;
+// with `defer` set, this screening's record is returned there instead of
+// launched (the next screening must then take it as `prev`).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
-                    int sub_per = 1, const StepPartial* parts = nullptr) {
+                    int sub_per = 1, const StepPartial* parts = nullptr, const FinishArgs* prev = nullptr,
+                    FinishArgs* defer = nullptr) {
     const int n = static_cast<int>(ctx->n_launch);
     const int tiles = ctx->ntiles();
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
+    FinishArgs fin{};
+    if (prev) fin = *prev;
     k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n), dt, thr, ctx->passes, midpoint,
-                                     ctx->sbits, tc);
+                                     ctx->sbits, tc, fin);
     k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
                                    ctx->idx[0], ctx->hist, tiles, ctx->rec);
     for (int pass = 0; pass < ctx->passes; ++pass) {
@@ -2673,9 +2703,13 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     mark(ctx, 4);
     k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
     mark(ctx, 5);
-    k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats + stat_slot, 1, ctx->pair_parts, grid_of(n), pair_blocks,
-                                 ctx->dense_parts, tc.dense_blocks, ctx->exact_parts, tc.exact_blocks,
-                                 info ? sub_per - 1 : 0);
+    const FinishArgs mine{ctx->stats + stat_slot, ctx->pair_parts, grid_of(n), pair_blocks, ctx->dense_parts,
+                          tc.dense_blocks, ctx->exact_parts, tc.exact_blocks, info ? sub_per - 1 : 0};
+    if (defer)
+        *defer = mine;
+    else
+        k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, mine.out, 1, mine.parts, mine.nparts, mine.npair, mine.dparts,
+                                     mine.ndparts, mine.eparts, mine.neparts, mine.fill);
     KHIP_TRY(ctx, hipGetLastError());
     return true;
 }
@@ -3229,6 +3263,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         // up to FUSED_WINDOWS_MAX windows' states and reductions
                         // come from one pass over the step's start and end.
                         const bool fused = nw <= FUSED_WINDOWS_MAX;
+                        FinishArgs pending{};  // window q - 1's record, written by window q's grid setup
                         if (!fused)
                             k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                         for (int q = 0; q < nw; ++q) {
@@ -3243,10 +3278,13 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                 k_windows<<<part_grid(n), BLOCK, 0, s>>>(
                                     pc, pn, ctx->alive, n, ctx->dt, nw, ctx->substates, c, ctx->win_parts, ctx->gs,
                                     static_cast<unsigned long long>(ctx->ev_cap));
+                            FinishArgs mine{};
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw,
-                                                fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr))
+                                                fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
+                                                q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr))
                                 return false;
+                            pending = mine;
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                                   static_cast<unsigned long long>(ctx->ev_cap), pc,
