@@ -190,7 +190,10 @@ int khip_reset_timing(khip_ctx* ctx);
  * threshold raised by margin_km; every event found is then solved again on
  * plain FP64 trajectories from the step's start (filter_pair, Hermite +
  * golden section, radius test, in the sub-step of tau / substeps holding
- * the TCA) and kept with the plain values. Event step indices count
+ * the TCA) at the configured threshold itself, and kept with the plain
+ * values when the plain solve accepts it; an event whose plain minimum
+ * leaves the learned step keeps its learned values if its learned miss
+ * distance is within the threshold (events_unverified). Event step indices count
  * sub-steps, as a plain run at dt = tau / substeps does; statistics have one
  * record per sub-step, a window's counts in the record of its first
  * sub-step and zero counts in the others. */
@@ -218,7 +221,8 @@ typedef struct {
     int64_t checks, steps;
     int64_t events_verified;      /* events confirmed by the plain solve */
     int64_t events_dropped;       /* events the plain solve rejected (not reported) */
-    int64_t events_unverified;    /* plain minimum outside the learned step (learned values kept) */
+    int64_t events_unverified;    /* plain minimum outside the learned step, learned miss within the
+                                     threshold (learned values kept) */
 } khip_learned_report;
 
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);

@@ -13,6 +13,9 @@
  *   khost_moon_position  moon_position (orbital.hpp:92, orbital.cpp:53-85)
  *   khost_generate       no counterpart: the seeded synthetic catalogue of
  *                        the benchmark configurations
+ *   khost_kepler_state   no counterpart: the Kepler-elements -> ECI
+ *                        conversion the generator uses (the inverse of the
+ *                        reference's elements(), orbital.cpp:120-130)
  */
 #ifndef KESSLER_HOST_H
 #define KESSLER_HOST_H
@@ -50,6 +53,13 @@ typedef struct {
  * mean anomaly uniform. Size classes SMALL / MEDIUM / LARGE at 50/30/20 %.
  * mix may be NULL for the defaults. */
 int khost_generate(int64_t n, uint64_t seed, const khost_mix* mix, khost_object* out);
+
+/* ECI state (km, km/s) of the two-body orbit with elements el = {a (km), e,
+ * inclination, RAAN, argument of perigee, mean anomaly} (angles in radians,
+ * 0 <= e < 1): Kepler's equation by Newton's method, perifocal position and
+ * velocity, rotation R3(-RAAN) R1(-i) R3(-argp). The conversion
+ * khost_generate applies to every object. */
+void khost_kepler_state(const double* el, double* state6);
 
 /* Applies forced impacts in sequence on one Fragmentation object seeded with
  * `seed` (L_c = lc_m). Objects 0..n-1 are the inputs (masses as given);

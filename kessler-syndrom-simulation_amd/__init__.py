@@ -62,6 +62,7 @@ PRECISION_MIXED = 1
 HOST_SYMBOLS = (
     "khost_generate", "khost_impacts", "khost_cascade", "khost_write_catalog",
     "khost_load_catalog", "khost_moon_position", "khost_write_snapshot", "khost_read_snapshot",
+    "khost_kepler_state",
 )
 
 _P = C.c_void_p
@@ -174,6 +175,7 @@ def library() -> C.CDLL:
         "khost_write_catalog": (_I, [C.c_char_p, _L, _P, C.c_char_p]),
         "khost_load_catalog": (_L, [C.c_char_p, _L, _P, C.POINTER(_D), C.POINTER(_I)]),
         "khost_moon_position": (None, [_D, _P]),
+        "khost_kepler_state": (None, [_P, _P]),
         "khost_write_snapshot": (_I, [C.c_char_p, _L, _P, _D, _I]),
         "khost_read_snapshot": (_L, [C.c_char_p, _L, _L, _P, C.POINTER(_D)]),
     }
@@ -458,6 +460,16 @@ def generate(n: int, seed: int = 0, mix=None) -> np.ndarray:
     rc = library().khost_generate(int(n), int(seed), m, _ptr(out))
     if rc != 0:
         raise RuntimeError(f"khost_generate failed ({rc})")
+    return out
+
+
+def kepler_state(elements) -> np.ndarray:
+    """ECI state of Kepler elements (a, e, i, RAAN, argp, M; radians) as the generator converts them."""
+    el = np.ascontiguousarray(elements, dtype=np.float64).reshape(-1, 6)
+    out = np.zeros_like(el)
+    lib = library()
+    for k in range(len(el)):
+        lib.khost_kepler_state(_ptr(el[k]), _ptr(out[k]))
     return out
 
 

@@ -1890,11 +1890,15 @@ __global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, Pa
 // learned step's start state by FP64 RK4 sub-steps (the plain method) to the
 // start and end of sub-step k, and the reference's exact test (filter_pair,
 // Hermite + golden section, radius test; collision.cpp:109-132) runs on that
-// pair of states. A minimum on a sub-step edge is looked for in the
-// neighbouring sub-step of the same learned step. The event's TCA, miss
-// distance, speed and collision flag become the plain solve's; an event the
-// plain solve rejects is dropped (pad = 1); one whose plain minimum leaves
-// the learned step keeps its learned values (pad = 2).
+// pair of states, with the caller's threshold itself (the raised threshold
+// serves the learned search only), so the decision is the plain run's. A
+// minimum on a sub-step edge is looked for in the neighbouring sub-step of
+// the same learned step. The event's TCA, miss distance, speed and
+// collision flag become the plain solve's; an event the plain solve rejects
+// is dropped (pad = 1). One whose plain minimum leaves the learned step
+// (a TCA within a fraction of a millisecond of the step's edge) keeps its
+// learned values (pad = 2) if its learned miss distance is within the
+// caller's threshold, and is dropped otherwise.
 
 struct ReverifyStats {
     unsigned long long verified, dropped, kept_unverified;
@@ -1956,7 +1960,7 @@ __device__ __forceinline__ void plain_at2(Planes s0, int i, int j, const MoonSam
 __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* events, unsigned long long cap,
                                                      Planes s0, const double* radius,
                                                      const MoonSample* __restrict__ moon13, const StepInfo* info,
-                                                     double h, int sub, double thr, double vmin2,
+                                                     double h, int sub, double thr_report, double vmin2,
                                                      ReverifyStats* rs) {
     const double t_step = info->t;
     const long long sub0 = info->macro * sub;
@@ -1973,7 +1977,7 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
         while (tries < 2 && k >= 0 && k < sub) {
             D3 rai, vai, rbi, vbi, raj, vaj, rbj, vbj;
             plain_at2(s0, i, j, moon13, h, sub, k, rai, vai, rbi, vbi, raj, vaj, rbj, vbj);
-            code = plain_refine(rai, vai, raj, vaj, rbi, vbi, rbj, vbj, h, thr, vmin2, s, miss, vrel);
+            code = plain_refine(rai, vai, raj, vaj, rbi, vbi, rbj, vbj, h, thr_report, vmin2, s, miss, vrel);
             ++tries;
             if (code == 2 && dir <= 0) {
                 dir = -1;
@@ -1993,7 +1997,7 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
             ev.collision = miss < radius[i] + radius[j] ? 1 : 0;
             ev.pad = 0;
             ++ver;
-        } else if ((code == 2 || code == 3) && (k < 0 || k >= sub)) {
+        } else if ((code == 2 || code == 3) && (k < 0 || k >= sub) && ev.miss_km <= thr_report) {
             ev.pad = 2;  // the plain minimum lies in the neighbouring learned step
             ++kept;
         } else {
@@ -3288,8 +3292,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                                   static_cast<unsigned long long>(ctx->ev_cap), pc,
-                                                                  ctx->radius, m13, info, h, sub, thr, vmin2,
-                                                                  ctx->rstats);
+                                                                  ctx->radius, m13, info, h, sub,
+                                                                  ctx->cfg.threshold_km, vmin2, ctx->rstats);
                         mark(ctx, 0);
                     } else {
                         k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,

@@ -183,6 +183,12 @@ int khost_generate(int64_t n, uint64_t seed, const khost_mix* mix, khost_object*
     return 0;
 }
 
+void khost_kepler_state(const double* el, double* state6) {
+    const State s = to_eci(Kepler{el[0], el[1], el[2], el[3], el[4], el[5]});
+    state6[0] = s.r.x; state6[1] = s.r.y; state6[2] = s.r.z;
+    state6[3] = s.v.x; state6[4] = s.v.y; state6[5] = s.v.z;
+}
+
 int64_t khost_impacts(int64_t n, const khost_object* in, double epoch_jd, double t, int nimp,
                       const int32_t* targets, const double* projectile_mass_kg,
                       const double* v_rel_km_s, uint64_t seed, double lc_m, int64_t cap,

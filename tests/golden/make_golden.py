@@ -16,6 +16,12 @@ Fixtures (all data):
       PortableRng draws.
   ref_catalog_1h.npz  1 h (360 steps, dt 10 s, 5 km) main-loop run on the
       real catalogue: events, per-step stats, final states.
+  ref_breakup.npz   Fragmentation::impact in sequence on small synthetic
+      catalogues (inputs from the package's generator, stored with the
+      outputs): every object after the impacts and the per-event
+      fragment / escaped counts (the cases of tests/test_breakup_host.py).
+
+``python tests/golden/make_golden.py --breakup`` writes ref_breakup.npz only.
 """
 from __future__ import annotations
 
@@ -173,5 +179,35 @@ def main() -> None:
     print("golden fixtures written:", sorted(os.listdir(HERE)))
 
 
+def breakup_fixture() -> None:
+    ref = reference()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libkessler_ref.so not built: run make -C oracle ref")
+    from tests.test_breakup_host import CASES, _case
+    from __graft_entry__ import load_package
+
+    kamd = load_package()
+    g = {}
+    for k in range(len(CASES)):
+        pop, targets, pm, vr, seed, lc = _case(kamd, k)
+        objs, ev = ref.impacts(pop["state"], pop["size_m"], pop["mass_kg"], EPOCH, 0.0, targets, pm, vr, seed, lc)
+        g[f"in_state_{k}"], g[f"in_size_{k}"], g[f"in_mass_{k}"] = pop["state"], pop["size_m"], pop["mass_kg"]
+        g[f"targets_{k}"], g[f"pm_{k}"], g[f"vr_{k}"] = targets, pm, vr
+        g[f"seed_{k}"], g[f"lc_{k}"] = np.array(seed), np.array(lc)
+        n0 = len(pop)
+        # Parents keep their states; only their alive flags and masses change.
+        g[f"out_alive_{k}"], g[f"out_mass_kg_{k}"] = objs["alive"], objs["mass_kg"]
+        for f in ("event", "state", "size_m", "area_to_mass"):
+            g[f"out_{f}_{k}"] = objs[f][n0:]
+        g[f"out_events_{k}"] = np.stack([ev["fragments"], ev["escaped"]], 1)
+    g["n_cases"] = np.array(len(CASES))
+    np.savez_compressed(os.path.join(HERE, "ref_breakup.npz"), **g)
+    print("ref_breakup.npz written")
+
+
 if __name__ == "__main__":
-    main()
+    if "--breakup" in sys.argv:
+        breakup_fixture()
+    else:
+        main()
+        breakup_fixture()

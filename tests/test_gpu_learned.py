@@ -166,16 +166,25 @@ def test_learned_screening_recall_at_2km(engine, kamd, trained):
         tl, ml = min(hits, key=lambda h: abs(h[0] - t))
         worst_t = max(worst_t, abs(tl - t))
         worst_m = max(worst_m, abs(ml - m))
+    # Precision: every learned event is a plain event (same pair, same or
+    # adjacent sub-step), except pairs on the threshold's edge, where the
+    # learned and plain trajectories' metre-level difference decides.
+    extra = [(key, m) for key, (t, m) in kl.items()
+             if not any(k2 in kp for k2 in ((key[0], key[1], key[2] + d) for d in (0, -1, 1)))]
     below = int((el["miss_km"] < 2.0).sum())
     print(f"1 h at 2 km: plain {len(kp)} events, learned {len(el)} ({below} below 2 km), missing {len(missing)}, "
+          f"extra {len(extra)} (miss {[round(m, 6) for _, m in extra[:5]]}), "
           f"max |dTCA| {worst_t * 1e3:.3e} ms, max |dmiss| {worst_m * 1e3:.3e} m; "
           f"candidates per sub-step {stl['candidate_pairs'].mean():.0f} vs plain {st64['candidate_pairs'].mean():.0f}; "
           f"{rep}")
     assert not missing
     assert worst_t < 1e-3 and worst_m < 1e-3
-    # Events are plain-verified, sorted by (step, t, i, j), within the raised threshold.
+    assert all(m > 2.0 - 0.01 for _, m in extra)
+    assert len(extra) <= max(2, len(kp) // 10000)
+    # Events are plain-verified at the configured threshold itself (the
+    # raised one serves the search only) and sorted by (step, t, i, j).
     assert rep["events_verified"] + rep["events_unverified"] >= len(el)
-    assert (el["miss_km"] <= 2.0 + 0.025).all()
+    assert (el["miss_km"] <= 2.0).all()
     order = np.lexsort((el["j"], el["i"], el["t"], el["step"]))
     assert np.array_equal(order, np.arange(len(el)))
 
