@@ -243,15 +243,33 @@ def event_keys(ev):
     return (ev["i"].astype(np.int64) << 43) | (ev["j"].astype(np.int64) << 23) | ev["step"].astype(np.int64)
 
 
-def recall_against(e_ref, e_new):
+EDGE_M = 10.0  # the learned form's 24 h position bound (m)
+
+
+def recall_against(e_ref, e_new, thr=None):
     """Recall of e_ref's (pair, step) set in e_new, an event of e_new in an
-    adjacent step counting as found (a TCA on a step edge); TCA and miss
-    spread over the exact (pair, step) matches."""
+    adjacent step counting as found (a TCA on a step edge); precision the
+    same the other way round, with the miss distances of e_new's unmatched
+    events (within how much of the threshold they lie); TCA and miss spread
+    over the exact (pair, step) matches."""
     kr, kn = event_keys(e_ref), event_keys(e_new)
     found = np.isin(kr, kn) | np.isin(kr - 1, kn) | np.isin(kr + 1, kn)
+    back = np.isin(kn, kr) | np.isin(kn - 1, kr) | np.isin(kn + 1, kr)
     out = {"reference_events": int(len(kr)), "events": int(len(kn)), "found": int(found.sum()),
-           "recall": float(found.mean()) if len(kr) else 1.0, "exact_matches": 0, "max_dtca_ms": 0.0,
-           "max_dmiss_m": 0.0}
+           "recall": float(found.mean()) if len(kr) else 1.0,
+           "precision": float(back.mean()) if len(kn) else 1.0, "unmatched_events": int((~back).sum()),
+           "exact_matches": 0, "max_dtca_ms": 0.0, "max_dmiss_m": 0.0}
+    if thr is not None:
+        # Disagreements of two trajectories that differ by metres can only
+        # concern pairs whose miss distance lies that close to the threshold:
+        # how close the unmatched events of either side come to it, and the
+        # recall over the reference events more than EDGE_M inside it.
+        if (~back).any():
+            out["unmatched_within_m_of_threshold"] = float((thr - e_new["miss_km"][~back]).max() * 1e3)
+        if (~found).any():
+            out["missed_within_m_of_threshold"] = float((thr - e_ref["miss_km"][~found]).max() * 1e3)
+        inner = e_ref["miss_km"] <= thr - EDGE_M * 1e-3
+        out[f"recall_beyond_{EDGE_M:g}_m_of_threshold"] = float(found[inner].mean()) if inner.any() else 1.0
     if len(kn) and len(kr):
         order = np.argsort(kn)
         skn = kn[order]
@@ -265,13 +283,15 @@ def recall_against(e_ref, e_new):
     return out
 
 
-# HBM bytes per launch of the pair kernels in the learned workload (20 s
-# screening windows), measured by two rocprofv3 PMC passes (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, factors from copy kernels of known size in the same runs).
-PAIR_TRAFFIC_B = {"k_pairs": 160.9e6, "k_pairs_dense": 0.1e6}
-# Screening windows per 60 s learned step (20 s each; propagation and the
-# plain re-verification keep 10 s sub-steps).
-WINDOWS = 3
+# HBM bytes per launch of the pair kernels in the learned workload (two 30 s
+# screening windows with 32 km radial shells), measured by two rocprofv3 PMC
+# passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from copy kernels of
+# known size in the same runs); None until measured at this configuration.
+PAIR_TRAFFIC_B = {}
+# Screening windows per 60 s learned step (30 s each; propagation and the
+# plain re-verification keep 10 s sub-steps) and the radial shell width.
+WINDOWS = 2
+SHELL_KM = 32.0
 
 # BF16 matrix-core flops of one corrector evaluation (10-128-128-128-6;
 # the input and output layers count once although they run hi/lo splits).
@@ -300,7 +320,7 @@ def learned_workload(kamd, args):
     def run(precision, dt, nsteps, nwarm):
         eng.set_precision(precision)
         if precision == kamd.PRECISION_LEARNED:
-            eng.set_learned(windows=WINDOWS)
+            eng.set_learned(windows=WINDOWS, shell_km=SHELL_KM)
         for w in (nwarm, 0):
             eng.set_objects(pop["state"], alive0, pop["size_m"])
             eng.set_config(thr, 1.0, 0.01)
@@ -341,7 +361,7 @@ def learned_workload(kamd, args):
     obj_l = int(lrn["stats"]["propagated"].sum())
     obj_p = int(plain["stats"]["propagated"].sum())
     v_l, v_p = obj_l / lrn["wall"], obj_p / plain["wall"]
-    rec = recall_against(plain["events"], lrn["events"])
+    rec = recall_against(plain["events"], lrn["events"], thr)
     tm = lrn["timing"]
     launches_sub = max(1, int(tm["steps"]) * WINDOWS)
     kernels = {"k_learned (propagation)": float(tm["propagate_ms"]), "grid (11-13 kernels)": float(tm["grid_ms"]),
@@ -385,8 +405,8 @@ def learned_workload(kamd, args):
         "ms_per_step": 1e3 * lrn["wall"] / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 corrector + f64 state + f32 perturbations", "data": "synthetic",
         "config": {"workload": ("learned corrector: 1,000,000 synthetic objects (seed 0, LEO shells x1.5), "
-                                f"60 s steps, {steps * 60 / 3600:g} h, screening 2 km (+25 m margin), "
-                                "object-steps counted in 10 s steps"),
+                                f"60 s steps, {steps * 60 / 3600:g} h, screening 2 km (+25 m margin, two 30 s "
+                                "windows, 32 km radial shells), object-steps counted in 10 s steps"),
                    "objects": int(len(pop)), "dt_s": 60.0, "threshold_km": thr},
         "plain_fp64_value": v_p, "learned_over_plain_fp64": v_l / v_p,
         "object_seconds_per_s": v_l * DT,
@@ -401,6 +421,9 @@ def learned_workload(kamd, args):
         "corrector_mfma_tflops_in_propagation": mf,
         "candidate_pairs_per_window": cand / launches_sub,
         "candidate_pairs_per_step_plain_fp64": float(plain["stats"]["candidate_pairs"].mean()),
+        "candidate_pairs_per_simulated_minute": {"learned": cand / steps,
+                                                 "plain_fp64": 6.0 * float(plain["stats"]["candidate_pairs"].mean())},
+        "screening": {"windows_per_step": WINDOWS, "shell_km": SHELL_KM},
         "roofline": roof, "step_roofline": model_l, "step_roofline_plain_fp64": model_p,
         "device": eng.device_name,
     }

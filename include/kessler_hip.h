@@ -106,6 +106,7 @@ typedef struct {
 #define KHIP_RERUN_ITEMS 4     /* dense-cell work items */
 #define KHIP_RERUN_SURVIVORS 8 /* pre-filter survivor queue */
 #define KHIP_RERUN_BREAKUPS 16 /* breakup jobs, fragment samples or reserved object slots */
+#define KHIP_RERUN_ENTRIES 32  /* radial-shell entries of the learned form's screening */
 
 /* Number of visible HIP devices (0 when none or the runtime fails). */
 int khip_device_count(void);
@@ -185,9 +186,15 @@ int khip_reset_timing(khip_ctx* ctx);
  * steps, and the first step after khip_set_objects / khip_set_precision /
  * khip_set_learned) runs the plain method for every object and keeps the
  * cheaper method for an object only if it stayed within tol_km of the plain
- * result in position and within tol_a_km in semi-major axis. Screening covers each window of tau / windows on the cubic
- * Hermite dense output of the step, with a grid on midpoints and the
- * threshold raised by margin_km; every event found is then solved again on
+ * result in position and within tol_a_km in semi-major axis. Screening
+ * covers each window of tau / windows on the cubic Hermite dense output of
+ * the step, with a grid on midpoints and the threshold raised by margin_km.
+ * With shell_km > 0 each object is also binned by the radius range its
+ * cubic covers in the window (fixed radial shells, shell_km wide below
+ * 2,000 km altitude, coarser above; an object enters every shell its range
+ * widened by half the threshold meets), and a pair is enumerated only
+ * inside a shared shell, once: two objects closer than the threshold have
+ * radii closer than the threshold. Every event found is then solved again on
  * plain FP64 trajectories from the step's start (filter_pair, Hermite +
  * golden section, radius test, in the sub-step of tau / substeps holding
  * the TCA) at the configured threshold itself, and kept with the plain
@@ -209,9 +216,14 @@ typedef struct {
     double a_max_km;     /* corrector domain: semi-major axis below (default 12000) */
     double tol_a_km;     /* semi-major-axis deviation tolerated at a check (default 5e-8: a
                             systematic 5e-8 km per 60 s step drifts ~5 m along track in 24 h) */
-    int32_t windows;     /* screening windows per learned step (divides substeps; 0: substeps / 2
-                            when even, else substeps). Fewer, longer windows mean fewer grid
-                            builds and more candidate pairs per build. */
+    int32_t windows;     /* screening windows per learned step (divides substeps; 0: two with
+                            shells when substeps is even, else substeps / 2 when even, else
+                            substeps). Fewer, longer windows mean fewer grid builds and more
+                            candidate pairs per build. */
+    double shell_km;     /* radial shell width of the screening (>= 1 km, with at most 4 windows):
+                            objects are binned by their radius range over the window as well
+                            as by cell, and pairs are enumerated within shells only (see
+                            KHIP_PRECISION_LEARNED); 0 screens on cells alone. Default 32. */
 } khip_learned_config;
 
 typedef struct {

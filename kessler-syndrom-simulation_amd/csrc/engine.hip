@@ -70,6 +70,12 @@ constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
 constexpr int TWAVES = TBLOCK / 64;
 constexpr int COL_BITS = 24;  // column table index width (see k_cell_write)
+// Shell screening (learned form): a sorted element is one (object, radial
+// shell) entry; its index word holds the object index in the low OBJ_BITS
+// and NOT_FIRST for every entry but the object's innermost shell.
+constexpr int OBJ_BITS = 30;
+constexpr int OBJ_MASK = (1 << OBJ_BITS) - 1;
+constexpr int NOT_FIRST = 1 << OBJ_BITS;
 // LSD radix sort of the packed cell keys, 9-bit digits: at 1M objects the
 // key has 27 bits (about 400 cells of 200 km per axis), which three passes
 // cover.
@@ -138,7 +144,13 @@ struct GridState {
     unsigned long long events;      // events appended since the last drain
     unsigned long long ev_mark;     // events before the current learned step
     int stamp;                      // screening counter (column-table entries of this screening)
-    int pad3;
+    int bbits;                      // radial-shell bits between y and z (shell screening; 0 otherwise)
+    unsigned int nsort;             // elements the radix passes sort (objects, or shell entries)
+    unsigned int nlive;             // live sorted elements (alive objects, or every shell entry)
+    unsigned int bmin;              // innermost shell of this screening (shell screening)
+    int entries_short;              // a screening had more shell entries than were launched
+    unsigned int entries_max;       // max shell entries over the chunk
+    int pad4;
 };
 
 struct DevEvent {
@@ -167,6 +179,7 @@ struct StepPartial {
     unsigned long long hi[3];      // ordered bits of max start-of-step position
     unsigned int alive;            // alive at the end of the step
     unsigned int propagated;       // alive at the start of the step
+    unsigned int bmin, bmax;       // shell range of the entries (shell screening)
 };
 
 struct PairPartial {
@@ -201,6 +214,8 @@ __device__ __forceinline__ StepPartial empty_step_partial() {
     }
     p.alive = 0;
     p.propagated = 0;
+    p.bmin = ~0u;
+    p.bmax = 0u;
     return p;
 }
 
@@ -212,6 +227,8 @@ __device__ __forceinline__ void combine(StepPartial& a, const StepPartial& b) {
     }
     a.alive += b.alive;
     a.propagated += b.propagated;
+    a.bmin = b.bmin < a.bmin ? b.bmin : a.bmin;
+    a.bmax = b.bmax > a.bmax ? b.bmax : a.bmax;
 }
 
 __device__ __forceinline__ StepPartial wave_reduce(StepPartial p) {
@@ -222,6 +239,8 @@ __device__ __forceinline__ StepPartial wave_reduce(StepPartial p) {
     }
     p.alive = wave_sum<unsigned int>(p.alive);
     p.propagated = wave_sum<unsigned int>(p.propagated);
+    p.bmin = static_cast<unsigned int>(wave_min_u64(p.bmin));
+    p.bmax = static_cast<unsigned int>(wave_max_u64(p.bmax));
     return p;
 }
 
@@ -249,6 +268,28 @@ __device__ __forceinline__ StepPartial object_partial(bool on, bool was, double 
     }
     p.propagated = was ? 1 : 0;
     return p;
+}
+
+// Block-wide exclusive scan of one int per thread (NW waves); returns the total.
+template <int NW = WAVES_PER_BLOCK>
+__device__ __forceinline__ int block_exclusive_scan(int v, int& total, int* s_wave) {
+    const int lane = __lane_id();
+    const int wave = threadIdx.x / WAVE;
+    int incl = v;
+    for (int off = 1; off < WAVE; off <<= 1) {
+        const int o = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += o;
+    }
+    if (lane == WAVE - 1) s_wave[wave] = incl;
+    __syncthreads();
+    int before = 0, sum = 0;
+    for (int w = 0; w < NW; ++w) {
+        if (w < wave) before += s_wave[w];
+        sum += s_wave[w];
+    }
+    __syncthreads();
+    total = sum;
+    return before + incl - v;
 }
 
 // ---------------------------------------------------------------------------
@@ -612,6 +653,56 @@ __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uin
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// Radial shells of the learned form's screening. Two objects closer than
+// thr at some instant have radii closer than thr then, so the intervals
+// [rho_lo - thr / 2, rho_hi + thr / 2] of their radius ranges over the
+// window share a point, and the shell holding that point holds an entry of
+// each. Every object gets one entry per shell its interval meets; a pair is
+// screened in its innermost common shell only, the one where at least one
+// of the two entries is its object's first (NOT_FIRST clear on one side).
+// Shells are fixed intervals of the radius: w km wide from r0 to r1 = r0 +
+// n1 w (the crowded low-Earth-orbit region), w2 km wide beyond, and the
+// last one (index nmax) open-ended. shell_of is non-decreasing in rho.
+struct ShellCfg {
+    double r0, w, r1, w2;
+    int n1, nmax;
+    double half;  // half the search threshold plus rounding slack (km)
+};
+
+__device__ __forceinline__ int shell_of(double rho, const ShellCfg& c) {
+    if (rho < c.r1) return rho <= c.r0 ? 0 : static_cast<int>((rho - c.r0) / c.w);
+    const double k = (rho - c.r1) / c.w2;
+    return k >= static_cast<double>(c.nmax - c.n1) ? c.nmax : c.n1 + static_cast<int>(k);
+}
+
+// Radius range of a window's cubic (the Hermite of its end states p0, v0
+// and p1, v1 over h seconds). The curve lies in the convex hull of its
+// Bezier control points P0 = p0, P1 = p0 + h v0 / 3, P2 = p1 - h v1 / 3,
+// P3 = p1, so |p(t)| <= max |Pk|, and |p(t)| >= p(t).u >= min Pk.u for the
+// unit vector u along P0 + P3.
+__device__ __forceinline__ void radius_range(const double p0[3], const double v0[3], const double p1[3],
+                                             const double v1[3], double h, double& lo, double& hi) {
+    double P[4][3];
+    for (int k = 0; k < 3; ++k) {
+        P[0][k] = p0[k];
+        P[1][k] = p0[k] + h * v0[k] * (1.0 / 3.0);
+        P[2][k] = p1[k] - h * v1[k] * (1.0 / 3.0);
+        P[3][k] = p1[k];
+    }
+    const double ux = P[0][0] + P[3][0], uy = P[0][1] + P[3][1], uz = P[0][2] + P[3][2];
+    const double un = sqrt(ux * ux + uy * uy + uz * uz);
+    const double inv = un > 0.0 ? 1.0 / un : 0.0;
+    double h2 = 0.0, dmin = 1e300;
+    for (int j = 0; j < 4; ++j) {
+        const double r2 = P[j][0] * P[j][0] + P[j][1] * P[j][1] + P[j][2] * P[j][2];
+        h2 = r2 > h2 ? r2 : h2;
+        const double d = (P[j][0] * ux + P[j][1] * uy + P[j][2] * uz) * inv;
+        dmin = d < dmin ? d : dmin;
+    }
+    hi = sqrt(h2);
+    lo = un > 0.0 ? dmin : 0.0;
+}
+
 // Every screening window of a learned step in one pass over its start (a)
 // and end (b) states: window q's end state (the dense output at (q + 1) / nw,
 // written to the q-th dense planes of `dense`, or b itself for the last
@@ -620,20 +711,31 @@ __global__ void __launch_bounds__(BLOCK) k_substep(Planes a, Planes b, const uin
 // so states and reductions equal k_substep's window by window. Also marks
 // the events before the step's screening (no kernel appends events while
 // this one runs), as k_mark_events does for the per-window path.
+// With shells (shells != 0): each alive object's first shell and shell count
+// per window (shell_lo / shell_n + q * cap), the shell range in the
+// partials, and the entries per object tile of TILE objects added to
+// tile_cnt + q * tstride (k_grid_setup reads and clears them).
 constexpr int FUSED_WINDOWS_MAX = 4;
 
 __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uint8_t* alive, int n, double dt,
                                                     int nw, double* dense, size_t cap, StepPartial* partials,
-                                                    GridState* gs, unsigned long long ev_cap) {
+                                                    GridState* gs, unsigned long long ev_cap, int shells,
+                                                    ShellCfg sc, int* shell_lo, uint16_t* shell_n, int* tile_cnt,
+                                                    int tstride) {
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     if (blockIdx.x == 0 && threadIdx.x == 0) gs->ev_mark = gs->events < ev_cap ? gs->events : ev_cap;
     StepPartial acc[FUSED_WINDOWS_MAX];
 #pragma unroll
     for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) acc[q] = empty_step_partial();
-    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-        const double x0[3] = {a.x[i], a.y[i], a.z[i]}, v0[3] = {a.vx[i], a.vy[i], a.vz[i]};
-        const double x1[3] = {b.x[i], b.y[i], b.z[i]}, v1[3] = {b.vx[i], b.vy[i], b.vz[i]};
-        const bool on = alive[i] != 0;
+    const double hw = dt / nw;
+    // Block-uniform trip count: the per-wave tile sums below need every lane.
+    for (int i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
+        const int i = i0 + static_cast<int>(threadIdx.x);
+        const bool in = i < n;
+        const int ii = in ? i : n - 1;
+        const double x0[3] = {a.x[ii], a.y[ii], a.z[ii]}, v0[3] = {a.vx[ii], a.vy[ii], a.vz[ii]};
+        const double x1[3] = {b.x[ii], b.y[ii], b.z[ii]}, v1[3] = {b.vx[ii], b.vy[ii], b.vz[ii]};
+        const bool on = in && alive[ii] != 0;
         double ps[3] = {x0[0], x0[1], x0[2]}, ws[3] = {v0[0], v0[1], v0[2]};
 #pragma unroll
         for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) {
@@ -643,9 +745,11 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
                 for (int k = 0; k < 3; ++k) pe[k] = x1[k], we[k] = v1[k];
             } else {
                 hermite_state(x0, v0, x1, v1, dt, static_cast<double>(q + 1) / nw, pe, we);
-                double* o = dense + static_cast<size_t>(q) * 6 * cap;
-                o[i] = pe[0]; o[cap + i] = pe[1]; o[2 * cap + i] = pe[2];
-                o[3 * cap + i] = we[0]; o[4 * cap + i] = we[1]; o[5 * cap + i] = we[2];
+                if (in) {
+                    double* o = dense + static_cast<size_t>(q) * 6 * cap;
+                    o[i] = pe[0]; o[cap + i] = pe[1]; o[2 * cap + i] = pe[2];
+                    o[3 * cap + i] = we[0]; o[4 * cap + i] = we[1]; o[5 * cap + i] = we[2];
+                }
             }
             double vn = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
             if (on) {
@@ -656,7 +760,26 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
                 const double vb = sqrt(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
                 vn = va > vb ? va : vb;
             }
-            combine(acc[q], object_partial(on, on, vn, rx, ry, rz));
+            StepPartial op = object_partial(on, on, vn, rx, ry, rz);
+            if (shells) {
+                int nb = 0;
+                if (on) {
+                    double lo, hi;
+                    radius_range(ps, ws, pe, we, hw, lo, hi);
+                    const int s0 = shell_of(lo - sc.half, sc), s1 = shell_of(hi + sc.half, sc);
+                    nb = s1 - s0 + 1;
+                    shell_lo[static_cast<size_t>(q) * cap + i] = s0;
+                    shell_n[static_cast<size_t>(q) * cap + i] = static_cast<uint16_t>(nb);
+                    op.bmin = static_cast<unsigned int>(s0);
+                    op.bmax = static_cast<unsigned int>(s1);
+                } else if (in) {
+                    shell_n[static_cast<size_t>(q) * cap + i] = 0;
+                }
+                // A wave's 64 objects lie in one tile (TILE is a multiple of 64).
+                const int sum = wave_sum<int>(nb);
+                if (__lane_id() == 0 && sum > 0) atomicAdd(&tile_cnt[q * tstride + i0 / TILE], sum);
+            }
+            combine(acc[q], op);
             for (int k = 0; k < 3; ++k) ps[k] = pe[k], ws[k] = we[k];
         }
     }
@@ -691,6 +814,8 @@ __global__ void k_reset(GridState* gs) {
     gs->max_cell = 0;
     gs->ncells = 0;
     gs->npasses = 0;
+    gs->entries_short = 0;
+    gs->entries_max = 0;
 }
 
 __global__ void k_zero_events(GridState* gs) {
@@ -720,9 +845,26 @@ __device__ __forceinline__ void finish_window(GridState* gs, DevStats* out, int 
     __shared__ unsigned long long s_red[TWAVES][6];
     unsigned long long a = 0, b = 0, c = 0, m = 0, dn = 0, sv = 0;
     if (screened) {
-        // max_cell from the k_neighbours blocks, candidates from the k_pairs blocks.
-        for (int k = threadIdx.x; k < nparts; k += TBLOCK) m = parts[k].max_cell > m ? parts[k].max_cell : m;
-        for (int k = threadIdx.x; k < npair; k += TBLOCK) a += parts[k].cand;
+        // max_cell from the k_neighbours blocks, candidates from the k_pairs
+        // blocks. Loads go in batches of FIN_BATCH per thread, so a thread
+        // waits once per batch rather than once per record.
+        constexpr int FIN_BATCH = 8;
+        const int np = nparts > npair ? nparts : npair;
+        for (int k0 = threadIdx.x; k0 < np; k0 += TBLOCK * FIN_BATCH) {
+            unsigned int mc[FIN_BATCH];
+            unsigned long long cd[FIN_BATCH];
+#pragma unroll
+            for (int u = 0; u < FIN_BATCH; ++u) {
+                const int k = k0 + u * TBLOCK;
+                mc[u] = k < nparts ? parts[k].max_cell : 0u;
+                cd[u] = k < npair ? parts[k].cand : 0ULL;
+            }
+#pragma unroll
+            for (int u = 0; u < FIN_BATCH; ++u) {
+                m = mc[u] > m ? mc[u] : m;
+                a += cd[u];
+            }
+        }
         for (int k = threadIdx.x; k < ndparts; k += TBLOCK) dn += dparts[k].cand;
         for (int k = threadIdx.x; k < neparts; k += TBLOCK) {
             b += eparts[k].refined;
@@ -797,16 +939,50 @@ __device__ __forceinline__ int bits_for(long long range) {
     return range <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(range));
 }
 
+// Shell screening arguments of k_grid_setup and k_keys_shell (on == 0: the
+// plain grid, one sorted element per object).
+struct ShellScreen {
+    int on;
+    const int* tile_cnt;  // entries per object tile (k_windows), cleared here after use
+    int* tile_base;       // their exclusive prefix: first entry of each object tile
+    int ntiles;           // object tiles
+    unsigned int ecap;    // entries the sort and cell kernels were launched for
+};
+
 // midpoint != 0: the grid is built on the midpoint of each object's start and
 // end positions (learned form); a pair that comes within thr during the step
 // then has midpoints closer than vmax dt + thr + G vmax dt^3 (each object is
 // within vmax dt / 2 of its midpoint over the step), so cells of that size
 // and the 13 forward neighbours enumerate every such pair.
+// nsort: elements the plain grid sorts (every object slot, dead ones last).
+// With shells, the sorted elements are the shell entries: the packed key
+// gains the shell (offset from the innermost, bbits wide) between y and z,
+// so neighbour cells stay z-runs of one shell, and the entry count comes
+// from the prefix of the object tiles' counts.
 __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
-                                                       TileCfg tc, FinishArgs fin) {
+                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh) {
     __shared__ StepPartial s_rows[TWAVES];
+    __shared__ int s_wave[TWAVES];
+    long long entries = 0;
+    if (sh.on) {
+        // Exclusive prefix of the object tiles' entry counts (every thread).
+        long long carry = 0;
+        for (int t0 = 0; t0 < sh.ntiles; t0 += TBLOCK) {
+            const int t = t0 + static_cast<int>(threadIdx.x);
+            const int v = t < sh.ntiles ? sh.tile_cnt[t] : 0;
+            int sum;
+            const int ex = block_exclusive_scan<TWAVES>(v, sum, s_wave);
+            if (t < sh.ntiles) {
+                const long long at = carry + ex;
+                sh.tile_base[t] = at < static_cast<long long>(sh.ecap) ? static_cast<int>(at) : static_cast<int>(sh.ecap);
+                const_cast<int*>(sh.tile_cnt)[t] = 0;
+            }
+            carry += sum;
+        }
+        entries = carry;
+    }
     // The previous screening's record first: it reads that screening's grid
     // fields, which thread 0 then overwrites below.
     if (fin.out)
@@ -823,6 +999,20 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
     }
     gs->alive = tot.alive;
     gs->propagated = tot.propagated;
+    if (sh.on) {
+        if (entries > static_cast<long long>(sh.ecap)) gs->entries_short = 1;
+        const unsigned int e = static_cast<unsigned int>(entries < static_cast<long long>(sh.ecap) ? entries : sh.ecap);
+        if (static_cast<unsigned int>(entries < 0xffffffffLL ? entries : 0xffffffffLL) > gs->entries_max)
+            gs->entries_max = static_cast<unsigned int>(entries < 0xffffffffLL ? entries : 0xffffffffLL);
+        gs->nsort = gs->nlive = e;
+        gs->bmin = tot.bmin;
+        gs->bbits = tot.alive ? bits_for(static_cast<long long>(tot.bmax) - static_cast<long long>(tot.bmin)) : 0;
+    } else {
+        gs->nsort = static_cast<unsigned int>(nsort);
+        gs->nlive = tot.alive;
+        gs->bmin = 0;
+        gs->bbits = 0;
+    }
     const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
     const double vrel_max = midpoint ? vmax : 2.0 * vmax;
     const double raw = vrel_max * dt + thr + GRAVITY_GRADIENT * vrel_max * dt * dt * dt;
@@ -851,7 +1041,7 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
     // cells (and the 13 forward neighbours) never mix scenarios.
     gs->sbits = sbits;
     for (;;) {
-        int total = sbits;
+        int total = sbits + gs->bbits;
         for (int a = 0; a < 3; ++a) {
             const long long i0 = static_cast<long long>(floor(lo[a] / cell));
             const long long i1 = static_cast<long long>(floor(hi[a] / cell));
@@ -865,9 +1055,10 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
             // order). A live key is all ones in its low `total` bits only
             // if its top cell field is; while that field's largest value
             // stays below all ones, `total` sorted bits keep dead keys
-            // last, otherwise one more bit is sorted.
+            // last, otherwise one more bit is sorted. Shell entries are
+            // all live: `total` bits.
             const long long top = static_cast<long long>(floor(hi[0] / cell)) - gs->imin[0];
-            const bool top_full = top == (1LL << gs->bits[0]) - 1;
+            const bool top_full = !sh.on && top == (1LL << gs->bits[0]) - 1;
             gs->npasses = (total + (top_full ? 1 : 0) + RADIX_BITS - 1) / RADIX_BITS;
             break;
         }
@@ -878,7 +1069,6 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
     if (gs->npasses > gs->passes_max) gs->passes_max = gs->npasses;
     if (gs->npasses > launched_passes) gs->passes_short = 1;
 }
-
 
 // Also writes, in object order (coalesced), each alive object's compact
 // 32-byte FP32 record for the pair kernels: position relative to its cell's
@@ -928,10 +1118,65 @@ __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int mid
     if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int n, int pass,
-                                                 const GridState* gs, int* hist, int ntiles) {
+// Shell entries of one object tile (TILE objects, the tiles k_windows
+// counted): each alive object's compact record in object order (as k_keys
+// writes it) and one (key, index word) per shell of the object, at the
+// tile's first entry (k_grid_setup) plus the exclusive prefix of the counts
+// within the tile. The key is the cell's packed key with the shell offset
+// between y and z; the index word is the object index, NOT_FIRST set on
+// every entry but the first.
+__global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, const uint8_t* alive,
+                                                       const int* __restrict__ scen, int n, const GridState* gs,
+                                                       const int* __restrict__ shell_lo,
+                                                       const uint16_t* __restrict__ shell_n,
+                                                       const int* __restrict__ tile_base, unsigned int ecap,
+                                                       uint64_t* keys, int* idx, float4* rec) {
+    __shared__ int s_wave[TWAVES];
+    const GridState g = *gs;
+    if (g.npasses == 0) return;
+    const int base = blockIdx.x * TILE;
+    const int bz = g.bits[2], bb = g.bbits;
+    unsigned int off = static_cast<unsigned int>(tile_base[blockIdx.x]);
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + threadIdx.x;
+        int nb = 0, s0 = 0;
+        uint64_t k = 0;
+        if (e < n && alive[e]) {
+            nb = shell_n[e];
+            s0 = shell_lo[e];
+            const double x = cur.x[e], y = cur.y[e], z = cur.z[e];
+            const double gx = 0.5 * (x + end.x[e]), gy = 0.5 * (y + end.y[e]), gz = 0.5 * (z + end.z[e]);
+            const double c = g.cell;
+            const long long ix = static_cast<long long>(floor(gx / c));
+            const long long iy = static_cast<long long>(floor(gy / c));
+            const long long iz = static_cast<long long>(floor(gz / c));
+            const uint64_t pz = static_cast<uint64_t>(iz - g.imin[2]);
+            k = (static_cast<uint64_t>(ix - g.imin[0]) << (g.bits[1] + bb + bz)) |
+                (static_cast<uint64_t>(iy - g.imin[1]) << (bb + bz)) | pz;
+            if (g.sbits) k |= static_cast<uint64_t>(scen[e]) << (g.bits[0] + g.bits[1] + bb + bz);
+            rec[2 * e] = make_float4(static_cast<float>(x - static_cast<double>(ix) * c),
+                                     static_cast<float>(y - static_cast<double>(iy) * c),
+                                     static_cast<float>(z - static_cast<double>(iz) * c), __int_as_float(e));
+            rec[2 * e + 1] = make_float4(static_cast<float>(cur.vx[e]), static_cast<float>(cur.vy[e]),
+                                         static_cast<float>(cur.vz[e]), __int_as_float(static_cast<int>(pz)));
+        }
+        int tot;
+        const int ex = block_exclusive_scan<TWAVES>(nb, tot, s_wave);
+        for (int j = 0; j < nb; ++j) {
+            const unsigned int pos = off + static_cast<unsigned int>(ex + j);
+            if (pos >= ecap) break;
+            keys[pos] = k | (static_cast<uint64_t>(s0 + j - static_cast<int>(g.bmin)) << bz);
+            idx[pos] = e | (j ? NOT_FIRST : 0);
+        }
+        off += static_cast<unsigned int>(tot);
+    }
+}
+
+__global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int pass, const GridState* gs,
+                                                 int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
     if (pass >= gs->npasses) return;
+    const int n = static_cast<int>(gs->nsort);
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
     const int shift = RADIX_BITS * pass;
@@ -942,28 +1187,6 @@ __global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int n, in
     }
     __syncthreads();
     if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
-}
-
-// Block-wide exclusive scan of one int per thread (NW waves); returns the total.
-template <int NW = WAVES_PER_BLOCK>
-__device__ __forceinline__ int block_exclusive_scan(int v, int& total, int* s_wave) {
-    const int lane = __lane_id();
-    const int wave = threadIdx.x / WAVE;
-    int incl = v;
-    for (int off = 1; off < WAVE; off <<= 1) {
-        const int o = __shfl_up(incl, off, WAVE);
-        if (lane >= off) incl += o;
-    }
-    if (lane == WAVE - 1) s_wave[wave] = incl;
-    __syncthreads();
-    int before = 0, sum = 0;
-    for (int w = 0; w < NW; ++w) {
-        if (w < wave) before += s_wave[w];
-        sum += s_wave[w];
-    }
-    __syncthreads();
-    total = sum;
-    return before + incl - v;
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts.
@@ -989,7 +1212,7 @@ __global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int
 // digit with RADIX_BITS ballots, and per-wave digit counts in LDS give the
 // cross-wave offsets.
 __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
-                                                     uint64_t* keys_out, int* idx_out, int n,
+                                                     uint64_t* keys_out, int* idx_out,
                                                      int pass, const GridState* gs,
                                                      const int* hist, const int* totals,
                                                      int ntiles) {
@@ -997,6 +1220,7 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
     __shared__ int s_wcnt[TWAVES][RADIX];
     __shared__ int s_wave[TWAVES];
     if (pass >= gs->npasses) return;
+    const int n = static_cast<int>(gs->nsort);
     const int t = threadIdx.x;
     {
         int tot;
@@ -1063,12 +1287,12 @@ __device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* 
     return (g.npasses & 1) ? i1 : i0;
 }
 
-// Cell heads per tile over the first `alive` sorted elements.
+// Cell heads per tile over the first `nlive` sorted elements.
 __global__ void __launch_bounds__(TBLOCK) k_cell_count(const uint64_t* k0, const uint64_t* k1,
                                                         const GridState* gs, int* bcount) {
     __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
-    const int A = static_cast<int>(g.alive);
+    const int A = static_cast<int>(g.nlive);
     const uint64_t* keys = sorted_keys(g, k0, k1);
     const int base = blockIdx.x * TILE;
     int cnt = 0;
@@ -1101,7 +1325,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     __shared__ int s_seg[TITEMS * TWAVES];
     __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
-    const int A = static_cast<int>(g.alive);
+    const int A = static_cast<int>(g.nlive);
     const int base = blockIdx.x * TILE;
     if (base >= A || g.npasses == 0) return;
     // Cells of the earlier tiles: the sum of their head counts (k_cell_count),
@@ -1137,9 +1361,12 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
         bal[r] = __ballot(head[r]);
         if (lane == 0) s_seg[r * TWAVES + wave] = __popcll(bal[r]);
         if (valid) {
+            // The index word (object index, and NOT_FIRST for shell entries)
+            // replaces the record's w.
             const int id = sidx[e];
-            p[r] = rec[2 * id];
-            v[r] = rec[2 * id + 1];
+            p[r] = rec[2 * (id & OBJ_MASK)];
+            v[r] = rec[2 * (id & OBJ_MASK) + 1];
+            p[r].w = __int_as_float(id);
         }
     }
     __syncthreads();
@@ -1155,7 +1382,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     // a column opens its entry and closes the previous column's; the last
     // element closes its own column.
     const bool use_tab = g.sbits + g.bits[0] + g.bits[1] <= COL_BITS;
-    const int bz = g.bits[2];
+    const int bz = g.bits[2] + g.bbits;  // a column: one (x, y) (and shell) run
 #pragma unroll
     for (int r = 0; r < TITEMS; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
@@ -1257,7 +1484,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
     __shared__ int s_base[2];
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     const int nc = static_cast<int>(gs->ncells);
-    const int A = static_cast<int>(gs->alive);
+    const int A = static_cast<int>(gs->nlive);
     unsigned int cnt = 0;
     int items = 0;
     if (c < nc) {
@@ -1266,13 +1493,17 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         cnt = static_cast<unsigned int>(e - b);
         cell_count[c] = static_cast<int>(cnt);
         const int bx = gs->bits[0], by = gs->bits[1], bz = gs->bits[2];
+        // Low field: the shell (bbits, shell screening) above z; neighbour
+        // cells share the shell, so their z-runs stay consecutive keys.
+        const int zf = bz + gs->bbits;
         const bool use_tab = gs->sbits + bx + by <= COL_BITS;
         const int stamp = gs->stamp;
         const uint64_t key = cell_key[c];
         const uint64_t mz = (1ULL << bz) - 1ULL;
-        const uint64_t scen_part = gs->sbits ? (key >> (bx + by + bz)) << (bx + by + bz) : 0ULL;
-        const long long x = static_cast<long long>((key >> (by + bz)) & ((1ULL << bx) - 1ULL));
-        const long long y = static_cast<long long>((key >> bz) & ((1ULL << by) - 1ULL));
+        const uint64_t shell_part = key & (((1ULL << zf) - 1ULL) ^ mz);
+        const uint64_t scen_part = gs->sbits ? (key >> (bx + by + zf)) << (bx + by + zf) : 0ULL;
+        const long long x = static_cast<long long>((key >> (by + zf)) & ((1ULL << bx) - 1ULL));
+        const long long y = static_cast<long long>((key >> zf) & ((1ULL << by) - 1ULL));
         const long long z = static_cast<long long>(key & mz);
         int2 rng[NRANGES];
         // Range 0: own cell, then (x, y, z+1) if it exists (the next key).
@@ -1299,13 +1530,13 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
             lo_key[r] = hi_key[r] = 0ULL;
             valid[r] = axis_fits(nx, bx) && axis_fits(ny, by);
             if (valid[r]) {
-                const uint64_t row = scen_part | (static_cast<uint64_t>(nx) << (by + bz)) |
-                                     (static_cast<uint64_t>(ny) << bz);
-                lo_key[r] = row | static_cast<uint64_t>(zlo);
-                hi_key[r] = row | static_cast<uint64_t>(zhi);
+                const uint64_t row = scen_part | (static_cast<uint64_t>(nx) << (by + zf)) |
+                                     (static_cast<uint64_t>(ny) << zf);
+                lo_key[r] = row | shell_part | static_cast<uint64_t>(zlo);
+                hi_key[r] = row | shell_part | static_cast<uint64_t>(zhi);
                 len[r] = nc;
                 if (use_tab) {
-                    const int4 t = coltab[row >> bz];
+                    const int4 t = coltab[row >> zf];
                     valid[r] = t.x == stamp;  // the column has cells in this screening
                     base[r] = valid[r] ? t.y : 0;
                     len[r] = valid[r] ? t.z - t.y : 0;
@@ -1658,12 +1889,15 @@ __device__ __forceinline__ int unit_bucket(int l) {
     return c > 0 ? c : 0;
 }
 
+// SHELL: the sorted elements are shell entries; a pair is tested in the
+// shell where one of its two entries is its object's first.
+template <bool SHELL>
 __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     __shared__ int s_hist[UNIT_BUCKETS];
     __shared__ int s_total;
     __shared__ int2 s_unit[PAIR_GROUP * NRANGES];
-    const int A = static_cast<int>(gs->alive);
+    const int A = static_cast<int>(gs->nlive);
     const int g0 = blockIdx.x * PAIR_GROUP;
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
@@ -1735,10 +1969,12 @@ __global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, Pa
 #pragma unroll
                 for (int k = 0; k < PAIR_BATCH; ++k) {
                     if (q + k > ql) continue;
+                    const int ia = __float_as_int(la.w), ib = __float_as_int(b[k].w);
+                    if (SHELL && (ia & ib & NOT_FIRST)) continue;  // tested in an inner common shell
                     const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
                     if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
                                             w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f, lim_max2))
-                        push_survivor(gs, P, __float_as_int(la.w), __float_as_int(b[k].w));
+                        push_survivor(gs, P, ia & OBJ_MASK, ib & OBJ_MASK);
                 }
             }
         }
@@ -1759,6 +1995,7 @@ constexpr int PARTNER_BATCH = 4;
 //     sparse cells): 64 partners per lane load, the own objects' records
 //     scalar loads.
 // Both evaluate the same expression on the same operands, own object first.
+template <bool SHELL>
 __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
                                                         const PairItem* __restrict__ items,
                                                         const float4* __restrict__ lrec,
@@ -1803,7 +2040,9 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                         const float4 la = ta[j], va = tv[j];
                         const int p = item.own_begin + j0 + j;
                         // Range 0 pairs only with later positions.
-                        const bool test = has && j0 + j < item.own_count && (item.kind != 0 || q > p);
+                        const int ia = __float_as_int(la.w), ib = __float_as_int(lb.w);
+                        const bool test = has && j0 + j < item.own_count && (item.kind != 0 || q > p) &&
+                                          !(SHELL && (ia & ib & NOT_FIRST));
                         bool keep = false;
                         if (test) {
                             ++cand;
@@ -1814,7 +2053,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                                                        vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
                                                        P.vmin2_f, lim_max2);
                         }
-                        slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
+                        slots.push(gs, P, keep, ia & OBJ_MASK, ib & OBJ_MASK, lane, below);
                     }
                 }
             }
@@ -1845,7 +2084,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
             for (int j = 0; j < PARTNER_BATCH; ++j) {
                 const float4 lb = tb[j];
                 const float4 vb = tw[j];
-                const bool test = active && q0 + j < q1 && q0 + j >= first;
+                const int ia = __float_as_int(la.w), ib = __float_as_int(lb.w);
+                const bool test = active && q0 + j < q1 && q0 + j >= first && !(SHELL && (ia & ib & NOT_FIRST));
                 bool keep = false;
                 if (test) {
                     ++cand;
@@ -1856,7 +2096,7 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
                                                vb.y - va.y, vb.z - va.z, P.dt_f, P.gdt3_f, P.base_f,
                                                P.vmin2_f, lim_max2);
                 }
-                slots.push(gs, P, keep, __float_as_int(la.w), __float_as_int(lb.w), lane, below);
+                slots.push(gs, P, keep, ia & OBJ_MASK, ib & OBJ_MASK, lane, below);
             }
         }
     }
@@ -2410,7 +2650,7 @@ struct khip_ctx {
 
     // Learned form: per-object method, per-method lists, dense-output
     // states of the sub-step boundaries, check and re-verification tallies.
-    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0};
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0, 32.0};
     int64_t lrn_cap = 0;
     uint8_t* mode = nullptr;
     uint8_t* snap_mode = nullptr;
@@ -2421,6 +2661,16 @@ struct khip_ctx {
     ReverifyStats* rstats = nullptr;
     double* substates = nullptr; // (substeps - 1) * 6 * cap: dense-output states
     int dense_k = 0;
+    // Shell screening: first shell and shell count per object and window,
+    // entries per object tile and their prefix (FUSED_WINDOWS_MAX windows).
+    int* shell_lo = nullptr;
+    uint16_t* shell_n = nullptr;
+    int* tile_cnt = nullptr;
+    int* tile_base = nullptr;
+    int tstride = 0;
+    int64_t shell_cap = 0;
+    int64_t ecap = 0;     // sorted-element capacity of the grid buffers (>= cap)
+    int64_t elaunch = 0;  // shell entries the grid kernels are launched for
     bool force_check = true;
     bool reset_checks = true;    // zero the device check tallies before the next learned run
     khip_learned_report lrep{};
@@ -2506,7 +2756,8 @@ void free_all(khip_ctx* ctx) {
                     ctx->step_parts, ctx->win_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
                     ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
                     ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
-                    ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates};
+                    ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates,
+                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tile_base};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -2525,33 +2776,42 @@ bool ensure_surv(khip_ctx* ctx, int64_t want) {
     return true;
 }
 
+// Buffers indexed by sorted element or cell: objects for the plain grid,
+// shell entries for the shell screening.
+bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
+    if (want <= ctx->ecap) return true;
+    const int64_t cap = std::max<int64_t>(want, ctx->cap);
+    const int64_t tiles = (cap + TILE - 1) / TILE;
+    ctx->ecap = cap;
+    return dev_alloc(ctx, &ctx->keys[0], cap) && dev_alloc(ctx, &ctx->keys[1], cap) &&
+           dev_alloc(ctx, &ctx->idx[0], cap) && dev_alloc(ctx, &ctx->idx[1], cap) &&
+           dev_alloc(ctx, &ctx->hist, RADIX * tiles) && dev_alloc(ctx, &ctx->bcount, tiles) &&
+           dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
+           dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
+           dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->lrec, 2 * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) && dev_alloc(ctx, &ctx->dense, cap);
+}
+
 bool ensure_capacity(khip_ctx* ctx, int64_t n) {
     if (n <= ctx->cap) return true;
     // Objects are re-uploaded by the caller after a growth.
     const int64_t cap = n + n / 2 + 1024;
-    const int64_t tiles = (cap + TILE - 1) / TILE;
     ctx->cap = cap;
     for (int b = 0; b < 2; ++b)
         if (!dev_alloc(ctx, &ctx->state[b], 6 * static_cast<size_t>(cap))) return false;
     return dev_alloc(ctx, &ctx->alive, cap) && dev_alloc(ctx, &ctx->radius, cap) &&
            dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->mass, cap) &&
-           dev_alloc(ctx, &ctx->snap_mass, cap) && dev_alloc(ctx, &ctx->keys[0], cap) &&
-           dev_alloc(ctx, &ctx->keys[1], cap) && dev_alloc(ctx, &ctx->idx[0], cap) &&
-           dev_alloc(ctx, &ctx->idx[1], cap) && dev_alloc(ctx, &ctx->hist, RADIX * tiles) &&
-           dev_alloc(ctx, &ctx->totals, RADIX) && dev_alloc(ctx, &ctx->bcount, tiles) &&
-           dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
-           dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
-           dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
-           dev_alloc(ctx, &ctx->lrec, 2 * static_cast<size_t>(cap)) &&
+           dev_alloc(ctx, &ctx->snap_mass, cap) && ensure_sort_capacity(ctx, cap) &&
+           dev_alloc(ctx, &ctx->totals, RADIX) &&
            dev_alloc(ctx, &ctx->rec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
            dev_alloc(ctx, &ctx->step_parts, (cap + BLOCK - 1) / BLOCK) &&
-           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) &&
            dev_alloc(ctx, &ctx->dense_parts, MAX_DENSE_BLOCKS) &&
            dev_alloc(ctx, &ctx->exact_parts, MAX_EXACT_BLOCKS) &&
            ensure_surv(ctx, std::max<int64_t>(ctx->surv_cap, 1 << 24)) &&
-           dev_alloc(ctx, &ctx->dense, cap) && ensure_items(ctx, std::max<int64_t>(ctx->item_cap, 1 << 20));
+           ensure_items(ctx, std::max<int64_t>(ctx->item_cap, 1 << 20));
 }
 
 bool ensure_events(khip_ctx* ctx, int64_t want) {
@@ -2638,28 +2898,46 @@ void mark(khip_ctx* ctx, int k) {
 // previous screening's record, written by this screening's k_grid_setup;
 // with `defer` set, this screening's record is returned there instead of
 // launched (the next screening must then take it as `prev`).
+// shell_q >= 0: shell screening of learned window shell_q (sorted elements
+// are the shell entries k_windows counted; ctx->elaunch of them at most).
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
                     int sub_per = 1, const StepPartial* parts = nullptr, const FinishArgs* prev = nullptr,
-                    FinishArgs* defer = nullptr) {
-    const int n = static_cast<int>(ctx->n_launch);
-    const int tiles = ctx->ntiles();
+                    FinishArgs* defer = nullptr, int shell_q = -1) {
+    const int n_obj = static_cast<int>(ctx->n_launch);
+    const int obj_tiles = ctx->ntiles();
+    const bool shells = shell_q >= 0;
+    // Sorted elements the grid kernels are launched for.
+    const int n = shells ? static_cast<int>(ctx->elaunch) : n_obj;
+    const int tiles = static_cast<int>((static_cast<int64_t>(n) + TILE - 1) / TILE);
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
     FinishArgs fin{};
     if (prev) fin = *prev;
-    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n), dt, thr, ctx->passes, midpoint,
-                                     ctx->sbits, tc, fin);
-    k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
-                                   ctx->idx[0], ctx->hist, tiles, ctx->rec);
+    ShellScreen sh{0, nullptr, nullptr, 0, 0};
+    if (shells)
+        sh = ShellScreen{1, ctx->tile_cnt + static_cast<size_t>(shell_q) * ctx->tstride,
+                         ctx->tile_base + static_cast<size_t>(shell_q) * ctx->tstride, obj_tiles,
+                         static_cast<unsigned int>(n)};
+    k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n_obj), dt, thr, ctx->passes,
+                                     midpoint, ctx->sbits, tc, fin, n_obj, sh);
+    if (shells)
+        k_keys_shell<<<obj_tiles, TBLOCK, 0, s>>>(
+            pb, pa, ctx->alive, ctx->scen, n_obj, ctx->gs, ctx->shell_lo + static_cast<size_t>(shell_q) * ctx->cap,
+            ctx->shell_n + static_cast<size_t>(shell_q) * ctx->cap, sh.tile_base, static_cast<unsigned int>(n),
+            ctx->keys[0], ctx->idx[0], ctx->rec);
+    else
+        k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
+                                       ctx->idx[0], ctx->hist, tiles, ctx->rec);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
-        if (pass > 0)
-            k_hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], n, pass, ctx->gs, ctx->hist, tiles);
+        // k_keys counts the first pass's digits; shell entries are not in
+        // object tiles, so their first pass is counted here too.
+        if (pass > 0 || shells)
+            k_hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], pass, ctx->gs, ctx->hist, tiles);
         k_digit_scan<<<RADIX, BLOCK, 0, s>>>(ctx->hist, tiles, pass, ctx->gs, ctx->totals);
-        k_scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst],
-                                          ctx->idx[dst], n, pass, ctx->gs, ctx->hist, ctx->totals,
-                                          tiles);
+        k_scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
+                                          ctx->gs, ctx->hist, ctx->totals, tiles);
     }
     k_cell_count<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
     k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
@@ -2700,10 +2978,17 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.gdt3_f = static_cast<float>(GRAVITY_GRADIENT) * P.dt_f * P.dt_f * P.dt_f;
     P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
     const int pair_blocks = std::max(1, (n + PAIR_GROUP - 1) / PAIR_GROUP);
-    k_pairs<<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    if (shells)
+        k_pairs<true><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    else
+        k_pairs<false><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     mark(ctx, 3);
-    k_pairs_dense<<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
-                                                 ctx->dense_parts);
+    if (shells)
+        k_pairs_dense<true><<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
+                                                             ctx->dense_parts);
+    else
+        k_pairs_dense<false><<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
+                                                              ctx->dense_parts);
     mark(ctx, 4);
     k_exact<<<tc.exact_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->exact_parts);
     mark(ctx, 5);
@@ -2803,6 +3088,16 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     if (screen && g.surv_short) {
         if (!ensure_surv(ctx, 2 * static_cast<int64_t>(g.surv_max) + 4096)) return false;
         ctx->last_rerun |= KHIP_RERUN_SURVIVORS;
+    }
+    if (screen && g.entries_short) {
+        const int64_t want = static_cast<int64_t>(g.entries_max) + static_cast<int64_t>(g.entries_max) / 4 + TILE;
+        if (want >= (1LL << 31) - TILE) {
+            ctx->error = "shell screening: entries exceed the sorted-element index range";
+            return false;
+        }
+        if (!ensure_sort_capacity(ctx, want)) return false;
+        ctx->elaunch = want;
+        ctx->last_rerun |= KHIP_RERUN_ENTRIES;
     }
     rerun = ctx->last_rerun != 0;
     if (rerun) return true;
@@ -3066,9 +3361,14 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
 constexpr size_t kGraphCache = 8;
 
 // Screening windows per learned step: the configured count, or by default
-// half the sub-steps (two sub-steps per window) when that divides evenly.
+// two (half a step each) with radial shells when the sub-steps split
+// evenly, and half the sub-steps (two sub-steps per window) without them.
+// Shells cut the candidates of a window so far that two windows beat three:
+// at 1M objects, 2 km, three windows cost a third more grid builds than
+// they save in pair tests.
 int learned_windows(const khip_learned_config& L) {
     if (L.windows > 0) return L.windows;
+    if (L.shell_km > 0 && L.substeps % 2 == 0) return 2;
     return L.substeps % 2 == 0 ? L.substeps / 2 : L.substeps;
 }
 
@@ -3089,6 +3389,38 @@ bool ensure_learned(khip_ctx* ctx) {
     ctx->dense_k = sub - 1;
     ctx->force_check = true;
     return true;
+}
+
+// Shell screening buffers: first shell and shell count per object and
+// window, and the per-object-tile entry counts (zeroed here once; each
+// window's k_grid_setup clears the counts it consumed).
+bool ensure_shells(khip_ctx* ctx) {
+    const int tiles = static_cast<int>((ctx->cap + TILE - 1) / TILE);
+    if (ctx->shell_cap >= ctx->cap && ctx->tile_cnt && ctx->tstride >= tiles) return true;
+    const size_t c = static_cast<size_t>(ctx->cap);
+    if (!dev_alloc(ctx, &ctx->shell_lo, FUSED_WINDOWS_MAX * c) || !dev_alloc(ctx, &ctx->shell_n, FUSED_WINDOWS_MAX * c) ||
+        !dev_alloc(ctx, &ctx->tile_cnt, static_cast<size_t>(FUSED_WINDOWS_MAX) * tiles) ||
+        !dev_alloc(ctx, &ctx->tile_base, static_cast<size_t>(FUSED_WINDOWS_MAX) * tiles))
+        return false;
+    KHIP_TRY(ctx, hipMemset(ctx->tile_cnt, 0, sizeof(int) * FUSED_WINDOWS_MAX * tiles));
+    ctx->tstride = tiles;
+    ctx->shell_cap = ctx->cap;
+    return true;
+}
+
+// The radial shells for a search threshold thr: shell_km wide from 6,300 km
+// to the first multiple past 8,400 km (about 2,000 km altitude), 256 km (or
+// shell_km, if wider) beyond, the last one open from about 60,000 km.
+ShellCfg shell_layout(double shell_km, double thr) {
+    ShellCfg c;
+    c.r0 = 6300.0;
+    c.w = shell_km;
+    c.n1 = static_cast<int>(std::ceil((8400.0 - c.r0) / c.w));
+    c.r1 = c.r0 + c.n1 * c.w;
+    c.w2 = std::max(256.0, shell_km);
+    c.nmax = c.n1 + static_cast<int>(std::ceil(std::max(0.0, 60000.0 - c.r1) / c.w2));
+    c.half = 0.5 * thr + 1e-6;  // + rounding of the radius bounds (~1e-12 relative)
+    return c;
 }
 
 Planes dense_planes(khip_ctx* ctx, int k) {
@@ -3170,6 +3502,21 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     const int nw = learned_windows(L);
     const int spw = sub / nw;
     const double hw = ctx->dt / nw;
+    // Radial shells: with the fused window pass only (its reductions count
+    // the entries), and within the index word's object range.
+    const bool shells = screen && L.shell_km > 0 && nw <= FUSED_WINDOWS_MAX;
+    if (shells) {
+        if (n >= (1 << OBJ_BITS)) {
+            ctx->error = "shell screening: more objects than the entry index word holds";
+            return false;
+        }
+        if (!ensure_shells(ctx)) return false;
+        if (ctx->elaunch < n) {
+            const int64_t want = n + n / 2 + TILE;
+            if (!ensure_sort_capacity(ctx, want)) return false;
+            ctx->elaunch = want;
+        }
+    }
     LearnedParams lp;
     lp.tau = ctx->dt;
     lp.substeps = sub;
@@ -3179,6 +3526,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     lp.tol_km = L.tol_km;
     lp.tol_a_km = L.tol_a_km;
     const double thr = ctx->cfg.threshold_km + L.margin_km;
+    const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
     const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
     const size_t c = static_cast<size_t>(ctx->cap);
     const int g_net = ctx->n_cus;
@@ -3281,12 +3629,14 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             else if (q == 0)
                                 k_windows<<<part_grid(n), BLOCK, 0, s>>>(
                                     pc, pn, ctx->alive, n, ctx->dt, nw, ctx->substates, c, ctx->win_parts, ctx->gs,
-                                    static_cast<unsigned long long>(ctx->ev_cap));
+                                    static_cast<unsigned long long>(ctx->ev_cap), shells ? 1 : 0, shell_cfg,
+                                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride);
                             FinishArgs mine{};
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw,
                                                 fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
-                                                q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr))
+                                                q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr,
+                                                shells ? q : -1))
                                 return false;
                             pending = mine;
                         }
@@ -3304,7 +3654,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 return true;
             };
             if (ctx->use_graph && !ctx->timing) {
-                std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n};
+                std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
+                                            shells ? ctx->elaunch : 0};
                 key.insert(key.end(), pattern.begin(), pattern.end());
                 hipGraphExec_t exec = nullptr;
                 for (size_t g = 0; g < ctx->graphs.size(); ++g) {
@@ -4171,7 +4522,8 @@ int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
     if (!ctx || !cfg || cfg->substeps < 1 || cfg->substeps > 12 || 12 % cfg->substeps != 0 ||
         cfg->check_every < 1 || !(cfg->tol_km >= 0) || !(cfg->margin_km >= 0) || !(cfg->e_max >= 0) ||
         !(cfg->a_max_km >= 0) || !(cfg->tol_a_km >= 0) || cfg->windows < 0 ||
-        (cfg->windows > 0 && cfg->substeps % cfg->windows != 0))
+        (cfg->windows > 0 && cfg->substeps % cfg->windows != 0) ||
+        !(cfg->shell_km == 0 || (cfg->shell_km >= 1.0 && cfg->shell_km < 1e5)))
         return KHIP_ERR_ARG;
     ctx->lcfg = *cfg;
     ctx->force_check = true;

@@ -23,7 +23,7 @@ C_FIELDS = {
     "khip_timing": ["propagate_ms", "grid_ms", "pairs_ms", "pairs_sparse_ms", "pairs_dense_ms", "exact_ms",
                     "total_ms", "steps", "reruns", "rerun_reasons", "reverify_ms"],
     "khip_learned_config": ["substeps", "check_every", "tol_km", "margin_km", "e_max", "a_max_km", "tol_a_km",
-                            "windows"],
+                            "windows", "shell_km"],
     "khip_learned_report": ["learned", "base", "plain", "rejected", "dev_max_km", "checks", "steps",
                             "events_verified", "events_dropped", "events_unverified"],
     "khip_breakup_config": ["enabled", "reserve", "lc_m", "catastrophic_j_per_g", "seed"],

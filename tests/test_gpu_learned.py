@@ -216,3 +216,34 @@ def test_screening_windows_do_not_change_events(engine, kamd, trained):
         assert np.array_equal(st["propagated"], st6["propagated"])
         first = np.arange(len(st)) % (6 // nw) == 0
         assert (st["candidate_pairs"][~first] == 0).all() and (st["candidate_pairs"][first] > 0).all()
+
+
+def test_radial_shells_do_not_change_events(engine, kamd, trained):
+    """Radial shells only drop pairs whose radii stay farther apart than the
+    search threshold over the window, and enumerate every other pair once,
+    so the events (found on the learned trajectories, then solved again on
+    plain ones) are the same with and without shells, value for value, for
+    one, two and three windows and two shell widths; the candidate count
+    falls."""
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    pop = kamd.generate(60000, seed=9, mix=MIX_DENSE)
+    runs = {}
+    try:
+        for nw in (1, 2, 3):
+            for shell in (0.0, 16.0, 4.0):
+                _, _, ev, st = _run(engine, pop, 2, 20, 60.0, True, windows=nw, shell_km=shell)
+                runs[nw, shell] = (np.sort(ev, order=["step", "t", "i", "j"]), st)
+    finally:
+        engine.set_learned()
+    for nw in (1, 2, 3):
+        ev0, st0 = runs[nw, 0.0]
+        c0 = int(st0["candidate_pairs"].sum())
+        for shell in (16.0, 4.0):
+            ev, st = runs[nw, shell]
+            c = int(st["candidate_pairs"].sum())
+            print(f"windows {nw}, shells {shell} km: {len(ev)} events, candidates {c} vs {c0} without shells")
+            assert np.array_equal(ev, ev0), (nw, shell)
+            assert np.array_equal(st["propagated"], st0["propagated"])
+            assert c < c0
+        assert len(ev0) > 50
