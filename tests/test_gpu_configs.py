@@ -1,0 +1,228 @@
+"""GPU tier: every benchmark configuration at its own size.
+
+* cascade, 255,220 objects (the 28,340-object catalogue plus 40 seeded
+  catastrophic breakups), plain FP64 at 5 km: the first hour (360 steps of
+  10 s) on the engine and on the reference's compiled CPU loop
+  (oracle/_ref; the oracle restatement where that is absent). Events in
+  every field, per-step statistics, final states and alive flags are
+  bit-identical (main.cpp:346-444, breakup.cpp:340-360 for the seeding).
+* precision form, 1,000,000 objects, 24 h: within 10 m of the plain FP64
+  run, whose states equal the oracle's on a sample of objects; screening at
+  2 km over the first hour finds the plain run's events.
+* learned form, 1,000,000 objects, 24 h at 60 s steps, screening at 2 km:
+  within 10 m of the oracle-anchored plain run; every plain event more than
+  10 m inside the threshold found (recall 1.0), and the two event sets
+  differ only on the threshold's 10 m edge; matched events agree in TCA
+  and miss distance within what the metre-level trajectory difference
+  allows.
+* ensemble, the 64 cascade scenarios (9.7 M objects) in one concatenated
+  population: three scenarios run alone equal their part of the
+  concatenated run bit for bit; the learned form's per-scenario conjunction
+  counts match the plain run's up to edge events.
+
+Each run is bounded so the module takes about two minutes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPOCH = 2461253.9166666665
+MIX_DENSE = (0.7, 0.1, 0.1, 0.1, 1.5)
+EDGE_KM = 0.010  # the learned form's 24 h position bound
+
+
+def _run(eng, pop, precision, steps, dt, thr, screen=True, offsets=None, **learned):
+    eng.set_precision(precision)
+    if learned:
+        eng.set_learned(**learned)
+    eng.set_objects(pop["state"], pop["alive"].astype(np.uint8), pop["size_m"])
+    if offsets is not None:
+        eng.set_scenarios(offsets)
+    eng.set_config(thr, 1.0, 0.01)
+    eng.set_time(EPOCH, 0.0, dt)
+    eng.run(steps, screen=screen)
+    s, a = eng.get_objects()
+    return s, a, eng.take_events(), eng.take_stats()
+
+
+def _ordered(ev):
+    return ev[np.lexsort((ev["j"], ev["i"], ev["t"], ev["step"]))]
+
+
+def _keys(ev):
+    """(i, j, step) in one int64: i, j < 2^24, step < 2^15."""
+    assert len(ev) == 0 or (ev["j"].max() < (1 << 24) and ev["step"].max() < (1 << 15))
+    return (ev["i"].astype(np.int64) << 39) | (ev["j"].astype(np.int64) << 15) | ev["step"].astype(np.int64)
+
+
+def _match(e_ref, e_new):
+    """found: e_ref events present in e_new in the same or an adjacent step;
+    back: the same for e_new in e_ref; (ir, jn): indices of exact (pair,
+    step) matches."""
+    kr, kn = _keys(e_ref), _keys(e_new)
+    found = np.isin(kr, kn) | np.isin(kr - 1, kn) | np.isin(kr + 1, kn)
+    back = np.isin(kn, kr) | np.isin(kn - 1, kr) | np.isin(kn + 1, kr)
+    order = np.argsort(kn)
+    pos = np.clip(np.searchsorted(kn[order], kr), 0, max(len(kn) - 1, 0))
+    exact = (kn[order][pos] == kr) if len(kn) else np.zeros(len(kr), bool)
+    return found, back, np.flatnonzero(exact), order[pos[exact]]
+
+
+def _checker(orc, ref):
+    return ref if ref is not None else orc
+
+
+def test_cascade_first_hour_matches_reference(engine, kamd, orc, ref):
+    pop = kamd.cascade(28340, 0, 40, 0.045)
+    assert 250_000 < len(pop) < 260_000
+    s, a, ev, st = _run(engine, pop, kamd.PRECISION_FP64, 360, 10.0, 5.0)
+    chk = _checker(orc, ref)
+    chk.threads(min(16, os.cpu_count() or 1))
+    rs, ra, rev, rst, loop_s = chk.run(pop["state"], pop["alive"].astype(np.uint8), pop["size_m"], EPOCH, 10.0,
+                                       360, screen=True, thr=5.0, cap=1 << 24)
+    print(f"cascade {len(pop)} objects, 1 h: {len(ev)} events ({int(ev['collision'].sum())} collisions), "
+          f"{int(st['candidate_pairs'].sum())} candidate pairs; {'reference' if ref is not None else 'oracle'} "
+          f"loop {loop_s:.1f} s")
+    assert len(ev) > 100_000
+    ev, rev = _ordered(ev), _ordered(rev)
+    assert len(ev) == len(rev)
+    for f in ("i", "j", "step", "t", "miss_km", "v_rel_km_s", "collision"):
+        assert np.array_equal(ev[f], rev[f]), f
+    for f in ("candidate_pairs", "refined_pairs", "co_orbiting", "cell_km", "max_cell_objects"):
+        assert np.array_equal(st[f], rst[f]), f
+    assert np.array_equal(a, ra)
+    assert np.array_equal(s, rs, equal_nan=True)
+
+
+@pytest.fixture(scope="module")
+def million(kamd):
+    return kamd.generate(1_000_000, seed=0, mix=MIX_DENSE)
+
+
+@pytest.fixture(scope="module")
+def plain_day(kamd, million, orc):
+    """The 1M catalogue's 24 h plain FP64 run (10 s, screened at 2 km),
+    anchored on the oracle for 200 sampled objects."""
+    eng = kamd.Engine(0)
+    try:
+        s, a, ev, st = _run(eng, million, kamd.PRECISION_FP64, 8640, 10.0, 2.0)
+    finally:
+        eng.close()
+    idx = np.linspace(0, len(million) - 1, 200).astype(int)
+    so, ao, _, _, _ = orc.run(million["state"][idx], million["alive"][idx].astype(np.uint8),
+                              million["size_m"][idx], EPOCH, 10.0, 8640, screen=False, per_step=False)
+    assert np.array_equal(so, s[idx], equal_nan=True) and np.array_equal(ao, a[idx])
+    return s, a, ev, st
+
+
+def test_precision_form_day_at_1m(engine, kamd, million, plain_day):
+    s64, a64, ev64, _ = plain_day
+    s32, a32, _, _ = _run(engine, million, kamd.PRECISION_MIXED, 8640, 10.0, 2.0, screen=False)
+    assert np.array_equal(a32, a64)
+    both = a64 == 1
+    dr = np.linalg.norm(s32[both, :3] - s64[both, :3], axis=1) * 1e3
+    dv = np.linalg.norm(s32[both, 3:] - s64[both, 3:], axis=1) * 1e6
+    # Screening over the first hour, against the plain run's first hour.
+    _, _, ev32, _ = _run(engine, million, kamd.PRECISION_MIXED, 360, 10.0, 2.0)
+    ref1h = ev64[ev64["step"] < 360]
+    found, back, ir, jn = _match(ref1h, ev32)
+    print(f"precision form 24 h at 1M: |dr| max {dr.max():.3f} m p99 {np.percentile(dr, 99):.3f} m, "
+          f"|dv| max {dv.max():.3f} mm/s; 1 h: {len(ref1h)} plain events, recall {found.mean():.6f}, "
+          f"precision {back.mean():.6f}")
+    assert dr.max() <= 10.0
+    assert found[ref1h["miss_km"] <= 2.0 - EDGE_KM].all()
+    assert (ref1h["miss_km"][~found] > 2.0 - EDGE_KM).all() and (ev32["miss_km"][~back] > 2.0 - EDGE_KM).all()
+    assert np.abs(ref1h["t"][ir] - ev32["t"][jn]).max() < 1e-3
+    assert np.abs(ref1h["miss_km"][ir] - ev32["miss_km"][jn]).max() < 1e-3
+
+
+def test_learned_form_day_at_1m(engine, kamd, million, plain_day, trained):
+    s64, a64, ev64, _ = plain_day
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    sl, al, evl, stl = _run(engine, million, kamd.PRECISION_LEARNED, 1440, 60.0, 2.0)
+    rep = engine.learned_report()
+    assert np.array_equal(al, a64)
+    both = a64 == 1
+    dr = np.linalg.norm(sl[both, :3] - s64[both, :3], axis=1) * 1e3
+    found, back, ir, jn = _match(ev64, evl)
+    inner = ev64["miss_km"] <= 2.0 - EDGE_KM
+    dt_ms = np.abs(ev64["t"][ir] - evl["t"][jn]) * 1e3
+    dm_m = np.abs(ev64["miss_km"][ir] - evl["miss_km"][jn]) * 1e3
+    vrel = ev64["v_rel_km_s"][ir]
+    print(f"learned 24 h at 1M: |dr| max {dr.max():.3f} m p99 {np.percentile(dr, 99):.3f} m; events plain "
+          f"{len(ev64)}, learned {len(evl)}; recall {found.mean():.7f} (beyond 10 m of the threshold "
+          f"{found[inner].mean():.7f}), precision {back.mean():.7f}; exact matches {len(ir)}: dTCA max "
+          f"{dt_ms.max():.3f} ms ({(dt_ms <= 1.0).mean():.7f} within 1 ms), dmiss max {dm_m.max():.3f} m "
+          f"({(dm_m <= 1.0).mean():.7f} within 1 m); candidates per window {stl['candidate_pairs'].sum() / 2880:.3e}; "
+          f"{rep}")
+    assert dr.max() <= 10.0
+    assert found[inner].all()
+    assert (ev64["miss_km"][~found] > 2.0 - EDGE_KM).all()
+    assert (evl["miss_km"][~back] > 2.0 - EDGE_KM).all()
+    assert (evl["miss_km"] <= 2.0).all()
+    # Matched events: the miss distance moves by at most the two objects'
+    # position differences, the TCA by that over the relative speed.
+    assert dm_m.max() <= 2 * 10.0
+    assert (dt_ms <= 1.0 + 2 * 10.0 / (vrel * 1e3) * 1e3).all()
+    assert (dt_ms <= 1.0).mean() > 0.9999 and (dm_m <= 1.0).mean() > 0.9999
+
+
+@pytest.fixture(scope="module")
+def ensemble(kamd):
+    """The 64 scenarios of the ensemble configuration (seeds 0-63)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(2026)
+    sizes = np.exp(rng.uniform(np.log(30000), np.log(300000), 64)).astype(int)
+    breakups = rng.integers(0, 61, 64)
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        pops = list(ex.map(lambda k: kamd.cascade(int(sizes[k]), k, int(breakups[k]), 0.1), range(64)))
+    off = np.concatenate([[0], np.cumsum([len(p) for p in pops])]).astype(np.int64)
+    return pops, off, sizes, breakups
+
+
+def _local(ev, lo):
+    e = ev.copy()
+    e["i"] -= lo
+    e["j"] -= lo
+    return _ordered(e)
+
+
+def test_ensemble_scenarios_concatenated_equal_alone(engine, kamd, ensemble, trained):
+    pops, off, sizes, breakups = ensemble
+    allpop = np.concatenate(pops)
+    assert 9_000_000 < len(allpop) < 10_500_000
+    steps = 30
+    s, a, ev, _ = _run(engine, allpop, kamd.PRECISION_FP64, steps, 10.0, 5.0, offsets=off)
+    sid = np.searchsorted(off, ev["i"], side="right") - 1
+    assert np.array_equal(sid, np.searchsorted(off, ev["j"], side="right") - 1), "a pair crosses scenarios"
+    picks = [int(np.argmin(sizes)), int(np.argmax(sizes)), int(np.argmax(breakups))]
+    for k in picks:
+        s1, a1, e1, _ = _run(engine, pops[k], kamd.PRECISION_FP64, steps, 10.0, 5.0)
+        lo, hi = off[k], off[k + 1]
+        assert np.array_equal(s[lo:hi], s1, equal_nan=True), k
+        assert np.array_equal(a[lo:hi], a1), k
+        mine, alone = _local(ev[sid == k], lo), _ordered(e1)
+        assert len(mine) == len(alone), k
+        for f in ("i", "j", "step", "t", "miss_km", "v_rel_km_s", "collision"):
+            assert np.array_equal(mine[f], alone[f]), (k, f)
+    # The learned form over the same 5 minutes: per-scenario counts.
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    _, al, evl, _ = _run(engine, allpop, kamd.PRECISION_LEARNED, steps // 6, 60.0, 5.0, offsets=off)
+    lid = np.searchsorted(off, evl["i"], side="right") - 1
+    cp, cl = np.bincount(sid, minlength=64), np.bincount(lid, minlength=64)
+    found, back, _, _ = _match(ev, evl)
+    inner = ev["miss_km"] <= 5.0 - EDGE_KM
+    print(f"ensemble: {len(allpop)} objects in 64 scenarios, 5 min: plain {len(ev)} events "
+          f"({int(ev['collision'].sum())} collisions), learned {len(evl)} ({int(evl['collision'].sum())}); "
+          f"per-scenario count difference max {int(np.abs(cp - cl).max())}; recall beyond the edge "
+          f"{found[inner].mean():.7f}")
+    assert len(ev) > 100_000
+    assert np.array_equal(al, a)
+    assert found[inner].all()
+    assert (evl["miss_km"][~back] > 5.0 - EDGE_KM).all()

@@ -22,18 +22,6 @@ R_E = 6378.137
 MIX_DENSE = (0.7, 0.1, 0.1, 0.1, 1.5)
 
 
-@pytest.fixture(scope="session")
-def trained(kamd):
-    eng = kamd.Engine(0)
-    orbits = kamd.generate(500000, seed=101)["state"]
-    eng.set_time(EPOCH, 0.0, 60.0)
-    rep, curve = eng.net_train(orbits, samples_per_orbit=10, spacing_steps=12, iterations=20000,
-                               batch=16384, lr_max=3e-3, seed=1)
-    params, norm = eng.net_get()
-    eng.close()
-    return rep, curve, params, norm
-
-
 def _elements(s):
     rn = np.linalg.norm(s[:, :3], axis=1)
     en = 0.5 * (s[:, 3:] ** 2).sum(1) - MU / rn
