@@ -160,6 +160,23 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
                 const uint8_t* alive_after, const double* size_m, double t0, double dt,
                 khip_event* events, int64_t cap, int64_t* nevents, khip_stats* stats);
 
+/* khip_screen without device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Every
+ * device buffer it uses (staged inputs, grid, pre-filter survivors, events)
+ * is a region of one fixed array of about 27 MB that each GPU holds from the
+ * library's load to the end of the process; the call makes no hipMalloc or
+ * hipFree. Limits per call: n <= 32,768 objects (else KHIP_ERR_ARG), at most
+ * 131,072 events, 1,048,576 pre-filter survivors and 65,536 dense-cell work
+ * items (else KHIP_ERR_STATE, with a message). `events` must hold every
+ * event: when cap is smaller, *nevents receives the count and nothing is
+ * copied (KHIP_ERR_ARG). The context's own population is left untouched.
+ * Calls on one GPU share its array, whatever the context: calls that
+ * overlap on one GPU race. Events and stats equal khip_screen's. */
+int khip_screen_static(khip_ctx* ctx, int64_t n, const double* before, const double* after,
+                       const uint8_t* alive_after, const double* size_m, double t0, double dt,
+                       khip_event* events, int64_t cap, int64_t* nevents, khip_stats* stats);
+
 /* Moves up to cap accumulated events (sorted by step, t, i, j) to `out`;
  * *n receives the number moved. Call with out == NULL to query the count. */
 int khip_take_events(khip_ctx* ctx, khip_event* out, int64_t cap, int64_t* n);

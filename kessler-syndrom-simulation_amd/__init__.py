@@ -39,7 +39,7 @@ TIMING_DTYPE = np.dtype([("propagate_ms", "<f8"), ("grid_ms", "<f8"), ("pairs_ms
 ABI_SYMBOLS = (
     "khip_device_count", "khip_create", "khip_destroy", "khip_last_error",
     "khip_set_objects", "khip_object_count", "khip_get_objects", "khip_set_config",
-    "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_take_events",
+    "khip_set_time", "khip_get_time", "khip_run", "khip_screen", "khip_screen_static", "khip_take_events",
     "khip_take_stats", "khip_set_timing", "khip_get_timing", "khip_reset_timing",
     "khip_set_chunk", "khip_fp64_selfcheck", "khip_get_previous", "khip_set_precision",
     "khip_net_train", "khip_net_set", "khip_net_get", "khip_net_infer", "khip_net_gradient",
@@ -142,6 +142,7 @@ def library() -> C.CDLL:
         "khip_get_time": (_I, [_P, C.POINTER(_D), C.POINTER(_L)]),
         "khip_run": (_I, [_P, _L, _I]),
         "khip_screen": (_I, [_P, _L, _P, _P, _P, _P, _D, _D, _P, _L, C.POINTER(_L), _P]),
+        "khip_screen_static": (_I, [_P, _L, _P, _P, _P, _P, _D, _D, _P, _L, C.POINTER(_L), _P]),
         "khip_take_events": (_I, [_P, _P, _L, C.POINTER(_L)]),
         "khip_take_stats": (_I, [_P, _P, _L, C.POINTER(_L)]),
         "khip_set_timing": (_I, [_P, _I]),

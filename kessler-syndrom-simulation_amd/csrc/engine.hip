@@ -47,6 +47,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1381,7 +1382,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     // Column table (when the column index fits COL_BITS): the first cell of
     // a column opens its entry and closes the previous column's; the last
     // element closes its own column.
-    const bool use_tab = g.sbits + g.bits[0] + g.bits[1] <= COL_BITS;
+    const bool use_tab = coltab != nullptr && g.sbits + g.bits[0] + g.bits[1] <= COL_BITS;
     const int bz = g.bits[2] + g.bbits;  // a column: one (x, y) (and shell) run
 #pragma unroll
     for (int r = 0; r < TITEMS; ++r) {
@@ -1496,7 +1497,7 @@ __global__ void __launch_bounds__(BLOCK) k_neighbours(GridState* gs, const uint6
         // Low field: the shell (bbits, shell screening) above z; neighbour
         // cells share the shell, so their z-runs stay consecutive keys.
         const int zf = bz + gs->bbits;
-        const bool use_tab = gs->sbits + bx + by <= COL_BITS;
+        const bool use_tab = coltab != nullptr && gs->sbits + bx + by <= COL_BITS;
         const int stamp = gs->stamp;
         const uint64_t key = cell_key[c];
         const uint64_t mz = (1ULL << bz) - 1ULL;
@@ -2693,6 +2694,8 @@ struct khip_ctx {
     std::vector<khip_stats> host_stats;
 
     bool timing = false;
+    unsigned char* scratch = nullptr;  // this device's copy of the allocation-free screening's array
+    int static_passes = 3;             // radix passes launched by khip_screen_static (adapted)
     std::vector<hipEvent_t> ev_pool;   // 4 per step of a chunk
     khip_timing tm{};
     std::vector<MoonSample> moon_host;
@@ -4221,9 +4224,279 @@ bool net_gradient(khip_ctx* ctx, int64_t n, const float* x, const float* y, floa
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free screening (khip_screen_static)
+// ---------------------------------------------------------------------------
+//
+// One screening of host-held states, as khip_screen does, for callers that
+// cannot allocate device memory: every device buffer it uses (the staged
+// inputs, the grid, the survivor queue, the events, the grid state) is a
+// region of `scratch`, a file-scope device array of fixed size that each GPU
+// holds from the library's load to the end of the process. Its address on a
+// device comes from hipGetSymbolAddress in khip_create (the symbol's name is
+// a host-side stand-in, not a device address). Inputs above the limits below
+// are rejected. Calls on one GPU share its array: they must not overlap.
+constexpr int64_t SCR_OBJECTS = 32768;     // objects per call
+constexpr int64_t SCR_EVENTS = 1 << 17;    // events per call
+constexpr int64_t SCR_SURVIVORS = 1 << 20; // pre-filter survivors per call
+constexpr int64_t SCR_ITEMS = 1 << 16;     // dense-cell work items per call
+constexpr size_t SCR_TILES = (SCR_OBJECTS + TILE - 1) / TILE;
+constexpr size_t SCR_BLOCKS = (SCR_OBJECTS + BLOCK - 1) / BLOCK;
+
+enum ScrRegion {
+    R_IN_BEFORE, R_IN_AFTER, R_IN_ALIVE, R_IN_SIZE, R_STATE0, R_STATE1, R_ALIVE, R_RADIUS, R_KEYS0, R_KEYS1,
+    R_IDX0, R_IDX1, R_HIST, R_TOTALS, R_BCOUNT, R_CELL_KEY, R_CELL_BEGIN, R_CELL_COUNT, R_CELL_OF, R_RANGES,
+    R_LREC, R_REC, R_STEP_PARTS, R_PAIR_PARTS, R_DENSE_PARTS, R_EXACT_PARTS, R_DENSE, R_ITEMS, R_SURV,
+    R_EVENTS, R_STATS, R_GS, R_COUNT
+};
+
+constexpr size_t scr_bytes(int r) {
+    constexpr size_t N = static_cast<size_t>(SCR_OBJECTS);
+    switch (r) {
+    case R_IN_BEFORE: case R_IN_AFTER: case R_STATE0: case R_STATE1: return 6 * N * sizeof(double);
+    case R_IN_ALIVE: case R_ALIVE: return N;
+    case R_IN_SIZE: case R_RADIUS: return N * sizeof(double);
+    case R_KEYS0: case R_KEYS1: case R_CELL_KEY: return N * sizeof(uint64_t);
+    case R_IDX0: case R_IDX1: case R_CELL_BEGIN: case R_CELL_COUNT: case R_CELL_OF: return N * sizeof(int);
+    case R_HIST: return RADIX * SCR_TILES * sizeof(int);
+    case R_TOTALS: return RADIX * sizeof(int);
+    case R_BCOUNT: return SCR_TILES * sizeof(int);
+    case R_RANGES: return NRANGES * N * sizeof(int2);
+    case R_LREC: case R_REC: return 2 * N * sizeof(float4);
+    case R_STEP_PARTS: return SCR_BLOCKS * sizeof(StepPartial);
+    case R_PAIR_PARTS: return SCR_BLOCKS * sizeof(PairPartial);
+    case R_DENSE_PARTS: return MAX_DENSE_BLOCKS * sizeof(PairPartial);
+    case R_EXACT_PARTS: return MAX_EXACT_BLOCKS * sizeof(PairPartial);
+    case R_DENSE: return N * sizeof(DenseCell);
+    case R_ITEMS: return SCR_ITEMS * sizeof(PairItem);
+    case R_SURV: return SCR_SURVIVORS * sizeof(uint2);
+    case R_EVENTS: return SCR_EVENTS * sizeof(DevEvent);
+    case R_STATS: return sizeof(DevStats);
+    case R_GS: return sizeof(GridState);
+    default: return 0;
+    }
+}
+
+constexpr size_t scr_offset(int r) {
+    size_t o = 0;
+    for (int k = 0; k < r; ++k) o += (scr_bytes(k) + 255) & ~static_cast<size_t>(255);
+    return o;
+}
+
+constexpr size_t SCR_TOTAL = scr_offset(R_COUNT);  // about 27 MB
+
+static __device__ __align__(256) unsigned char scratch[SCR_TOTAL];
+
+// Staged host inputs (array-of-structures states, alive bytes, sizes) into
+// the structure-of-arrays planes, alive flags and collision radii the
+// screening kernels read; radius as upload_objects computes it.
+__global__ void __launch_bounds__(BLOCK) k_stage(const double* before, const double* after, const uint8_t* alive_in,
+                                                 const double* size_in, int have_alive, int have_size, int n,
+                                                 double* s0, double* s1, uint8_t* alive, double* radius,
+                                                 double scale) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    constexpr size_t c = static_cast<size_t>(SCR_OBJECTS);
+    for (int k = 0; k < 6; ++k) {
+        s0[k * c + i] = before[6 * static_cast<size_t>(i) + k];
+        s1[k * c + i] = after[6 * static_cast<size_t>(i) + k];
+    }
+    alive[i] = have_alive ? (alive_in[i] ? 1 : 0) : 1;
+    const double size = have_size ? size_in[i] : 0.8;
+    radius[i] = 0.5 * size * 1e-3 * scale;
+}
+
+// The context's grid pointers and sizes, pointed at the scratch regions for
+// one allocation-free screening and swapped back afterwards, so the
+// screening kernels run unchanged on the array's memory.
+struct StaticView {
+    double* state[2];
+    int cur;
+    uint8_t* alive;
+    double* radius;
+    uint64_t* keys[2];
+    int* idx[2];
+    int *hist, *totals, *bcount;
+    uint64_t* cell_key;
+    int *cell_begin, *cell_count, *cell_of;
+    int2* ranges;
+    float4 *lrec, *rec;
+    int4* coltab;
+    GridState* gs;
+    StepPartial* step_parts;
+    PairPartial *pair_parts, *dense_parts, *exact_parts;
+    uint2* surv;
+    int64_t surv_cap;
+    DenseCell* dense;
+    PairItem* items;
+    int64_t item_cap;
+    DevEvent* events;
+    int64_t ev_cap;
+    DevStats* stats;
+    int64_t stats_cap, cap, ecap, n, n_launch;
+    int passes, sbits;
+    int* scen;
+    bool timing;
+};
+
+void swap_view(khip_ctx* c, StaticView& v) {
+    using std::swap;
+    swap(c->state[0], v.state[0]); swap(c->state[1], v.state[1]); swap(c->cur, v.cur);
+    swap(c->alive, v.alive); swap(c->radius, v.radius);
+    swap(c->keys[0], v.keys[0]); swap(c->keys[1], v.keys[1]); swap(c->idx[0], v.idx[0]); swap(c->idx[1], v.idx[1]);
+    swap(c->hist, v.hist); swap(c->totals, v.totals); swap(c->bcount, v.bcount);
+    swap(c->cell_key, v.cell_key); swap(c->cell_begin, v.cell_begin); swap(c->cell_count, v.cell_count);
+    swap(c->cell_of, v.cell_of); swap(c->ranges, v.ranges); swap(c->lrec, v.lrec); swap(c->rec, v.rec);
+    swap(c->coltab, v.coltab); swap(c->gs, v.gs); swap(c->step_parts, v.step_parts);
+    swap(c->pair_parts, v.pair_parts); swap(c->dense_parts, v.dense_parts); swap(c->exact_parts, v.exact_parts);
+    swap(c->surv, v.surv); swap(c->surv_cap, v.surv_cap); swap(c->dense, v.dense); swap(c->items, v.items);
+    swap(c->item_cap, v.item_cap); swap(c->events, v.events); swap(c->ev_cap, v.ev_cap);
+    swap(c->stats, v.stats); swap(c->stats_cap, v.stats_cap); swap(c->cap, v.cap); swap(c->ecap, v.ecap);
+    swap(c->n, v.n); swap(c->n_launch, v.n_launch); swap(c->passes, v.passes); swap(c->sbits, v.sbits);
+    swap(c->scen, v.scen); swap(c->timing, v.timing);
+}
+
+// khip_screen_static's work. `code` receives the C-ABI status.
+bool screen_static(khip_ctx* ctx, int64_t n, const double* before, const double* after, const uint8_t* alive_after,
+                   const double* size_m, double t0, double dt, khip_event* events, int64_t cap, int64_t* nevents,
+                   khip_stats* stats, int& code) {
+    code = KHIP_ERR_HIP;
+    hipStream_t s = ctx->stream;
+    const double thr = ctx->cfg.threshold_km;
+    if (n == 0) {
+        if (nevents) *nevents = 0;
+        if (stats) *stats = khip_stats{0, 0, 0, std::max(1.0, thr), 0, 0, 0, 0, 0};
+        code = KHIP_OK;
+        return true;
+    }
+    unsigned char* b = ctx->scratch;
+    auto at = [&](int r) { return b + scr_offset(r); };
+    KHIP_TRY(ctx, hipMemcpyAsync(at(R_IN_BEFORE), before, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    KHIP_TRY(ctx, hipMemcpyAsync(at(R_IN_AFTER), after, sizeof(double) * 6 * n, hipMemcpyHostToDevice, s));
+    if (alive_after) KHIP_TRY(ctx, hipMemcpyAsync(at(R_IN_ALIVE), alive_after, n, hipMemcpyHostToDevice, s));
+    if (size_m) KHIP_TRY(ctx, hipMemcpyAsync(at(R_IN_SIZE), size_m, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    const int ni = static_cast<int>(n);
+    k_stage<<<grid_of(n), BLOCK, 0, s>>>(
+        reinterpret_cast<const double*>(at(R_IN_BEFORE)), reinterpret_cast<const double*>(at(R_IN_AFTER)),
+        at(R_IN_ALIVE), reinterpret_cast<const double*>(at(R_IN_SIZE)), alive_after ? 1 : 0, size_m ? 1 : 0, ni,
+        reinterpret_cast<double*>(at(R_STATE0)), reinterpret_cast<double*>(at(R_STATE1)), at(R_ALIVE),
+        reinterpret_cast<double*>(at(R_RADIUS)), ctx->cfg.radius_scale);
+    StaticView v;
+    v.state[0] = reinterpret_cast<double*>(at(R_STATE0));
+    v.state[1] = reinterpret_cast<double*>(at(R_STATE1));
+    v.cur = 0;
+    v.alive = at(R_ALIVE);
+    v.radius = reinterpret_cast<double*>(at(R_RADIUS));
+    v.keys[0] = reinterpret_cast<uint64_t*>(at(R_KEYS0));
+    v.keys[1] = reinterpret_cast<uint64_t*>(at(R_KEYS1));
+    v.idx[0] = reinterpret_cast<int*>(at(R_IDX0));
+    v.idx[1] = reinterpret_cast<int*>(at(R_IDX1));
+    v.hist = reinterpret_cast<int*>(at(R_HIST));
+    v.totals = reinterpret_cast<int*>(at(R_TOTALS));
+    v.bcount = reinterpret_cast<int*>(at(R_BCOUNT));
+    v.cell_key = reinterpret_cast<uint64_t*>(at(R_CELL_KEY));
+    v.cell_begin = reinterpret_cast<int*>(at(R_CELL_BEGIN));
+    v.cell_count = reinterpret_cast<int*>(at(R_CELL_COUNT));
+    v.cell_of = reinterpret_cast<int*>(at(R_CELL_OF));
+    v.ranges = reinterpret_cast<int2*>(at(R_RANGES));
+    v.lrec = reinterpret_cast<float4*>(at(R_LREC));
+    v.rec = reinterpret_cast<float4*>(at(R_REC));
+    v.coltab = nullptr;  // the column table (256 MB) stays out: neighbour runs by binary search
+    v.gs = reinterpret_cast<GridState*>(at(R_GS));
+    v.step_parts = reinterpret_cast<StepPartial*>(at(R_STEP_PARTS));
+    v.pair_parts = reinterpret_cast<PairPartial*>(at(R_PAIR_PARTS));
+    v.dense_parts = reinterpret_cast<PairPartial*>(at(R_DENSE_PARTS));
+    v.exact_parts = reinterpret_cast<PairPartial*>(at(R_EXACT_PARTS));
+    v.surv = reinterpret_cast<uint2*>(at(R_SURV));
+    v.surv_cap = SCR_SURVIVORS;
+    v.dense = reinterpret_cast<DenseCell*>(at(R_DENSE));
+    v.items = reinterpret_cast<PairItem*>(at(R_ITEMS));
+    v.item_cap = SCR_ITEMS;
+    v.events = reinterpret_cast<DevEvent*>(at(R_EVENTS));
+    v.ev_cap = SCR_EVENTS;
+    v.stats = reinterpret_cast<DevStats*>(at(R_STATS));
+    v.stats_cap = 1;
+    v.cap = SCR_OBJECTS;
+    v.ecap = SCR_OBJECTS;
+    v.n = n;
+    v.n_launch = n;
+    v.passes = ctx->static_passes;
+    v.sbits = 0;
+    v.scen = nullptr;
+    v.timing = false;
+    const TileCfg tile0 = ctx->tile;
+    const TileCfg defaults{1.0, DENSE_MIN, LONG_RANGE, SEG, DENSE_BLOCKS, EXACT_BLOCKS, 0};
+    GridState g{};
+    bool ok = true;
+    swap_view(ctx, v);
+    ctx->tile = defaults;
+    for (;;) {
+        // Every field a screening starts from is cleared here: the array
+        // keeps the previous call's contents.
+        k_reset<<<1, 64, 0, s>>>(ctx->gs);
+        k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+        k_reduce<<<part_grid(n), BLOCK, 0, s>>>(ctx->planes(0), ctx->alive, ni, ctx->step_parts);
+        ok = enqueue_screen(ctx, ctx->planes(0), ctx->planes(1), t0, 0, 0, dt, thr, 0);
+        if (ok) ok = hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, s) == hipSuccess &&
+                     hipStreamSynchronize(s) == hipSuccess;
+        if (!ok || !g.passes_short) break;
+        ctx->passes = std::min(MAX_PASSES, std::max(g.passes_max, ctx->passes + 1));  // more key bits: again
+    }
+    const int passes_used = ctx->passes;
+    swap_view(ctx, v);
+    ctx->tile = tile0;
+    ctx->static_passes = passes_used;
+    if (!ok) {
+        if (ctx->error.empty()) ctx->error = "khip_screen_static: a HIP call failed";
+        return false;
+    }
+    code = KHIP_ERR_STATE;
+    if (g.surv_short || g.items_short || static_cast<int64_t>(g.events) > SCR_EVENTS) {
+        ctx->error = "khip_screen_static: this screening exceeds the fixed buffers (" +
+                     std::to_string(g.events) + " events, " + std::to_string(g.surv_max) +
+                     " pre-filter survivors, " + std::to_string(g.items_max) + " work items); use khip_screen";
+        return false;
+    }
+    const int64_t nev = static_cast<int64_t>(g.events);
+    if (nevents) *nevents = nev;
+    if (nev > cap || (nev > 0 && !events)) {
+        code = KHIP_ERR_ARG;
+        ctx->error = "khip_screen_static: the event buffer holds " + std::to_string(cap) + " events, the screening found " +
+                     std::to_string(nev);
+        return false;
+    }
+    code = KHIP_ERR_HIP;
+    if (nev > 0)
+        KHIP_TRY(ctx, hipMemcpy(events, at(R_EVENTS), sizeof(khip_event) * nev, hipMemcpyDeviceToHost));
+    // (t, i, j) order (collision.cpp:739-747), in place in the caller's buffer.
+    std::sort(events, events + nev, [](const khip_event& a, const khip_event& b2) {
+        if (a.t != b2.t) return a.t < b2.t;
+        if (a.i != b2.i) return a.i < b2.i;
+        return a.j < b2.j;
+    });
+    if (stats) KHIP_TRY(ctx, hipMemcpy(stats, at(R_STATS), sizeof(khip_stats), hipMemcpyDeviceToHost));
+    code = KHIP_OK;
+    return true;
+}
+
 int status(khip_ctx* ctx, bool ok) { return ok ? KHIP_OK : KHIP_ERR_HIP; }
 
 }  // namespace
+
+// Device address of `scratch` per device, looked up once per device.
+static bool scratch_address(int device, unsigned char** out) {
+    static std::mutex lock;
+    static std::vector<unsigned char*> addr;
+    std::lock_guard<std::mutex> hold(lock);
+    if (device >= static_cast<int>(addr.size())) addr.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return false;
+        addr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = addr[device];
+    return true;
+}
 
 extern "C" {
 
@@ -4274,6 +4547,10 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
         k_zero_events<<<1, 64, 0, ctx->stream>>>(ctx->gs);
         ok = hipStreamSynchronize(ctx->stream) == hipSuccess;
     }
+    // This device's copy of the allocation-free screening's array (the
+    // symbol itself is a host-side stand-in): looked up once per device,
+    // with the device current, and kept.
+    if (ok) ok = scratch_address(device, &ctx->scratch);
     if (!ok) {
         say("initialisation failed on " + ctx->name);
         khip_destroy(ctx);
@@ -4462,6 +4739,23 @@ int khip_screen(khip_ctx* ctx, int64_t n, const double* before, const double* af
     }
     ctx->dt = dt_saved;
     return status(ctx, ok);
+}
+
+int khip_screen_static(khip_ctx* ctx, int64_t n, const double* before, const double* after,
+                       const uint8_t* alive_after, const double* size_m, double t0, double dt,
+                       khip_event* events, int64_t cap, int64_t* nevents, khip_stats* stats) {
+    if (!ctx || n < 0 || (n > 0 && (!before || !after)) || !(dt > 0) || cap < 0) return KHIP_ERR_ARG;
+    if (n > SCR_OBJECTS) {
+        ctx->error = "khip_screen_static: " + std::to_string(n) + " objects, at most " + std::to_string(SCR_OBJECTS);
+        return KHIP_ERR_ARG;
+    }
+    if (!ctx->scratch) {
+        ctx->error = "khip_screen_static: the device array's address is unknown";
+        return KHIP_ERR_STATE;
+    }
+    int code = KHIP_OK;
+    screen_static(ctx, n, before, after, alive_after, size_m, t0, dt, events, cap, nevents, stats, code);
+    return code;
 }
 
 int khip_take_events(khip_ctx* ctx, khip_event* out, int64_t cap, int64_t* n) {
