@@ -274,8 +274,10 @@ int khip_set_graph(khip_ctx* ctx, int enabled);
  * n, n + 1, ...) without a host round trip. The draws come from a
  * counter-based generator, not the reference's mt19937_64 sequence: the
  * fragments follow the model's distributions, not the reference's exact
- * draws (the host restatement in kessler_host.h keeps those). Plain and
- * mixed forms only; the learned form returns an error while enabled. */
+ * draws (the host restatement in kessler_host.h keeps those). All three
+ * forms: in the learned form the collisions are those the plain
+ * re-verification confirms, the parents come from the 60 s step's cubic,
+ * and each fragment starts on the plain method until the next check. */
 typedef struct {
     int32_t enabled;
     int32_t reserve;             /* object slots reserved per chunk (grown when short) */

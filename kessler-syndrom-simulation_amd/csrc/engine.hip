@@ -2270,7 +2270,7 @@ __global__ void k_mark_events(GridState* gs, unsigned long long cap) {
 }
 
 // ---------------------------------------------------------------------------
-// In-run breakups (plain and mixed forms): collisions of the step -> jobs ->
+// In-run breakups (every form): collisions of the step -> jobs ->
 // fragment samples -> mass budgets -> new objects in reserved slots
 // ---------------------------------------------------------------------------
 
@@ -2285,7 +2285,8 @@ __global__ void __launch_bounds__(BLOCK) k_bu_events(const GridState* gs, const 
     __syncthreads();
     const unsigned long long e0 = gs->ev_mark, e1 = gs->events < cap ? gs->events : cap;
     for (unsigned long long e = e0 + threadIdx.x; e < e1; e += BLOCK) {
-        if (!events[e].collision) continue;
+        // pad 1: a learned-form event the plain re-verification rejected.
+        if (!events[e].collision || events[e].pad == 1) continue;
         const int k = atomicAdd(&s_n, 1);
         if (k < bu::MAX_JOBS) s_idx[k] = e;
     }
@@ -2503,12 +2504,20 @@ __global__ void k_bu_bases(bu::BreakupState* st, bu::Job* jobs, bu::LogRecord* l
 
 // New objects: birth at the parent's collision state plus delta-v, RK4 over
 // the rest of the step (Moon from the step's samples), object arrays.
+// moon: the Moon at the step's start, middle and end, `mstride` entries
+// apart. Learned form (mode != nullptr): each fragment starts on the plain
+// method, appended to the plain list (plain_list, counts[CLASS_PLAIN]) for
+// the steps up to the next check, and its state goes to both state buffers
+// (pb too): objects outside the lists, as a fragment born below the surface
+// is, are not copied forward step by step.
 __global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, const bu::Job* __restrict__ jobs,
                                                      const bu::Sample* __restrict__ samples, int n0, int reserve,
                                                      Planes pa, uint8_t* alive, double* radius, double* size_m,
-                                                     double* mass, int* scen, const MoonSample* __restrict__ moon3,
-                                                     double t0, double dt, bu::BreakupCfg cfg) {
+                                                     double* mass, int* scen, const MoonSample* __restrict__ moon,
+                                                     int mstride, double t0, double dt, bu::BreakupCfg cfg,
+                                                     uint8_t* mode, int* plain_list, int* counts, Planes pb) {
     const int ns = st->nsamples, nj = st->njobs;
+    const MoonSample moon3[3] = {moon[0], moon[mstride], moon[2 * mstride]};
     for (int f = blockIdx.x * BLOCK + threadIdx.x; f < ns; f += gridDim.x * BLOCK) {
         const bu::Sample sm = samples[f];
         if (sm.rank < 0) continue;
@@ -2531,11 +2540,22 @@ __global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, 
         pa.vx[slot] = v.x;
         pa.vy[slot] = v.y;
         pa.vz[slot] = v.z;
-        alive[slot] = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) >= dev::R_EARTH ? 1 : 0;
+        const bool up = sqrt(r.x * r.x + r.y * r.y + r.z * r.z) >= dev::R_EARTH;
+        alive[slot] = up ? 1 : 0;
         radius[slot] = 0.5 * sm.L * 1e-3 * cfg.radius_scale;
         size_m[slot] = sm.L;
         mass[slot] = sm.m;
         if (scen) scen[slot] = jb.scen;
+        if (mode) {
+            mode[slot] = net::CLASS_PLAIN;
+            if (up) plain_list[atomicAdd(&counts[net::CLASS_PLAIN], 1)] = slot;
+            pb.x[slot] = r.x;
+            pb.y[slot] = r.y;
+            pb.z[slot] = r.z;
+            pb.vx[slot] = v.x;
+            pb.vy[slot] = v.y;
+            pb.vz[slot] = v.z;
+        }
     }
 }
 
@@ -2588,7 +2608,7 @@ struct khip_ctx {
     float4* lrec = nullptr;  // compact records in sorted order, 2 float4 each
     TileCfg tile{1.0, DENSE_MIN, LONG_RANGE, SEG, DENSE_BLOCKS, EXACT_BLOCKS, 0};
 
-    // In-run breakups (plain and mixed forms).
+    // In-run breakups.
     khip_breakup_config bcfg{0, 65536, 0.1, 40.0, 1};
     int64_t n_launch = 0;          // objects the per-step kernels cover (n + reserved slots)
     double* mass = nullptr;        // kg per object (0: the model's fallback)
@@ -3277,7 +3297,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                         k_bu_write<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
                                                                   static_cast<int>(n0), reserve, pa, ctx->alive,
                                                                   ctx->radius, ctx->size_m, ctx->mass, sc,
-                                                                  ctx->moon + 2 * k, t, ctx->dt, bc);
+                                                                  ctx->moon + 2 * k, 1, t, ctx->dt, bc, nullptr,
+                                                                  nullptr, nullptr, pb);
                     }
                 } else {
                     k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0,
@@ -3379,6 +3400,30 @@ bool ensure_learned(khip_ctx* ctx) {
     const int sub = ctx->lcfg.substeps;
     if (ctx->lrn_cap >= ctx->cap && ctx->mode && ctx->dense_k >= sub - 1) return true;
     const size_t c = static_cast<size_t>(ctx->cap);
+    if (ctx->mode && ctx->lrn_cap < ctx->cap) {
+        // The population outgrew the arrays (breakup room): the methods and
+        // lists move to the new capacity as they are, so results do not
+        // depend on when the room grew.
+        const size_t c0 = static_cast<size_t>(ctx->lrn_cap);
+        uint8_t* mode = nullptr;
+        int* lists = nullptr;
+        KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&mode), c));
+        KHIP_TRY(ctx, hipMalloc(reinterpret_cast<void**>(&lists), sizeof(int) * 3 * c));
+        KHIP_TRY(ctx, hipMemset(mode, net::CLASS_PLAIN, c));
+        KHIP_TRY(ctx, hipMemcpy(mode, ctx->mode, c0, hipMemcpyDeviceToDevice));
+        for (int k = 0; k < 3; ++k)
+            KHIP_TRY(ctx, hipMemcpy(lists + k * c, ctx->lists + k * c0, sizeof(int) * c0, hipMemcpyDeviceToDevice));
+        (void)hipFree(ctx->mode);
+        (void)hipFree(ctx->lists);
+        ctx->mode = mode;
+        ctx->lists = lists;
+        ctx->gen += 1;
+        if (!dev_alloc(ctx, &ctx->snap_mode, c) || !dev_alloc(ctx, &ctx->snap_lists, 3 * c + 3) ||
+            !dev_alloc(ctx, &ctx->substates, static_cast<size_t>(std::max(1, sub - 1)) * 6 * c))
+            return false;
+        ctx->lrn_cap = ctx->cap;
+        if (ctx->dense_k >= sub - 1) return true;
+    }
     if (!dev_alloc(ctx, &ctx->mode, c) || !dev_alloc(ctx, &ctx->snap_mode, c) ||
         !dev_alloc(ctx, &ctx->lists, 3 * c) || !dev_alloc(ctx, &ctx->snap_lists, 3 * c + 3) ||
         !dev_alloc(ctx, &ctx->counts, 3) || !dev_alloc(ctx, &ctx->cstats, 1) || !dev_alloc(ctx, &ctx->rstats, 1) ||
@@ -3478,11 +3523,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         ctx->error = "learned form: substeps must divide 12";
         return false;
     }
-    if (ctx->bcfg.enabled) {
-        ctx->error = "learned form: in-run breakups run in the plain and mixed forms only";
-        return false;
-    }
-    const int n = static_cast<int>(ctx->n);
+    // In-run breakups act on the collisions the screening confirms.
+    const bool bk = ctx->bcfg.enabled && screen;
+    int n = static_cast<int>(ctx->n);
     ctx->n_launch = ctx->n;
     hipStream_t s = ctx->stream;
     const int sub = L.substeps;
@@ -3502,24 +3545,17 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         KHIP_TRY(ctx, hipMemsetAsync(ctx->cstats, 0, sizeof(CheckStats), s));
         ctx->reset_checks = false;
     }
+    if (bk && !ensure_breakup(ctx)) return false;
+    if (bk && ctx->reset_breakups) {
+        KHIP_TRY(ctx, hipMemsetAsync(ctx->bstate, 0, sizeof(bu::BreakupState), s));
+        ctx->reset_breakups = false;
+    }
     const int nw = learned_windows(L);
     const int spw = sub / nw;
     const double hw = ctx->dt / nw;
     // Radial shells: with the fused window pass only (its reductions count
     // the entries), and within the index word's object range.
     const bool shells = screen && L.shell_km > 0 && nw <= FUSED_WINDOWS_MAX;
-    if (shells) {
-        if (n >= (1 << OBJ_BITS)) {
-            ctx->error = "shell screening: more objects than the entry index word holds";
-            return false;
-        }
-        if (!ensure_shells(ctx)) return false;
-        if (ctx->elaunch < n) {
-            const int64_t want = n + n / 2 + TILE;
-            if (!ensure_sort_capacity(ctx, want)) return false;
-            ctx->elaunch = want;
-        }
-    }
     LearnedParams lp;
     lp.tau = ctx->dt;
     lp.substeps = sub;
@@ -3531,9 +3567,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     const double thr = ctx->cfg.threshold_km + L.margin_km;
     const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
     const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
-    const size_t c = static_cast<size_t>(ctx->cap);
+    size_t c = static_cast<size_t>(ctx->cap);
     const int g_net = ctx->n_cus;
-    const int g_all = std::min(grid_of(n), 4 * ctx->n_cus);
+    int g_all = std::min(grid_of(n), 4 * ctx->n_cus);
     int64_t done = 0, next_len = 1;
     while (done < nsteps) {
         const int64_t len = std::min<int64_t>(std::min<int64_t>(next_len, ctx->chunk), nsteps - done);
@@ -3542,6 +3578,51 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             !ensure_events(ctx, std::max<int64_t>(ctx->ev_cap, kInitialEvents)) ||
             (ctx->timing && !ensure_event_pool(ctx, len * sub)) || !upload_moon12(ctx, ctx->t, len))
             return false;
+        // Breakup room, as in the plain form: `reserve` dead slots past the
+        // population, taken in order by this chunk's fragments. A grown
+        // capacity carries the learned arrays over (ensure_learned).
+        const int64_t n0 = ctx->n;
+        int reserve = 0;
+        if (bk) {
+            reserve = ctx->bu_reserve;
+            if (!grow_preserve(ctx, n0 + reserve)) return false;
+            if (n0 + reserve >= (1LL << 31) - TILE) {
+                ctx->error = "breakup room exceeds the object index range";
+                return false;
+            }
+            if (!ensure_learned(ctx)) return false;
+            KHIP_TRY(ctx, hipMemsetAsync(ctx->alive + n0, 0, static_cast<size_t>(reserve), s));
+        }
+        n = static_cast<int>(n0 + reserve);
+        ctx->n_launch = n;
+        c = static_cast<size_t>(ctx->cap);
+        g_all = std::min(grid_of(n), 4 * ctx->n_cus);
+        if (shells) {
+            if (n >= (1 << OBJ_BITS)) {
+                ctx->error = "shell screening: more objects than the entry index word holds";
+                return false;
+            }
+            if (!ensure_shells(ctx)) return false;
+            if (ctx->elaunch < n) {
+                const int64_t want = n + n / 2 + TILE;
+                if (!ensure_sort_capacity(ctx, want)) return false;
+                ctx->elaunch = want;
+            }
+        }
+        bu::BreakupCfg bc{};
+        if (bk) {
+            bc.lc_m = ctx->bcfg.lc_m;
+            bc.catastrophic_j_per_g = ctx->bcfg.catastrophic_j_per_g;
+            bc.radius_scale = ctx->cfg.radius_scale;
+            bc.seed = ctx->bcfg.seed;
+            bc.reserve = reserve;
+            bc.sample_cap = ctx->sample_cap;
+            k_bu_chunk_begin<<<1, 64, 0, s>>>(ctx->bstate);
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_mass, ctx->mass, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_bstate, ctx->bstate, sizeof(bu::BreakupState),
+                                         hipMemcpyDeviceToDevice, s));
+        }
+        bool retry_chunk = false;  // breakup room grown: the chunk runs again from the top
         KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
                                      hipMemcpyDeviceToDevice, s));
         KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_alive, ctx->alive, c, hipMemcpyDeviceToDevice, s));
@@ -3647,6 +3728,27 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                                   static_cast<unsigned long long>(ctx->ev_cap), pc,
                                                                   ctx->radius, m13, info, h, sub,
                                                                   ctx->cfg.threshold_km, vmin2, ctx->rstats);
+                        if (bk) {
+                            // The step's confirmed collisions, in (t, i, j) order:
+                            // parents at the collision instant from the step's
+                            // cubic, fragments stepped to its end (Moon: the
+                            // step's start, middle and end samples, 6 entries
+                            // apart), started on the plain method.
+                            const double t_step = t_chunk + static_cast<double>(k) * ctx->dt;
+                            int* sc = ctx->sbits ? ctx->scen : nullptr;
+                            k_bu_events<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->events,
+                                                            static_cast<unsigned long long>(ctx->ev_cap), pc, pn,
+                                                            ctx->alive, ctx->mass, ctx->size_m, sc, t_step, ctx->dt,
+                                                            bc, ctx->bstate, ctx->jobs, ctx->blogs, ctx->blog_cap);
+                            k_bu_sample<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples, bc);
+                            k_bu_budget<<<bu::MAX_JOBS / 64, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
+                                                                          ctx->blogs);
+                            k_bu_bases<<<1, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->blogs, reserve);
+                            k_bu_write<<<EXACT_BLOCKS, BLOCK, 0, s>>>(
+                                ctx->bstate, ctx->jobs, ctx->samples, static_cast<int>(n0), reserve, pn, ctx->alive,
+                                ctx->radius, ctx->size_m, ctx->mass, sc, m13, 6, t_step, ctx->dt, bc, ctx->mode,
+                                ctx->lists + c, ctx->counts, pc);
+                        }
                         mark(ctx, 0);
                     } else {
                         k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,
@@ -3656,7 +3758,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 }
                 return true;
             };
-            if (ctx->use_graph && !ctx->timing) {
+            if (ctx->use_graph && !ctx->timing && !bk) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
                                             shells ? ctx->elaunch : 0};
                 key.insert(key.end(), pattern.begin(), pattern.end());
@@ -3698,7 +3800,42 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
             if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
+            bu::BreakupState bs{};
+            if (bk) {
+                KHIP_TRY(ctx, hipMemcpy(&bs, ctx->bstate, sizeof(bs), hipMemcpyDeviceToHost));
+                if (bs.jobs_short || bs.samples_short || bs.reserve_short) {
+                    if (!overflow) {
+                        // drain_chunk kept this chunk's events and statistics: drop them.
+                        ctx->host_events.resize(ctx->host_events.size() - ctx->last_chunk_events);
+                        ctx->host_stats.resize(ctx->host_stats.size() - static_cast<size_t>(len * sub));
+                    }
+                    overflow = true;
+                    retry_chunk = true;
+                }
+            }
             if (!overflow) {
+                if (bk) {
+                    const int used = std::min(bs.frag_used, reserve);
+                    const int nl = std::min(bs.nlog, ctx->blog_cap);
+                    std::vector<bu::LogRecord> lg(static_cast<size_t>(nl));
+                    if (nl > 0)
+                        KHIP_TRY(ctx, hipMemcpy(lg.data(), ctx->blogs, sizeof(bu::LogRecord) * nl,
+                                                hipMemcpyDeviceToHost));
+                    for (const bu::LogRecord& r : lg) {
+                        khip_breakup_event e;
+                        std::memcpy(&e, &r, sizeof(e));
+                        e.first_index = static_cast<int32_t>(n0) + r.first;
+                        ctx->host_breakups.push_back(e);
+                    }
+                    if (used > 0) {
+                        std::vector<double> sz(static_cast<size_t>(used));
+                        KHIP_TRY(ctx, hipMemcpy(sz.data(), ctx->size_m + n0, sizeof(double) * used,
+                                                hipMemcpyDeviceToHost));
+                        ctx->host_size.insert(ctx->host_size.end(), sz.begin(), sz.end());
+                        ctx->n = n0 + used;
+                    }
+                    ctx->n_launch = ctx->n;
+                }
                 if (ctx->timing) accumulate_timing_learned(ctx, len, sub, learned_windows(ctx->lcfg), screen != 0);
                 ctx->cur = cur;
                 ctx->t = t;
@@ -3732,6 +3869,26 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->counts, ctx->snap_lists + 3 * c, sizeof(int) * 3,
                                          hipMemcpyDeviceToDevice, s));
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->cstats, &cs0, sizeof(cs0), hipMemcpyHostToDevice, s));
+            if (bk) {
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->mass, ctx->snap_mass, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->bstate, ctx->snap_bstate, sizeof(bu::BreakupState),
+                                             hipMemcpyDeviceToDevice, s));
+            }
+            if (retry_chunk) {
+                if (bs.reserve_short) ctx->bu_reserve = std::max(2 * ctx->bu_reserve, bs.frag_used + 4096);
+                if (bs.samples_short) {
+                    const int want = std::max(2 * ctx->sample_cap, bs.nsamples + (1 << 16));
+                    if (!dev_alloc(ctx, &ctx->samples, static_cast<size_t>(want))) return false;
+                    ctx->sample_cap = want;
+                }
+                ctx->tm.rerun_reasons |= KHIP_RERUN_BREAKUPS;
+                KHIP_TRY(ctx, hipStreamSynchronize(s));
+                break;
+            }
+        }
+        if (retry_chunk) {
+            next_len = len;
+            continue;
         }
         done += len;
     }
@@ -3940,7 +4097,8 @@ bool grow_preserve(khip_ctx* ctx, int64_t want) {
         KHIP_TRY(ctx, hipMemcpy(ctx->scen, sc.data(), sizeof(int) * c, hipMemcpyHostToDevice));
     }
     ctx->cur = 0;
-    ctx->force_check = true;
+    // The learned form's methods and lists move with the population
+    // (ensure_learned), so the check pattern does not depend on growth.
     return true;
 }
 

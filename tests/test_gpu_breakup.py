@@ -1,4 +1,6 @@
-"""GPU tier: in-run breakups on the device (khip_set_breakup).
+"""GPU tier: in-run breakups on the device (khip_set_breakup), in the plain
+form and in the learned form (60 s steps; collisions confirmed by the plain
+re-verification; fragments start on the plain method).
 
 Scene: pairs of LEO objects set on head-on-ish courses (relative speed
 10 km/s) that meet half-way through the first 10 s step, far from each
@@ -54,14 +56,15 @@ def _scene(npairs=200, seed=3, masses_kg=(1000.0, 500.0)):
     return np.array(states), np.array(sizes), np.array(masses)
 
 
-def _run(engine, steps=3, chunk=256, reserve=65536, seed=9, npairs=200, masses_kg=(1000.0, 500.0)):
+def _run(engine, steps=3, chunk=256, reserve=65536, seed=9, npairs=200, masses_kg=(1000.0, 500.0), precision=0,
+         dt=10.0):
     s, size, mass = _scene(npairs, masses_kg=masses_kg)
-    engine.set_precision(0)
+    engine.set_precision(precision)
     engine.set_chunk(chunk)
     engine.set_objects(s, np.ones(len(s), np.uint8), size)
     engine.set_masses(mass)
     engine.set_config(5.0, 1000.0, 0.01)
-    engine.set_time(EPOCH, 0.0, 10.0)
+    engine.set_time(EPOCH, 0.0, dt)
     engine.set_breakup(True, reserve=reserve, lc_m=LC, seed=seed)
     try:
         engine.run(steps, screen=True)
@@ -144,3 +147,59 @@ def test_breakups_are_deterministic_and_chunk_independent(engine):
     st = a[6]
     print(f"candidate pairs per step: {st['candidate_pairs'].tolist()}, refined {st['refined_pairs'].tolist()}")
     assert st["candidate_pairs"][0] < 10000 and st["candidate_pairs"][1] > 1_000_000
+
+
+def test_learned_form_breakups_match_plain(engine, trained):
+    """The same 40 collisions in the learned form (one 60 s step, which is a
+    check step: every object takes the plain method) and in the plain form
+    (six 10 s steps): the same collisions in the same order give the same
+    breakup records, the same fragment draws (sizes, masses), and fragment
+    states within the learned form's 10 m bound (parents at the collision
+    from the 60 s step's cubic, whose velocity is off by about a cm/s a few
+    seconds into the step; fragments stepped to the end of the 60 s step in
+    one RK4 step)."""
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    kw = dict(npairs=40, masses_kg=(10000.0, 5000.0))
+    sp, ap, evp, bkp, szp, msp, _ = _run(engine, steps=6, **kw)
+    sl, al, evl, bkl, szl, msl, stl = _run(engine, steps=2, precision=2, dt=60.0, **kw)
+    print(f"plain: {len(bkp)} breakups, {int(bkp['fragments'].sum())} fragments; learned: {len(bkl)} breakups, "
+          f"{int(bkl['fragments'].sum())} fragments")
+    assert len(bkp) == len(bkl) == 40
+    for f in ("id", "target", "projectile", "catastrophic", "drawn", "fragments", "escaped", "first_index"):
+        assert np.array_equal(bkp[f], bkl[f]), f
+    assert np.allclose(bkp["expected_fragments"], bkl["expected_fragments"], rtol=1e-12)
+    assert np.allclose(bkp["t"], bkl["t"], atol=1e-6)
+    assert len(sp) == len(sl)
+    n0 = 80
+    assert np.array_equal(szp[n0:], szl[n0:]) and np.array_equal(msp[n0:], msl[n0:])
+    # The states at one instant: 120 s, two learned steps and twelve plain ones.
+    sp2, ap2, _, _, _, _, _ = _run(engine, steps=12, **kw)
+    print(f"alive after 120 s: plain {int(ap2.sum())}, learned {int(al.sum())}, differing {int((ap2 != al).sum())}")
+    assert np.array_equal(ap2, al)
+    both = (ap2 == 1) & (al == 1)
+    d = np.linalg.norm(sp2[both, :3] - sl[both, :3], axis=1) * 1e3
+    print(f"fragment positions after 120 s: |dr| max {d.max():.4f} m, median {np.median(d):.5f} m")
+    assert d.max() < 10.0
+    # The fragments are screened from the second step on (cloud siblings).
+    assert stl["candidate_pairs"][6:].sum() > 100 * max(1, stl["candidate_pairs"][:6].sum())
+
+
+def test_learned_form_breakups_deterministic_and_chunk_independent(engine, trained):
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    a = _run(engine, steps=4, chunk=256, reserve=65536, precision=2, dt=60.0)
+    b = _run(engine, steps=4, chunk=1, reserve=1000, precision=2, dt=60.0)  # the room grows several times
+    print(f"learned: {len(a[3])} breakups, {int(a[3]['fragments'].sum())} fragments, {len(a[2])} events")
+    assert len(a[3]) >= 200  # the 200 meetings, then collisions inside the clouds (radius scale 1000)
+    rows = np.flatnonzero(~(a[0] == b[0]).all(axis=1))
+    if len(rows):
+        first = np.searchsorted(np.sort(a[3]["first_index"]), rows, side="right")
+        print(f"rows differing: {len(rows)} (alive a/b: {int(a[1][rows].sum())}/{int(b[1][rows].sum())}; "
+              f"first {rows[:8].tolist()}; breakup ordinal {first[:8].tolist()}; max |dr| "
+              f"{np.abs(a[0][rows, :3] - b[0][rows, :3]).max():.3e} km)")
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert len(a[2]) == len(b[2]) and np.array_equal(np.sort(a[2], order=["step", "t", "i", "j"]),
+                                                     np.sort(b[2], order=["step", "t", "i", "j"]))
+    assert np.array_equal(a[3], b[3])
+    assert np.array_equal(a[6]["candidate_pairs"], b[6]["candidate_pairs"])
