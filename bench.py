@@ -13,6 +13,10 @@ their ratio, the 24 h position difference and the event recall.
 28,340-object synthetic catalogue plus 40 seeded catastrophic breakups at
 t = 0 (255,220 objects), dt 10 s, 5 km, 24 h.
 --workload precision: the mixed-precision form at 1,000,000 objects, 2 km.
+--workload catalogue: plain FP64 at catalogue scale, the 28,340-object
+synthetic catalogue of seed 0, dt 10 s, 5 km, 24 h, chunks replayed as HIP
+graphs, with the reference's CPU path run over the same full day beside it
+(events and final states compared bit for bit).
 
 value = object-steps/s = (sum over the timed steps of the objects
 propagated, counted in 10 s steps) / wall time of the timed steps, with the
@@ -158,6 +162,7 @@ def cpu_baseline(kamd, eng, pop, windows_h=(0.0, 6.0, 12.0, 18.0), budget_s=24.0
         "value": obj_steps / cpu_s,
         "unit": "object-steps/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": kind,
         "sample": (f"{steps_done} steps of the same workload ({'; '.join(sample)}), start states "
                    f"from the engine (bit-identical to the CPU path), {cpu_s:.1f} s of CPU loop time"),
@@ -471,13 +476,19 @@ def ensemble_workload(kamd, args):
     eng = kamd.Engine(0)
     train = train_corrector(eng, kamd, args.train_iters)
 
-    def setup(precision, dt):
+    bk_on = not args.no_breakups
+
+    def setup(precision, dt, breakups=False):
         eng.set_precision(precision)
         eng.set_graph(precision == kamd.PRECISION_LEARNED)
         eng.set_objects(pop["state"], alive0, pop["size_m"])
         eng.set_scenarios(off)
+        eng.set_masses(pop["mass_kg"])
         eng.set_config(thr, 1.0, 0.01)
         eng.set_time(EPOCH, 0.0, dt)
+        # In-run breakups in both forms: collisions fragment on the device
+        # (the draws are keyed by breakup id, so both forms draw alike).
+        eng.set_breakup(breakups, reserve=1 << 18, lc_m=0.1, seed=7)
 
     def run(precision, dt, nsteps, piece, warm):
         # Untimed warm-up of 10 simulated minutes, then the tiling is tuned
@@ -488,18 +499,35 @@ def ensemble_workload(kamd, args):
         eng.take_stats()
         eng.take_events()
         tune = eng.autotune(steps=4)
-        setup(precision, dt)
+        setup(precision, dt, breakups=bk_on)
         conj = np.zeros(64, np.int64)
         coll = np.zeros(64, np.int64)
+        nbk = np.zeros(64, np.int64)
+        nfrag = np.zeros(64, np.int64)
+        # Scenario of every object: fragments take their target's.
+        scen_of = [np.repeat(np.arange(64), np.diff(off))]
+        nscen = len(scen_of[0])
         objs, done, wall = 0, 0, 0.0
         while done < nsteps:
             k = min(piece, nsteps - done)
             t1 = time.perf_counter()
             eng.run(k, screen=True)
-            eng.take_stats()  # (per sub-step records; the rate uses the alive counts below)
+            st = eng.take_stats()
             wall += time.perf_counter() - t1
+            objs += int(st["propagated"].sum())  # per 10 s record in both forms
             ev = eng.take_events()
-            sc = np.searchsorted(off, ev["i"], side="right") - 1
+            bks = eng.take_breakups()
+            so = np.concatenate(scen_of)
+            for b in bks[np.argsort(bks["first_index"], kind="stable")]:
+                sct = so[b["target"]] if b["target"] < len(so) else -1
+                if b["fragments"] and b["first_index"] == nscen:
+                    scen_of.append(np.full(int(b["fragments"]), sct))
+                    nscen += int(b["fragments"])
+                    so = np.concatenate(scen_of)
+                if sct >= 0:
+                    nbk[sct] += 1
+                    nfrag[sct] += int(b["fragments"])
+            sc = so[ev["i"]]
             below = ev["miss_km"] < thr
             conj += np.bincount(sc[below], minlength=64)
             coll += np.bincount(sc[ev["collision"] == 1], minlength=64)
@@ -508,16 +536,14 @@ def ensemble_workload(kamd, args):
             print(f"ensemble {'learned' if precision == kamd.PRECISION_LEARNED else 'plain FP64'}: "
                   f"{done}/{nsteps} steps, {wall:.1f} s", file=sys.stderr, flush=True)
         s, a = eng.get_objects()
-        return dict(wall=wall, conj=conj, coll=coll, alive=a, tune=tune)
+        return dict(wall=wall, conj=conj, coll=coll, alive=a, tune=tune, objs=objs, breakups=nbk, fragments=nfrag)
 
     lrn = run(kamd.PRECISION_LEARNED, 60.0, steps, 120, 10)
     captures, launches = eng.graph_counts()
     plain = run(kamd.PRECISION_FP64, DT, 6 * steps, 720, 60)
-    # object-steps in 10 s steps: the alive population times the plain steps
-    # (the population changes only by re-entries, which the two forms share).
-    n_alive = int(alive0.sum())
-    obj = n_alive * 6 * steps
-    v_l, v_p = obj / lrn["wall"], obj / plain["wall"]
+    # object-steps in 10 s steps: the statistics records' propagated counts
+    # (one record per 10 s sub-step in the learned form).
+    v_l, v_p = lrn["objs"] / lrn["wall"], plain["objs"] / plain["wall"]
     out = {
         "metric": METRIC, "value": v_l, "unit": "object-steps/s", "n_gpus": 1, "steps": steps, "warmup": 10,
         "ms_per_step": 1e3 * lrn["wall"] / steps, "higher_is_better": True, "scaling": "weak",
@@ -530,12 +556,140 @@ def ensemble_workload(kamd, args):
         "scenario_objects": [int(x) for x in np.diff(off)], "scenario_breakups": [int(x) for x in breakups],
         "conjunctions_learned": lrn["conj"].tolist(), "conjunctions_plain_fp64": plain["conj"].tolist(),
         "collisions_learned": lrn["coll"].tolist(), "collisions_plain_fp64": plain["coll"].tolist(),
-        "alive_flags_equal": bool(np.array_equal(lrn["alive"], plain["alive"])),
+        "in_run_breakups": bk_on,
+        "breakups_learned": lrn["breakups"].tolist(), "breakups_plain_fp64": plain["breakups"].tolist(),
+        "fragments_learned": lrn["fragments"].tolist(), "fragments_plain_fp64": plain["fragments"].tolist(),
+        "alive_flags_equal": bool(len(lrn["alive"]) == len(plain["alive"]) and
+                                  np.array_equal(lrn["alive"], plain["alive"])),
         "graph": {"captures": captures, "launches": launches},
         "autotune_learned": lrn["tune"], "autotune_plain": plain["tune"],
         "training": train, "host_generation_s": gen_s, "device": eng.device_name,
     }
     eng.close()
+    print(json.dumps(out))
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def catalogue_workload(kamd, args):
+    """Plain FP64 at catalogue scale: 28,340 synthetic objects (the catalogue
+    orbit mix, seed 0), dt 10 s, 24 h (8,640 steps), screening at 5 km,
+    radius scale 1, v_min 10 m/s. At this size a step is a few microseconds
+    of device work per kernel, so the chunks run as captured HIP graphs. The
+    reference's compiled CPU path runs the same full day on this host's
+    cores; its events and final states must equal the engine's bit for bit."""
+    steps = args.steps if args.steps is not None else 8640
+    warm = args.warmup if args.warmup is not None else 20
+    thr = THRESHOLD_KM
+    pop = kamd.generate(28340, seed=0)
+    alive0 = pop["alive"].astype(np.uint8)
+    eng = kamd.Engine(0)
+
+    def reset():
+        eng.set_objects(pop["state"], alive0, pop["size_m"])
+        eng.set_config(thr, 1.0, 0.01)
+        eng.set_time(EPOCH, 0.0, DT)
+
+    eng.set_graph(True)
+    reset()
+    eng.run(warm, screen=True)
+    eng.take_events()
+    eng.take_stats()
+    reset()
+    c0, l0 = eng.graph_counts()
+    t0 = time.perf_counter()
+    eng.run(steps, screen=True)
+    wall = time.perf_counter() - t0
+    c1, l1 = eng.graph_counts()
+    ev, st = eng.take_events(), eng.take_stats()
+    s_gpu, a_gpu = eng.get_objects()
+    # Per-kernel device times: the same run with HIP event marks (direct
+    # launches; marks and graphs do not mix).
+    eng.set_graph(False)
+    reset()
+    eng.set_timing(True)
+    eng.reset_timing()
+    t1 = time.perf_counter()
+    eng.run(steps, screen=True)
+    wall_marked = time.perf_counter() - t1
+    eng.set_timing(False)
+    tm = eng.timing()
+    eng.take_events()
+    eng.take_stats()
+    obj_steps = int(st["propagated"].sum())
+    value = obj_steps / wall
+    n = len(pop)
+    cand = float(st["candidate_pairs"].mean())
+    kernels = {"k_propagate": float(tm["propagate_ms"]), "grid (11-13 kernels)": float(tm["grid_ms"]),
+               "k_pairs": float(tm["pairs_sparse_ms"]), "k_pairs_dense": float(tm["pairs_dense_ms"]),
+               "k_exact": float(tm["exact_ms"])}
+    per_step_ms = {k: v / max(1, int(tm["steps"])) for k, v in kernels.items()}
+    single = {k: v for k, v in kernels.items() if not k.startswith("grid")}
+    dominant = max(single, key=single.get)
+    ms = single[dominant]
+    if dominant == "k_propagate":
+        ach = BYTES_PER_OBJECT_STEP * obj_steps / (ms * 1e-3) / 1e9
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": ach / PEAK_HBM_GBS, "traffic": None, "unit_of_work": "propagated object (97 B)"}
+    else:
+        units = {"k_pairs": int(st["candidate_pairs"].sum() - st["dense_candidates"].sum()),
+                 "k_pairs_dense": int(st["dense_candidates"].sum()),
+                 "k_exact": int(st["survivors"].sum())}[dominant]
+        flop = FLOP_PER_CANDIDATE if dominant != "k_exact" else 35.0
+        ach = flop * units / (ms * 1e-3) / 1e12
+        peak = PEAK_FP32_TFLOPS if dominant != "k_exact" else PEAK_FP64_TFLOPS
+        roof = {"kernel": dominant, "bound": "valu-fp32" if dominant != "k_exact" else "valu-fp64",
+                "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
+                "unit_of_work": "candidate pair (60 FP32 flops)" if dominant != "k_exact"
+                else "pre-filter survivor (35 FP64 flops)"}
+    roof["avg_launch_us"] = 1e3 * ms / max(1, int(tm["steps"]))
+    roof["kernel_ms_per_step"] = per_step_ms
+    model = step_model(n, 1, n, cand, 0, wall / steps)
+    out = {
+        "metric": METRIC, "value": value, "unit": "object-steps/s", "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": (f"plain FP64 at catalogue scale: 28,340 synthetic objects (seed 0), dt 10 s, "
+                                f"{steps * DT / 3600:g} h, screening 5 km, chunks replayed as HIP graphs"),
+                   "objects": n, "dt_s": DT, "threshold_km": thr},
+        "ms_per_step_direct_launches_with_marks": 1e3 * wall_marked / steps,
+        "graphs": {"captures": c1 - c0, "launches": l1 - l0},
+        "events": int(len(ev)), "candidate_pairs_per_step": cand,
+        "roofline": roof, "step_roofline": model, "device": eng.device_name,
+    }
+    eng.close()
+    if not args.no_cpu:
+        from oracle.bind import oracle, reference
+
+        ref = reference()
+        kind = "reference"
+        if ref is None:
+            ref, kind = oracle(), "port"
+        cores = ref.threads(0)
+        rs, ra, rev, _, loop_s = ref.run(pop["state"], alive0, pop["size_m"], EPOCH, DT, steps, screen=True,
+                                         thr=thr, cap=1 << 22, per_step=False)
+        order = lambda e: e[np.lexsort((e["j"], e["i"], e["t"], e["step"]))]
+        ga, ra_ = order(ev), order(rev)
+        same = len(ga) == len(ra_) and all(np.array_equal(ga[f], ra_[f]) for f in
+                                           ("i", "j", "step", "t", "miss_km", "v_rel_km_s", "collision"))
+        out["cpu_baseline"] = {"value": obj_steps / loop_s, "unit": "object-steps/s", "cores": cores,
+                               "kind": kind, "cpu_model": cpu_model(),
+                               "sample": f"the same full run: {steps} steps of 10 s, screening at {thr:g} km, "
+                                         f"{loop_s:.1f} s of CPU loop time"}
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        out["parity_vs_cpu"] = {"events_equal": bool(same), "events_cpu": int(len(rev)),
+                                "final_states_equal": bool(np.array_equal(s_gpu, rs, equal_nan=True)),
+                                "alive_equal": bool(np.array_equal(a_gpu, ra))}
     print(json.dumps(out))
 
 
@@ -559,7 +713,7 @@ def cpu_baseline_window(kamd, pop, thr, budget_s=12.0, max_steps=400):
         spent += loop_s
         obj += n_alive
         k += 1
-    return {"value": obj / spent, "unit": "object-steps/s", "cores": cores, "kind": kind,
+    return {"value": obj / spent, "unit": "object-steps/s", "cores": cores, "kind": kind, "cpu_model": cpu_model(),
             "sample": f"the first {k} steps (10 s, plain FP64 with screening at {thr:g} km) of the same "
                       f"{len(pop):,}-object catalogue, {spent:.1f} s of CPU loop time"}
 
@@ -571,12 +725,15 @@ def main():
                     help="timed steps (learned: 60 s steps, default 1440; cascade: 10 s steps, default 8640)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-breakups", action="store_true", help="ensemble: collisions do not fragment")
     ap.add_argument("--train-iters", type=int, default=20000,
                     help="corrector training steps before the timed runs (fewer only for profiling runs)")
-    ap.add_argument("--workload", default="learned", choices=("learned", "cascade", "precision", "ensemble"),
+    ap.add_argument("--workload", default="learned",
+                    choices=("learned", "cascade", "precision", "ensemble", "catalogue"),
                     help="learned: the learned corrector at 1M objects vs plain FP64 (default, the headline "
                          "line); cascade: plain FP64 at cascade scale; precision: mixed precision at 1M; "
-                         "ensemble: 64 cascade scenarios in one concatenated population")
+                         "ensemble: 64 cascade scenarios in one concatenated population; catalogue: plain FP64 "
+                         "at 28,340 objects with the reference CPU path over the same day")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only: --gpus must be 1 "
@@ -589,6 +746,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble_workload(kamd, args)
+        return
+    if args.workload == "catalogue":
+        catalogue_workload(kamd, args)
         return
     if args.steps is None:
         args.steps = 8640 if args.workload == "cascade" else 360

@@ -255,11 +255,12 @@ typedef struct {
 } khip_learned_report;
 
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);
-/* Learned form: when enabled (and timing is off), each chunk of steps is
+/* When enabled (and timing is off), each screened chunk of steps is
  * captured once into a HIP graph and replayed for every later chunk of the
  * same shape (length, check pattern, radix passes, buffers); per-step times
- * and indices come from a device table, so replays need no re-capture.
- * No reference counterpart. */
+ * and indices come from a device table, so replays need no re-capture. All
+ * forms; not with in-run breakups, whose chunks launch directly. Results
+ * are identical to direct launches. No reference counterpart. */
 int khip_set_graph(khip_ctx* ctx, int enabled);
 
 /* ---- In-run breakups (cascade mode) ---------------------------------------

@@ -5,37 +5,50 @@
 // one step reads the start-of-step buffer and writes the end-of-step one;
 // the screening of that step then has both at hand with no host copy.
 //
-// One step is a fixed sequence of kernels on one stream:
+// One plain step is a fixed sequence of kernels on one stream:
 //
 //   k_propagate   fused four-stage RK4, one object per lane, coalesced SoA
 //                 loads/stores; the Moon samples of the step are uniform
-//                 (scalar) loads; reduces, over the objects alive at the end
-//                 of the step, max |v| and the bounding box of the
-//                 start-of-step positions (wave reductions + one atomic per
-//                 wave).
-//   k_grid_setup  cell size (collision.cpp:339-356) and a dense cell key:
-//                 per-axis cell offsets from the bounding box, packed into
-//                 as few bits as the box needs (27 at 1M objects), so the
-//                 radix sort runs that many bits / 9 passes.
+//                 (scalar) loads; per-block partials (max |v|, bounding box
+//                 of the start-of-step positions, alive counts) for the
+//                 next single-block kernel, no shared-address atomics.
+//   k_grid_setup  combines the partials: cell size (collision.cpp:339-356)
+//                 and a packed cell key (per-axis offsets from the bounding
+//                 box, as few bits as it needs: 27 at 1M objects, so the
+//                 radix sort runs three 9-bit passes); also writes the
+//                 previous screening's statistics record when one is
+//                 deferred to it (the learned form's windows).
 //   k_keys        packed cell key per object (dead objects get a sentinel
-//                 that sorts last) + first-pass digit histogram.
+//                 that sorts last), the object's compact FP32 record, and
+//                 the first pass's digit histogram.
 //   k_hist / k_digit_scan / k_scatter
-//                 stable LSD radix sort of (key, index), 9-bit digits:
-//                 per-tile digit histograms, per-digit scans, then a scatter
-//                 whose in-tile ranks come from wave64 ballots (match on the
-//                 9 digit bits).
+//                 stable LSD radix sort of (key, index): per-tile digit
+//                 histograms, per-digit scans, then a scatter whose in-tile
+//                 ranks come from wave64 ballots.
 //   k_cell_count / k_cell_write
-//                 cell boundaries by a wavefront scan of the head flags
-//                 (per-tile counts, summed by each tile of k_cell_write);
-//                 writes the cell table and the compact FP32 copy of each
-//                 sorted object (cell-relative position, velocity, index).
-//   k_neighbours  per cell: population and the 13 forward neighbours
-//                 (binary search in the sorted cell keys).
-//   k_pairs       one lane per sorted object: balanced intra-cell pairs +
-//                 13 forward cells, FP32 prefilter_keep, FP64 filter_pair,
-//                 FP64 Hermite / golden-section TCA and radius test; events
-//                 appended with wave-aggregated atomics.
-//   k_finish      per-step statistics record, accumulators reset.
+//                 cell boundaries from wave ballots of the head flags; the
+//                 cell table, the column table and the records gathered in
+//                 sorted order.
+//   k_neighbours  per cell: its five neighbour runs (the 13 forward cells of
+//                 collision_filter.hpp:137-140 as z-runs of four columns plus
+//                 the cell's own tail), found through the column table;
+//                 dense cells registered for work items.
+//   k_items       work items of dense cells and long runs.
+//   k_pairs / k_pairs_dense
+//                 the FP32 pre-filter on every candidate pair (sparse cells:
+//                 (object, run) units ranked by length, a lane per unit;
+//                 dense cells: a wave per item); survivors go to a queue.
+//   k_exact       the queue: FP64 filter_pair, Hermite + golden-section TCA,
+//                 radius test; events appended with wave-aggregated atomics.
+//   k_finish      the step's statistics record, accumulators reset.
+//
+// The learned form (KHIP_PRECISION_LEARNED) replaces k_propagate by the
+// corrector / base / plain-method kernels (k_learned) over per-method lists,
+// screens each step in windows on the step's cubic dense output (k_windows:
+// window boundary states, reductions and, with radial shells, each object's
+// shells; k_keys_shell: one sorted element per (object, shell)), and solves
+// every event again on plain FP64 trajectories (k_reverify). In-run
+// breakups (k_bu_*) follow a step's screening in every form.
 //
 // The host only enqueues; it synchronises once per chunk of steps to drain
 // events and statistics.
@@ -47,6 +60,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -3195,6 +3209,55 @@ bool ensure_breakup(khip_ctx* ctx) {
     return true;
 }
 
+constexpr size_t kGraphCache = 8;
+
+// Launches a chunk as a HIP graph: captured once per key (what the launches
+// depend on beyond device memory: buffers, lengths, check pattern, radix
+// passes) and replayed for every later chunk of the same key. Up to
+// kGraphCache instantiated graphs are kept (most recently used last).
+bool launch_graph(khip_ctx* ctx, const std::vector<int64_t>& key, const std::function<bool()>& enqueue) {
+    hipStream_t s = ctx->stream;
+    hipGraphExec_t exec = nullptr;
+    for (size_t g = 0; g < ctx->graphs.size(); ++g) {
+        if (ctx->graphs[g].first != key) continue;
+        exec = ctx->graphs[g].second;
+        std::rotate(ctx->graphs.begin() + g, ctx->graphs.begin() + g + 1, ctx->graphs.end());
+        break;
+    }
+    if (!exec) {
+        // Entries built for an older generation can never match again.
+        for (size_t g = 0; g < ctx->graphs.size();) {
+            if (ctx->graphs[g].first[0] != key[0] || ctx->graphs.size() >= kGraphCache) {
+                (void)hipGraphExecDestroy(ctx->graphs[g].second);
+                ctx->graphs.erase(ctx->graphs.begin() + g);
+            } else {
+                ++g;
+            }
+        }
+        KHIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const bool ok = enqueue();
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (!ok) return false;
+        if (ec != hipSuccess) return fail(ctx, "hipStreamEndCapture", ec);
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(ctx, "hipGraphInstantiate", ei);
+        ctx->graphs.emplace_back(key, exec);
+        ctx->graph_captures += 1;
+    }
+    KHIP_TRY(ctx, hipGraphLaunch(exec, s));
+    ctx->graph_launches += 1;
+    return true;
+}
+
+bool ensure_steptab(khip_ctx* ctx, int64_t len) {
+    if (len <= ctx->steptab_cap) return true;
+    if (!dev_alloc(ctx, &ctx->steptab, len)) return false;
+    ctx->steptab_cap = len;
+    return true;
+}
+
 bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
     hipStream_t s = ctx->stream;
     if (ctx->n == 0) {
@@ -3261,10 +3324,29 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             bc.seed = ctx->bcfg.seed;
             bc.reserve = reserve;
             bc.sample_cap = ctx->sample_cap;
-            k_reset<<<1, 64, 0, s>>>(ctx->gs);
-            k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+            // Without breakups (whose kernels take the step's time as an
+            // argument) a chunk may run as a captured graph: the per-step
+            // time and index then come from the step table.
+            const bool graphed = ctx->use_graph && !ctx->timing && !bk && screen;
+            if (graphed) {
+                if (!ensure_steptab(ctx, len)) return false;
+                std::vector<StepInfo> tab(static_cast<size_t>(len));
+                double tt = t_chunk;
+                for (int64_t k = 0; k < len; ++k) {
+                    tab[k].t = tt;
+                    tab[k].macro = ctx->step + done + k;
+                    tt += ctx->dt;
+                }
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->steptab, tab.data(), sizeof(StepInfo) * len,
+                                             hipMemcpyHostToDevice, s));
+            }
             double t = t_chunk;
             int cur = cur0;
+            auto enqueue_chunk = [&]() -> bool {
+            k_reset<<<1, 64, 0, s>>>(ctx->gs);
+            k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
+            t = t_chunk;
+            cur = cur0;
             for (int64_t k = 0; k < len; ++k) {
                 const int nxt = cur ^ 1;
                 g_slot = static_cast<int>(k);
@@ -3281,7 +3363,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 if (screen) {
                     if (bk) k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                     if (!enqueue_screen(ctx, ctx->planes(cur), ctx->planes(nxt), t, ctx->step + done + k,
-                                        static_cast<int>(k), ctx->dt, ctx->cfg.threshold_km, 0))
+                                        static_cast<int>(k), ctx->dt, ctx->cfg.threshold_km, 0,
+                                        graphed ? ctx->steptab + k : nullptr))
                         return false;
                     if (bk) {
                         const Planes pb = ctx->planes(cur), pa = ctx->planes(nxt);
@@ -3308,6 +3391,19 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 t += ctx->dt;
                 cur = nxt;
             }
+            return true;
+            };
+            if (graphed) {
+                const std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
+                                                  ctx->precision, -1};
+                if (!launch_graph(ctx, key, enqueue_chunk)) return false;
+            } else if (!enqueue_chunk()) {
+                return false;
+            }
+            // The chunk's end (a replayed graph runs no host code).
+            t = t_chunk;
+            for (int64_t k = 0; k < len; ++k) t += ctx->dt;
+            cur = (len & 1) ? cur0 ^ 1 : cur0;
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
             if (!drain_chunk(ctx, ctx->step + done, len, screen, overflow)) return false;
@@ -3381,8 +3477,6 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
 // ---------------------------------------------------------------------------
 // Learned form: host orchestration
 // ---------------------------------------------------------------------------
-
-constexpr size_t kGraphCache = 8;
 
 // Screening windows per learned step: the configured count, or by default
 // two (half a step each) with radial shells when the sub-steps split
@@ -3637,10 +3731,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         const bool force0 = ctx->force_check;
         for (;;) {
             // Step table of the chunk and the host's copy of the check pattern.
-            if (len > ctx->steptab_cap) {
-                if (!dev_alloc(ctx, &ctx->steptab, len)) return false;
-                ctx->steptab_cap = len;
-            }
+            if (!ensure_steptab(ctx, len)) return false;
             std::vector<StepInfo> tab(static_cast<size_t>(len));
             std::vector<int64_t> pattern;
             int64_t checks = 0;
@@ -3762,37 +3853,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
                                             shells ? ctx->elaunch : 0};
                 key.insert(key.end(), pattern.begin(), pattern.end());
-                hipGraphExec_t exec = nullptr;
-                for (size_t g = 0; g < ctx->graphs.size(); ++g) {
-                    if (ctx->graphs[g].first != key) continue;
-                    exec = ctx->graphs[g].second;
-                    std::rotate(ctx->graphs.begin() + g, ctx->graphs.begin() + g + 1, ctx->graphs.end());
-                    break;
-                }
-                if (!exec) {
-                    // Entries built for an older generation can never match again.
-                    for (size_t g = 0; g < ctx->graphs.size();) {
-                        if (ctx->graphs[g].first[0] != key[0] || ctx->graphs.size() >= kGraphCache) {
-                            (void)hipGraphExecDestroy(ctx->graphs[g].second);
-                            ctx->graphs.erase(ctx->graphs.begin() + g);
-                        } else {
-                            ++g;
-                        }
-                    }
-                    KHIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    const bool ok = enqueue_chunk();
-                    hipGraph_t graph = nullptr;
-                    const hipError_t ec = hipStreamEndCapture(s, &graph);
-                    if (!ok) return false;
-                    if (ec != hipSuccess) return fail(ctx, "hipStreamEndCapture", ec);
-                    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(graph);
-                    if (ei != hipSuccess) return fail(ctx, "hipGraphInstantiate", ei);
-                    ctx->graphs.emplace_back(key, exec);
-                    ctx->graph_captures += 1;
-                }
-                KHIP_TRY(ctx, hipGraphLaunch(exec, s));
-                ctx->graph_launches += 1;
+                if (!launch_graph(ctx, key, enqueue_chunk)) return false;
             } else if (!enqueue_chunk()) {
                 return false;
             }

@@ -76,6 +76,28 @@ def test_catalog_one_hour_matches_reference(engine, catalog, golden_1h):
     _same_stats(st, golden_1h["stats"])
 
 
+def test_graph_replay_matches_reference_hour(engine, catalog, golden_1h):
+    """The plain path's chunks captured into HIP graphs and replayed (chunks
+    of 16 steps: one capture serves every full chunk) give the reference's
+    hour bit for bit, as direct launches do."""
+    n = catalog["states"].shape[0]
+    engine.set_graph(True)
+    try:
+        c0, l0 = engine.graph_counts()
+        s, a, ev, st = _run_engine(engine, catalog["states"], np.ones(n, np.uint8), catalog["size_m"], 10.0, 360,
+                                   chunk=16)
+        c1, l1 = engine.graph_counts()
+    finally:
+        engine.set_graph(False)
+        engine.set_chunk(256)
+    print(f"graph captures {c1 - c0}, launches {l1 - l0}")
+    assert l1 - l0 >= 20 and 1 <= c1 - c0 <= 10
+    assert hashlib.sha256(s.tobytes()).hexdigest() == str(golden_1h["final_sha256"])
+    assert np.array_equal(a, golden_1h["alive"])
+    _same_events(ev, golden_1h["events"])
+    _same_stats(st, golden_1h["stats"])
+
+
 def test_chunking_does_not_change_results(engine, catalog, orc):
     n = 4000
     states = catalog["states"][:n]
