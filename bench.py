@@ -442,7 +442,7 @@ def learned_workload(kamd, args):
 def train_corrector(eng, kamd, iterations=20000):
     eng.set_time(EPOCH, 0.0, 60.0)
     t0 = time.perf_counter()
-    rep, _ = eng.net_train(kamd.generate(500000, seed=101)["state"], samples_per_orbit=10, spacing_steps=12,
+    rep, _ = eng.net_train(kamd.generate(500000, seed=101)["state"], samples_per_orbit=15, spacing_steps=12,
                            iterations=iterations, batch=16384, lr_max=3e-3, seed=1)
     rep["wall_s"] = time.perf_counter() - t0
     return rep

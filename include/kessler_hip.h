@@ -214,10 +214,18 @@ int khip_reset_timing(khip_ctx* ctx);
  * radii closer than the threshold. Every event found is then solved again on
  * plain FP64 trajectories from the step's start (filter_pair, Hermite +
  * golden section, radius test, in the sub-step of tau / substeps holding
- * the TCA) at the configured threshold itself, and kept with the plain
- * values when the plain solve accepts it; an event whose plain minimum
- * leaves the learned step keeps its learned values if its learned miss
- * distance is within the threshold (events_unverified). Event step indices count
+ * the TCA) at the configured threshold plus the drift band, and kept with
+ * the plain values when the plain solve accepts it; an event whose plain
+ * minimum leaves the learned step keeps its learned values if its learned
+ * miss distance is within that (events_unverified). The drift band
+ * (margin_factor) covers the distance the learned trajectories have drifted
+ * from plain ones since the run started: with it, every event of a plain run
+ * from the same states is reported, together with events whose miss lies
+ * within the band above the threshold (miss_km > threshold_km marks them;
+ * the plain run's miss for those may lie on either side). Likewise the
+ * velocity band lowers the minimum encounter speed (relative speeds drift
+ * too). The search threshold is threshold + margin_km + band. Event step
+ * indices count
  * sub-steps, as a plain run at dt = tau / substeps does; statistics have one
  * record per sub-step, a window's counts in the record of its first
  * sub-step and zero counts in the others. */
@@ -237,6 +245,16 @@ typedef struct {
                             shells when substeps is even, else substeps / 2 when even, else
                             substeps). Fewer, longer windows mean fewer grid builds and more
                             candidate pairs per build. */
+    double margin_factor; /* drift band from the measured error (> 0; 0: no band): every chunk the
+                             band is margin_factor x D x ((T + L) / T)^2, with D the largest
+                             distance between learned and plain trajectories so far over a sample
+                             of up to 8,192 objects (evenly spaced) that the plain method follows
+                             alongside from the last khip_set_objects / khip_set_time /
+                             khip_set_precision change / khip_set_learned, T the time since then
+                             and L the chunk's span (along-track drift grows quadratically in
+                             time), rounded up to a ladder of 2^(1/4) steps. Default 4: both
+                             objects of a pair drift, and the sample's maximum stands for the
+                             population's. */
     double shell_km;     /* radial shell width of the screening (>= 1 km, with at most 4 windows):
                             objects are binned by their radius range over the window as well
                             as by cell, and pairs are enumerated within shells only (see
@@ -252,6 +270,12 @@ typedef struct {
     int64_t events_dropped;       /* events the plain solve rejected (not reported) */
     int64_t events_unverified;    /* plain minimum outside the learned step, learned miss within the
                                      threshold (learned values kept) */
+    double sample_dev_max_km;     /* D: largest learned-vs-plain distance over the followed sample */
+    double sample_dev_v_max_km_s; /* the same in velocity */
+    double band_km;               /* the drift band of the last chunk */
+    double band_v_km_s;           /* its velocity band: relative speeds down to
+                                     min_encounter_speed_km_s - band_v_km_s are kept */
+    int64_t sample_objects;       /* size of the followed sample */
 } khip_learned_report;
 
 int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);

@@ -85,13 +85,16 @@ class TrainConfig(C.Structure):
 
 class LearnedConfig(C.Structure):
     _fields_ = [("substeps", C.c_int32), ("check_every", C.c_int32), ("tol_km", _D), ("margin_km", _D),
-                ("e_max", _D), ("a_max_km", _D), ("tol_a_km", _D), ("windows", C.c_int32), ("shell_km", _D)]
+                ("e_max", _D), ("a_max_km", _D), ("tol_a_km", _D), ("windows", C.c_int32), ("margin_factor", _D),
+                ("shell_km", _D)]
 
 
 LEARNED_REPORT_DTYPE = np.dtype([("learned", "<i8"), ("base", "<i8"), ("plain", "<i8"), ("rejected", "<i8"),
                                  ("dev_max_km", "<f8"), ("checks", "<i8"), ("steps", "<i8"),
                                  ("events_verified", "<i8"), ("events_dropped", "<i8"),
-                                 ("events_unverified", "<i8")])
+                                 ("events_unverified", "<i8"), ("sample_dev_max_km", "<f8"),
+                                 ("sample_dev_v_max_km_s", "<f8"), ("band_km", "<f8"), ("band_v_km_s", "<f8"),
+                                 ("sample_objects", "<i8")])
 
 
 class TrainReport(C.Structure):
@@ -367,9 +370,9 @@ class Engine:
         return grad, loss.value
 
     def set_learned(self, substeps=6, check_every=10, tol_km=1e-5, margin_km=0.025, e_max=0.1,
-                    a_max_km=12000.0, tol_a_km=5e-8, windows=0, shell_km=32.0):
+                    a_max_km=12000.0, tol_a_km=5e-8, windows=0, shell_km=32.0, margin_factor=4.0):
         cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km, tol_a_km, windows,
-                            shell_km)
+                            margin_factor, shell_km)
         self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")
 
     def set_scenarios(self, offsets):

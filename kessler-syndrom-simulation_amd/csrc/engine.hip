@@ -59,6 +59,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <mutex>
@@ -165,6 +166,8 @@ struct GridState {
     unsigned int bmin;              // innermost shell of this screening (shell screening)
     int entries_short;              // a screening had more shell entries than were launched
     unsigned int entries_max;       // max shell entries over the chunk
+    int sort_fault;                 // a radix pass's look-back gave up waiting (never expected)
+    int tile_ctr[MAX_PASSES];       // tiles claimed per radix pass (one-sweep passes)
     int pad4;
 };
 
@@ -392,6 +395,19 @@ struct CheckStats {
     unsigned long long dev_max;   // largest accepted deviation (km), double bits
 };
 
+// Followed sample: up to SAMPLE_MAX objects, evenly spaced over the
+// population, that the plain method carries alongside a learned run from
+// the last change of population, time, form or configuration. Their largest
+// distance from the learned states is the measured drift of the learned
+// trajectories, and the search margin of the next chunk follows it.
+constexpr int SAMPLE_MAX = 8192;
+
+struct SampleObj {
+    double r[3], v[3];
+    int idx;  // object index
+    int on;   // 0: dropped (the object or its plain twin died)
+};
+
 // One object through `substeps` FP64 RK4 steps; moon13 holds the Moon every
 // tau / 12 over the step. Stops at the sub-step whose end is under the
 // surface (the object dies there, as a plain step of tau / substeps would).
@@ -542,6 +558,152 @@ __global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8
         }
     }
     if (KIND == KIND_CHECK) {
+        for (int q = 0; q < 3; ++q) {
+            const unsigned int s = wave_sum(cnt[q]);
+            if (lane == 0 && s) atomicAdd(&cs->count[q], s);
+        }
+        const unsigned int rj = wave_sum(rejected);
+        if (lane == 0 && rj) atomicAdd(&cs->rejected, rj);
+        const unsigned long long dm =
+            wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(dev_max)));
+        if (lane == 0) atomicMax(&cs->dev_max, dm);
+    }
+}
+
+// The same step split by occupancy: the FP64 work runs in light kernels, one
+// object per lane at full occupancy, and only the corrector (the MFMA work,
+// whose weights fill LDS) in a kernel of its own. Arithmetic and results are
+// those of k_learned, bit for bit.
+//
+// k_base_step: the mixed RK4 step of tau. KIND CLASS_LEARNED / CLASS_BASE:
+// the list's objects into `out` (= the next states; for CLASS_BASE with the
+// re-entry test, the corrected objects get theirs after the correction);
+// KIND_CHECK: every live object of the corrector's or base step's domain
+// into `out` (a scratch set of planes), the flags untouched.
+template <int KIND>
+__global__ void __launch_bounds__(BLOCK) k_base_step(Planes cur, Planes out, uint8_t* alive,
+                                                      const int* __restrict__ list, const int* __restrict__ counts,
+                                                      int n, const MoonSample* __restrict__ moon13, LearnedParams lp) {
+    const int count = KIND == KIND_CHECK ? n : counts[KIND];
+    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < count; k += gridDim.x * BLOCK) {
+        const int i = KIND == KIND_CHECK ? k : list[k];
+        D3 r{cur.x[i], cur.y[i], cur.z[i]}, v{cur.vx[i], cur.vy[i], cur.vz[i]};
+        if (!alive[i]) {
+            if (KIND != KIND_CHECK) store_state(out, i, r, v);  // frozen
+            continue;
+        }
+        if (KIND == KIND_CHECK && net::domain_class(r, v, lp.e_max, lp.a_max) == net::CLASS_PLAIN) continue;
+        rk4(r, v, lp.tau, moon13[0], moon13[6], moon13[12], AccelMixed());
+        store_state(out, i, r, v);
+        if (KIND == net::CLASS_BASE && sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) alive[i] = 0;
+    }
+}
+
+// k_check_plain: the plain method for every object (a check step's states),
+// dead objects frozen; an object under the surface at a sub-step's end dies.
+__global__ void __launch_bounds__(BLOCK) k_check_plain(Planes cur, Planes nxt, uint8_t* alive, int n,
+                                                        const MoonSample* __restrict__ moon13, LearnedParams lp) {
+    const double h = lp.tau / lp.substeps;
+    for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+        D3 r{cur.x[i], cur.y[i], cur.z[i]}, v{cur.vx[i], cur.vy[i], cur.vz[i]};
+        if (alive[i]) {
+            if (!plain_substeps(r, v, moon13, h, lp.substeps)) alive[i] = 0;
+        }
+        store_state(nxt, i, r, v);
+    }
+}
+
+// k_net_step: the corrector on top of k_base_step's states. CHECK false:
+// the corrected list, base states in `nxt`, corrected in place (re-entry
+// test after the correction). CHECK true: every object after k_check_plain
+// (the plain states in `nxt`, `alive` = alive and survived the plain
+// sub-steps), base states in `base`: an object keeps the corrected or base
+// method if it stayed within the tolerances of its plain state, and the
+// check tallies and method flags are written as k_learned<KIND_CHECK> does.
+template <bool CHECK>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) k_net_step(Planes cur, Planes nxt, Planes base, uint8_t* alive,
+                                                     uint8_t* mode, const int* __restrict__ list,
+                                                     const int* __restrict__ counts, int n,
+                                                     const MoonSample* __restrict__ moon13,
+                                                     const uint16_t* __restrict__ ffwd, const float* __restrict__ P,
+                                                     net::NetNorm nn, LearnedParams lp, CheckStats* cs) {
+    __shared__ uint4 s_frag[net::NFRAG_FWD * net::FRAG_HALVES / 8];
+    {
+        const uint4* src = reinterpret_cast<const uint4*>(ffwd);
+        for (int k = threadIdx.x; k < net::NFRAG_FWD * net::FRAG_HALVES / 8; k += BLOCK) s_frag[k] = src[k];
+        __syncthreads();
+    }
+    const uint16_t* frags = reinterpret_cast<const uint16_t*>(s_frag);
+    const int count = CHECK ? n : counts[net::CLASS_LEARNED];
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
+    unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
+    double dev_max = 0.0;
+    for (int tile = blockIdx.x * WAVES_PER_BLOCK + wave; tile * WAVE < count; tile += gridDim.x * WAVES_PER_BLOCK) {
+        const int k = tile * WAVE + lane;
+        const bool active = k < count;
+        const int i = active ? (CHECK ? k : list[k]) : 0;
+        const bool on = active && alive[i] != 0;
+        D3 r{0.0, 0.0, 0.0}, v{0.0, 0.0, 0.0};
+        if (on) {
+            r = D3{cur.x[i], cur.y[i], cur.z[i]};
+            v = D3{cur.vx[i], cur.vy[i], cur.vz[i]};
+        }
+        const int cls = !on ? net::CLASS_PLAIN
+                            : (CHECK ? net::domain_class(r, v, lp.e_max, lp.a_max) : net::CLASS_LEARNED);
+        // The base state and the correction frame are taken after the
+        // network (fewer registers live across it; the frame is the same
+        // arithmetic on the same inputs).
+        const bool want_net = cls == net::CLASS_LEARNED;
+        float o[net::NOUT];
+        const bool any_net = __ballot(want_net) != 0ULL;
+        if (any_net) {
+            float x[net::NIN];
+            if (want_net) {
+                net::Frame unused;
+                net::features(r, v, moon13[0], nn, x, unused);
+            } else {
+#pragma unroll
+                for (int q = 0; q < net::NIN; ++q) x[q] = 0.0f;
+            }
+            net_wave(x, frags, P, o);
+        }
+        const Planes src = CHECK ? base : nxt;
+        D3 rb{0.0, 0.0, 0.0}, vb{0.0, 0.0, 0.0};
+        if (cls != net::CLASS_PLAIN) {
+            rb = D3{src.x[i], src.y[i], src.z[i]};
+            vb = D3{src.vx[i], src.vy[i], src.vz[i]};
+        }
+        if (want_net) {
+            float x[net::NIN];
+            net::Frame fr;
+            net::features(r, v, moon13[0], nn, x, fr);
+            net::apply_correction(rb, vb, o, nn, fr);
+        }
+        if (!CHECK) {
+            if (on) {
+                store_state(nxt, i, rb, vb);
+                if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) alive[i] = 0;
+            }
+        } else if (active) {
+            int m = net::CLASS_PLAIN;
+            if (cls != net::CLASS_PLAIN) {
+                const D3 rp{nxt.x[i], nxt.y[i], nxt.z[i]}, vp{nxt.vx[i], nxt.vy[i], nxt.vz[i]};
+                const double dx = rb.x - rp.x, dy = rb.y - rp.y, dz = rb.z - rp.z;
+                const double d = sqrt(dx * dx + dy * dy + dz * dz);
+                const double da = fabs(semi_major(rb, vb) - semi_major(rp, vp));
+                if (d <= lp.tol_km && da <= lp.tol_a_km) {
+                    m = cls;
+                    dev_max = d > dev_max ? d : dev_max;
+                } else {
+                    ++rejected;
+                }
+            }
+            mode[i] = static_cast<uint8_t>(m);
+            if (on) ++cnt[m];
+        }
+    }
+    if (CHECK) {
         for (int q = 0; q < 3; ++q) {
             const unsigned int s = wave_sum(cnt[q]);
             if (lane == 0 && s) atomicAdd(&cs->count[q], s);
@@ -831,6 +993,7 @@ __global__ void k_reset(GridState* gs) {
     gs->npasses = 0;
     gs->entries_short = 0;
     gs->entries_max = 0;
+    gs->sort_fault = 0;
 }
 
 __global__ void k_zero_events(GridState* gs) {
@@ -977,9 +1140,16 @@ struct ShellScreen {
 __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
-                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh) {
+                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh,
+                                                       int* ghist) {
     __shared__ StepPartial s_rows[TWAVES];
     __shared__ int s_wave[TWAVES];
+    // One-sweep sorting: the digit counts of every pass (accumulated by the
+    // key kernel) and the passes' tile counters start from zero.
+    if (ghist) {
+        for (int k = threadIdx.x; k < MAX_PASSES * RADIX; k += TBLOCK) ghist[k] = 0;
+        if (threadIdx.x < MAX_PASSES) gs->tile_ctr[threadIdx.x] = 0;
+    }
     long long entries = 0;
     if (sh.on) {
         // Exclusive prefix of the object tiles' entry counts (every thread).
@@ -1093,11 +1263,15 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
 // sorted object.
 __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  const int* __restrict__ scen, int n, const GridState* gs,
-                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec) {
+                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec,
+                                                 int* ghist) {
     __shared__ int s_hist[RADIX];
+    __shared__ int s_gh[MAX_PASSES * RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
+    if (ghist)
+        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK) s_gh[k] = 0;
     __syncthreads();
     const int base = blockIdx.x * TILE;
     for (int r = 0; r < TITEMS; ++r) {
@@ -1126,11 +1300,20 @@ __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int mid
             }
             keys[e] = k;
             idx[e] = e;
-            atomicAdd(&s_hist[static_cast<int>(k & DIGIT_MASK)], 1);
+            if (ghist)
+                for (int p = 0; p < g.npasses; ++p)
+                    atomicAdd(&s_gh[p * RADIX + static_cast<int>((k >> (RADIX_BITS * p)) & DIGIT_MASK)], 1);
+            else
+                atomicAdd(&s_hist[static_cast<int>(k & DIGIT_MASK)], 1);
         }
     }
     __syncthreads();
-    if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+    if (ghist) {
+        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK)
+            if (s_gh[k]) atomicAdd(&ghist[k], s_gh[k]);
+    } else if (threadIdx.x < RADIX) {
+        hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
+    }
 }
 
 // Shell entries of one object tile (TILE objects, the tiles k_windows
@@ -1145,10 +1328,15 @@ __global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, c
                                                        const int* __restrict__ shell_lo,
                                                        const uint16_t* __restrict__ shell_n,
                                                        const int* __restrict__ tile_base, unsigned int ecap,
-                                                       uint64_t* keys, int* idx, float4* rec) {
+                                                       uint64_t* keys, int* idx, float4* rec, int* ghist) {
     __shared__ int s_wave[TWAVES];
+    __shared__ int s_gh[MAX_PASSES * RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
+    if (ghist) {
+        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK) s_gh[k] = 0;
+        __syncthreads();
+    }
     const int base = blockIdx.x * TILE;
     const int bz = g.bits[2], bb = g.bbits;
     unsigned int off = static_cast<unsigned int>(tile_base[blockIdx.x]);
@@ -1180,10 +1368,19 @@ __global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, c
         for (int j = 0; j < nb; ++j) {
             const unsigned int pos = off + static_cast<unsigned int>(ex + j);
             if (pos >= ecap) break;
-            keys[pos] = k | (static_cast<uint64_t>(s0 + j - static_cast<int>(g.bmin)) << bz);
+            const uint64_t kj = k | (static_cast<uint64_t>(s0 + j - static_cast<int>(g.bmin)) << bz);
+            keys[pos] = kj;
             idx[pos] = e | (j ? NOT_FIRST : 0);
+            if (ghist)
+                for (int p = 0; p < g.npasses; ++p)
+                    atomicAdd(&s_gh[p * RADIX + static_cast<int>((kj >> (RADIX_BITS * p)) & DIGIT_MASK)], 1);
         }
         off += static_cast<unsigned int>(tot);
+    }
+    if (ghist) {
+        __syncthreads();
+        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK)
+            if (s_gh[k]) atomicAdd(&ghist[k], s_gh[k]);
     }
 }
 
@@ -1280,6 +1477,138 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
             const int pos = s_off[d] + pre + rank;
             keys_out[pos] = key;
             idx_out[pos] = id;
+        }
+        __syncthreads();
+        if (t < RADIX) {
+            int add = 0;
+            for (int w = 0; w < TWAVES; ++w) {
+                add += s_wcnt[w][t];
+                s_wcnt[w][t] = 0;
+            }
+            s_off[t] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// One-sweep radix pass: histogram, offsets and scatter of one digit in a
+// single launch. The digit counts of the whole array (ghist, all passes,
+// from the key kernel) give each digit's start; a tile's offset within its
+// digit comes from the tiles before it by decoupled look-back: each tile
+// publishes its own digit counts at once (AGG), then walks back over its
+// predecessors' words, adding AGG counts until it meets an inclusive prefix
+// (PREFIX), and publishes its own prefix. Tiles are claimed in launch order
+// from a counter, so every tile waited on belongs to a block already
+// running, which publishes its counts without waiting on anything: the
+// walk always ends. A word is (epoch << 34 | flag << 32 | count) with the
+// epoch unique to the screening and pass, so words of earlier passes read
+// as not yet published; one 64-bit store carries flag and count together.
+// The walk's wait is bounded all the same: past the bound it gives up, sets
+// sort_fault (the host reports an error) and leaves positions inside the
+// array. The scatter itself is k_scatter's stable in-tile ranking, so the
+// order is the same as the three-launch pass.
+constexpr unsigned long long LB_AGG = 1ULL, LB_PREFIX = 2ULL;
+constexpr int LB_SPIN_MAX = 1 << 22;
+
+__global__ void __launch_bounds__(TBLOCK) k_onesweep(const uint64_t* keys_in, const int* idx_in, uint64_t* keys_out,
+                                                      int* idx_out, int pass, GridState* gs,
+                                                      const int* __restrict__ ghist, unsigned long long* status) {
+    __shared__ int s_off[RADIX];
+    __shared__ int s_cnt[RADIX];
+    __shared__ int s_wcnt[TWAVES][RADIX];
+    __shared__ int s_wave[TWAVES];
+    __shared__ int s_tile;
+    if (pass >= gs->npasses) return;
+    const int n = static_cast<int>(gs->nsort);
+    const int ntiles = (n + TILE - 1) / TILE;
+    const int t = threadIdx.x;
+    if (t == 0) s_tile = atomicAdd(&gs->tile_ctr[pass], 1);
+    if (t < RADIX) s_cnt[t] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) return;  // uniform: blocks beyond the live tiles
+    const unsigned long long epoch =
+        (static_cast<unsigned long long>(static_cast<unsigned int>(gs->stamp)) * MAX_PASSES + pass) & 0x3fffffffULL;
+    const int shift = RADIX_BITS * pass;
+    const int base = tile * TILE;
+    uint64_t keys[TITEMS];
+    int ids[TITEMS];
+#pragma unroll
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + t;
+        keys[r] = e < n ? keys_in[e] : 0ULL;
+        ids[r] = e < n ? idx_in[e] : 0;
+        if (e < n) atomicAdd(&s_cnt[static_cast<int>((keys[r] >> shift) & DIGIT_MASK)], 1);
+    }
+    __syncthreads();
+    int excl = 0;
+    if (t < RADIX) {
+        const unsigned int own = static_cast<unsigned int>(s_cnt[t]);
+        unsigned long long* mine = status + static_cast<size_t>(tile) * RADIX + t;
+        if (tile == 0) {
+            __hip_atomic_store(mine, (epoch << 34) | (LB_PREFIX << 32) | own, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(mine, (epoch << 34) | (LB_AGG << 32) | own, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            unsigned int acc = 0;
+            int tp = tile - 1, spins = 0;
+            for (;;) {
+                const unsigned long long w = __hip_atomic_load(status + static_cast<size_t>(tp) * RADIX + t,
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long flag = (w >> 32) & 3ULL;
+                if ((w >> 34) != epoch || flag == 0ULL) {
+                    if (++spins > LB_SPIN_MAX) {
+                        gs->sort_fault = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                acc += static_cast<unsigned int>(w);
+                if (flag == LB_PREFIX || tp == 0) break;
+                --tp;
+            }
+            __hip_atomic_store(mine, (epoch << 34) | (LB_PREFIX << 32) | (acc + own), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            excl = static_cast<int>(acc);
+        }
+    }
+    {
+        int tot;
+        const int digit_base = block_exclusive_scan<TWAVES>(t < RADIX ? ghist[pass * RADIX + t] : 0, tot, s_wave);
+        if (t < RADIX) {
+            s_off[t] = digit_base + excl;
+            for (int w = 0; w < TWAVES; ++w) s_wcnt[w][t] = 0;
+        }
+    }
+    __syncthreads();
+    const int wave = t / WAVE;
+    const unsigned long long below = lanes_below();
+#pragma unroll
+    for (int r = 0; r < TITEMS; ++r) {
+        const int e = base + r * TBLOCK + t;
+        const bool valid = e < n;
+        const uint64_t key = keys[r];
+        const int id = ids[r];
+        const int d = static_cast<int>((key >> shift) & DIGIT_MASK);
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < RADIX_BITS; ++b) {
+            const bool bit = (d >> b) & 1;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & below);
+        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            int pre = 0;
+            for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
+            const int pos = s_off[d] + pre + rank;
+            if (pos < n) {
+                keys_out[pos] = key;
+                idx_out[pos] = id;
+            }
         }
         __syncthreads();
         if (t < RADIX) {
@@ -2145,25 +2474,31 @@ __global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, Pa
 // learned step's start state by FP64 RK4 sub-steps (the plain method) to the
 // start and end of sub-step k, and the reference's exact test (filter_pair,
 // Hermite + golden section, radius test; collision.cpp:109-132) runs on that
-// pair of states, with the caller's threshold itself (the raised threshold
-// serves the learned search only), so the decision is the plain run's. A
-// minimum on a sub-step edge is looked for in the neighbouring sub-step of
-// the same learned step. The event's TCA, miss distance, speed and
+// pair of states, with the caller's threshold plus the drift band (the
+// distance learned trajectories may have drifted from plain ones since the
+// run began; the search margin above that serves the learned search only).
+// A minimum on a sub-step edge is looked for in the neighbouring sub-step of
+// the same learned step, and so is one of a pair the filter rejects in the
+// event's own sub-step (receding there: the minimum may lie just before the
+// boundary). The event's TCA, miss distance, speed and
 // collision flag become the plain solve's; an event the plain solve rejects
 // is dropped (pad = 1). One whose plain minimum leaves the learned step
 // (a TCA within a fraction of a millisecond of the step's edge) keeps its
 // learned values (pad = 2) if its learned miss distance is within the
-// caller's threshold, and is dropped otherwise.
+// reporting threshold, and is dropped otherwise.
 
 struct ReverifyStats {
     unsigned long long verified, dropped, kept_unverified;
 };
 
-// 0: event (s, miss, vrel); 1: rejected; 2: minimum at s ~ 0; 3: at s ~ 1.
+// 0: event (s, miss, vrel); 1: rejected; 2: minimum at s ~ 0; 3: at s ~ 1;
+// 4: rejected, receding at the start.
 __device__ __forceinline__ int plain_refine(D3 ri, D3 vi, D3 rj, D3 vj, D3 ai, D3 wi, D3 aj, D3 wj, double dt,
                                             double thr, double vmin2, double& s, double& miss, double& vrel) {
-    const int res = filter_pair(rj.x - ri.x, rj.y - ri.y, rj.z - ri.z, vj.x - vi.x, vj.y - vi.y, vj.z - vi.z, dt,
-                                thr, vmin2);
+    const double rx = rj.x - ri.x, ry = rj.y - ri.y, rz = rj.z - ri.z;
+    const double ux = vj.x - vi.x, uy = vj.y - vi.y, uz = vj.z - vi.z;
+    const int res = filter_pair(rx, ry, rz, ux, uy, uz, dt, thr, vmin2);
+    if (res == FILTER_REJECT && rx * ux + ry * uy + rz * uz >= 0.0) return 4;
     if (res != FILTER_REFINE) return 1;
     const Hermite h{{rj.x - ri.x, rj.y - ri.y, rj.z - ri.z},
                     {(vj.x - vi.x) * dt, (vj.y - vi.y) * dt, (vj.z - vi.z) * dt},
@@ -2234,13 +2569,19 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
             plain_at2(s0, i, j, moon13, h, sub, k, rai, vai, rbi, vbi, raj, vaj, rbj, vbj);
             code = plain_refine(rai, vai, raj, vaj, rbi, vbi, rbj, vbj, h, thr_report, vmin2, s, miss, vrel);
             ++tries;
-            if (code == 2 && dir <= 0) {
+            // A minimum on the sub-step's start (code 2), or a pair already
+            // receding there on the first try (code 4: the learned drift can
+            // move a TCA from just after a sub-step boundary to just before
+            // it), is looked for in the previous sub-step.
+            if ((code == 2 || (code == 4 && tries == 1)) && dir <= 0) {
+                code = 2;
                 dir = -1;
                 --k;
             } else if (code == 3 && dir >= 0) {
                 dir = 1;
                 ++k;
             } else {
+                if (code == 4) code = 1;
                 break;
             }
         }
@@ -2685,7 +3026,7 @@ struct khip_ctx {
 
     // Learned form: per-object method, per-method lists, dense-output
     // states of the sub-step boundaries, check and re-verification tallies.
-    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0, 32.0};
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0, 4.0, 32.0};
     int64_t lrn_cap = 0;
     uint8_t* mode = nullptr;
     uint8_t* snap_mode = nullptr;
@@ -2709,12 +3050,30 @@ struct khip_ctx {
     bool force_check = true;
     bool reset_checks = true;    // zero the device check tallies before the next learned run
     khip_learned_report lrep{};
+    // Followed sample (plain twins of up to SAMPLE_MAX objects) and its
+    // largest drift; the twins step on a side stream beside the screening.
+    SampleObj* sample = nullptr;
+    SampleObj* snap_sample = nullptr;
+    unsigned long long* sample_dev = nullptr;
+    int sample_n = 0;
+    bool sample_reset = true;   // start the sample afresh at the next learned chunk
+    double sample_t0 = 0.0;     // time the sample started
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     int n_cus = 256;
 
     // Learned-form step table and the captured chunk graph.
     StepInfo* steptab = nullptr;
     int64_t steptab_cap = 0;
     bool use_graph = false;
+    // Learned steps with the FP64 work in full-occupancy kernels and the
+    // corrector alone in the MFMA kernel (KHIP_LEARNED_FUSED=1: one kernel
+    // per method, for comparison; results are bit-identical).
+    bool split_learned = true;
+    // Radix passes in one launch each (look-back); KHIP_SORT_CLASSIC=1:
+    // histogram, digit scan and scatter launches per pass, for comparison
+    // (the same order either way).
+    bool onesweep = true;
     uint64_t gen = 1;               // bumped by every (re)allocation and configuration change
     // Instantiated chunk graphs keyed by what they were captured for (most
     // recently used last; at most kGraphCache).
@@ -2794,7 +3153,8 @@ void free_all(khip_ctx* ctx) {
                     ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
                     ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
                     ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates,
-                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tile_base};
+                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tile_base, ctx->sample, ctx->snap_sample,
+                    ctx->sample_dev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -2815,6 +3175,13 @@ bool ensure_surv(khip_ctx* ctx, int64_t want) {
 
 // Buffers indexed by sorted element or cell: objects for the plain grid,
 // shell entries for the shell screening.
+// Zero-fills a fresh allocation (outside any graph capture).
+template <typename T>
+bool clear(khip_ctx* ctx, T* p, size_t count) {
+    KHIP_TRY(ctx, hipMemset(p, 0, sizeof(T) * count));
+    return true;
+}
+
 bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
     if (want <= ctx->ecap) return true;
     const int64_t cap = std::max<int64_t>(want, ctx->cap);
@@ -2822,12 +3189,13 @@ bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
     ctx->ecap = cap;
     return dev_alloc(ctx, &ctx->keys[0], cap) && dev_alloc(ctx, &ctx->keys[1], cap) &&
            dev_alloc(ctx, &ctx->idx[0], cap) && dev_alloc(ctx, &ctx->idx[1], cap) &&
-           dev_alloc(ctx, &ctx->hist, RADIX * tiles) && dev_alloc(ctx, &ctx->bcount, tiles) &&
+           dev_alloc(ctx, &ctx->hist, 2 * RADIX * tiles) && dev_alloc(ctx, &ctx->bcount, tiles) &&
            dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
            dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lrec, 2 * static_cast<size_t>(cap)) &&
-           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) && dev_alloc(ctx, &ctx->dense, cap);
+           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) && dev_alloc(ctx, &ctx->dense, cap) &&
+           clear(ctx, ctx->hist, 2 * RADIX * tiles);  // look-back words: no stale epoch
 }
 
 bool ensure_capacity(khip_ctx* ctx, int64_t n) {
@@ -2840,7 +3208,7 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
     return dev_alloc(ctx, &ctx->alive, cap) && dev_alloc(ctx, &ctx->radius, cap) &&
            dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->mass, cap) &&
            dev_alloc(ctx, &ctx->snap_mass, cap) && ensure_sort_capacity(ctx, cap) &&
-           dev_alloc(ctx, &ctx->totals, RADIX) &&
+           dev_alloc(ctx, &ctx->totals, MAX_PASSES * RADIX) &&
            dev_alloc(ctx, &ctx->rec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
@@ -2940,7 +3308,7 @@ void mark(khip_ctx* ctx, int k) {
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
                     int sub_per = 1, const StepPartial* parts = nullptr, const FinishArgs* prev = nullptr,
-                    FinishArgs* defer = nullptr, int shell_q = -1) {
+                    FinishArgs* defer = nullptr, int shell_q = -1, double vmin_km_s = -1.0) {
     const int n_obj = static_cast<int>(ctx->n_launch);
     const int obj_tiles = ctx->ntiles();
     const bool shells = shell_q >= 0;
@@ -2956,18 +3324,27 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
         sh = ShellScreen{1, ctx->tile_cnt + static_cast<size_t>(shell_q) * ctx->tstride,
                          ctx->tile_base + static_cast<size_t>(shell_q) * ctx->tstride, obj_tiles,
                          static_cast<unsigned int>(n)};
+    // One-sweep passes: digit counts of every pass from the key kernel
+    // (into `totals`), look-back words in `hist`.
+    int* ghist = ctx->onesweep ? ctx->totals : nullptr;
     k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n_obj), dt, thr, ctx->passes,
-                                     midpoint, ctx->sbits, tc, fin, n_obj, sh);
+                                     midpoint, ctx->sbits, tc, fin, n_obj, sh, ghist);
     if (shells)
         k_keys_shell<<<obj_tiles, TBLOCK, 0, s>>>(
             pb, pa, ctx->alive, ctx->scen, n_obj, ctx->gs, ctx->shell_lo + static_cast<size_t>(shell_q) * ctx->cap,
             ctx->shell_n + static_cast<size_t>(shell_q) * ctx->cap, sh.tile_base, static_cast<unsigned int>(n),
-            ctx->keys[0], ctx->idx[0], ctx->rec);
+            ctx->keys[0], ctx->idx[0], ctx->rec, ghist);
     else
         k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
-                                       ctx->idx[0], ctx->hist, tiles, ctx->rec);
+                                       ctx->idx[0], ctx->hist, tiles, ctx->rec, ghist);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
+        if (ctx->onesweep) {
+            k_onesweep<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
+                                               ctx->gs, ctx->totals,
+                                               reinterpret_cast<unsigned long long*>(ctx->hist));
+            continue;
+        }
         // k_keys counts the first pass's digits; shell entries are not in
         // object tiles, so their first pass is counted here too.
         if (pass > 0 || shells)
@@ -3008,7 +3385,8 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.sub_per = sub_per;
     P.dt = dt;
     P.thr = thr;
-    P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
+    const double vmin = vmin_km_s >= 0.0 ? vmin_km_s : ctx->cfg.min_encounter_speed_km_s;
+    P.vmin2 = vmin * vmin;
     P.dt_f = static_cast<float>(P.dt);
     P.thr_f = static_cast<float>(P.thr);
     P.vmin2_f = static_cast<float>(P.vmin2);
@@ -3110,6 +3488,10 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     rerun = false;
     ctx->last_rerun = 0;
+    if (g.sort_fault) {
+        ctx->error = "radix sort: a pass's look-back did not complete";
+        return false;
+    }
     if (screen && g.passes_short) {
         ctx->passes = std::min(MAX_PASSES, std::max(g.passes_max, ctx->passes + 1));
         ctx->last_rerun |= KHIP_RERUN_PASSES;
@@ -3603,6 +3985,80 @@ __global__ void k_check_begin(CheckStats* cs, int* counts) {
     }
 }
 
+__global__ void __launch_bounds__(BLOCK) k_sample_init(SampleObj* s, int ns, int stride, Planes cur,
+                                                        const uint8_t* __restrict__ alive, unsigned long long* dev) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    if (k == 0) {
+        dev[0] = 0ULL;
+        dev[1] = 0ULL;
+    }
+    if (k >= ns) return;
+    const int i = k * stride;
+    SampleObj o;
+    o.r[0] = cur.x[i]; o.r[1] = cur.y[i]; o.r[2] = cur.z[i];
+    o.v[0] = cur.vx[i]; o.v[1] = cur.vy[i]; o.v[2] = cur.vz[i];
+    o.idx = i;
+    o.on = alive[i] != 0;
+    s[k] = o;
+}
+
+// One learned step of the sample: the plain sub-steps from the twin's own
+// state, then its distances from the object's learned end-of-step state in
+// position (dev[0]) and velocity (dev[1]) (wave maxima, atomicMax of the
+// doubles' bits: distances are >= 0).
+__global__ void __launch_bounds__(BLOCK) k_sample_step(SampleObj* s, int ns, Planes nxt,
+                                                        const uint8_t* __restrict__ alive,
+                                                        const MoonSample* __restrict__ moon13, double h, int sub,
+                                                        unsigned long long* dev) {
+    const int k = blockIdx.x * BLOCK + threadIdx.x;
+    double d = 0.0, dv = 0.0;
+    if (k < ns && s[k].on) {
+        SampleObj o = s[k];
+        D3 r{o.r[0], o.r[1], o.r[2]}, v{o.v[0], o.v[1], o.v[2]};
+        const int i = o.idx;
+        if (!plain_substeps(r, v, moon13, h, sub) || !alive[i]) {
+            s[k].on = 0;
+        } else {
+            o.r[0] = r.x; o.r[1] = r.y; o.r[2] = r.z;
+            o.v[0] = v.x; o.v[1] = v.y; o.v[2] = v.z;
+            s[k] = o;
+            const double dx = nxt.x[i] - r.x, dy = nxt.y[i] - r.y, dz = nxt.z[i] - r.z;
+            d = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ex = nxt.vx[i] - v.x, ey = nxt.vy[i] - v.y, ez = nxt.vz[i] - v.z;
+            dv = sqrt(ex * ex + ey * ey + ez * ez);
+        }
+    }
+    const unsigned long long m = wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(d)));
+    const unsigned long long mv = wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(dv)));
+    if (__lane_id() == 0 && m) atomicMax(dev, m);
+    if (__lane_id() == 0 && mv) atomicMax(dev + 1, mv);
+}
+
+// The drift band for a chunk of `span` seconds, with the sample's drift D
+// (position, km, or velocity, km/s) measured `elapsed` seconds after its
+// start: an error in the semi-major axis grows the along-track drift
+// quadratically in time, so D is carried to the chunk's end as
+// D ((elapsed + span) / elapsed)^2, times `factor` (both objects of a pair
+// drift, and the sample's maximum stands for the population's), rounded up
+// to a ladder of 2^(1/4) steps (1e-3 x 2^(k/4) for integer k) so the chunk
+// graphs come in few sizes.
+double drift_band(double factor, double dev_km, double elapsed, double span) {
+    if (!(factor > 0) || !(dev_km > 0) || !(elapsed > 0)) return 0.0;
+    const double g = (elapsed + span) / elapsed;
+    const double want = factor * dev_km * g * g;
+    const double level = std::ceil(4.0 * std::log2(want / 1e-3)) / 4.0;
+    return 1e-3 * std::exp2(level);
+}
+
+bool ensure_sample(khip_ctx* ctx) {
+    if (ctx->sample) return true;
+    if (!ctx->side) KHIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    if (!ctx->ev_fork) KHIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    if (!ctx->ev_join) KHIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    return dev_alloc(ctx, &ctx->sample, SAMPLE_MAX) && dev_alloc(ctx, &ctx->snap_sample, SAMPLE_MAX) &&
+           dev_alloc(ctx, &ctx->sample_dev, 2);
+}
+
 bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     if (!ctx->net_ready) {
         ctx->error = "learned form: no corrector loaded or trained";
@@ -3635,6 +4091,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         return true;
     }
     if (!ensure_learned(ctx)) return false;
+    const bool follow = L.margin_factor > 0;
+    if (follow && !ensure_sample(ctx)) return false;
     if (ctx->reset_checks) {
         KHIP_TRY(ctx, hipMemsetAsync(ctx->cstats, 0, sizeof(CheckStats), s));
         ctx->reset_checks = false;
@@ -3658,9 +4116,6 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
     lp.a_max = L.a_max_km;
     lp.tol_km = L.tol_km;
     lp.tol_a_km = L.tol_a_km;
-    const double thr = ctx->cfg.threshold_km + L.margin_km;
-    const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
-    const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
     size_t c = static_cast<size_t>(ctx->cap);
     const int g_net = ctx->n_cus;
     int g_all = std::min(grid_of(n), 4 * ctx->n_cus);
@@ -3716,6 +4171,23 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_bstate, ctx->bstate, sizeof(bu::BreakupState),
                                          hipMemcpyDeviceToDevice, s));
         }
+        // The followed sample: started afresh after a change of population,
+        // time or configuration; saved with the chunk's other state.
+        if (follow && ctx->sample_reset) {
+            const int ns = static_cast<int>(std::min<int64_t>(SAMPLE_MAX, n0));
+            const int stride = static_cast<int>(std::max<int64_t>(1, n0 / SAMPLE_MAX));
+            k_sample_init<<<(ns + BLOCK - 1) / BLOCK, BLOCK, 0, s>>>(ctx->sample, ns, stride, ctx->planes(ctx->cur),
+                                                                    ctx->alive, ctx->sample_dev);
+            ctx->sample_n = ns;
+            ctx->sample_t0 = ctx->t;
+            ctx->sample_reset = false;
+        }
+        unsigned long long dev0[2] = {0, 0};
+        if (follow) {
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_sample, ctx->sample, sizeof(SampleObj) * ctx->sample_n,
+                                         hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(dev0, ctx->sample_dev, sizeof(dev0), hipMemcpyDeviceToHost, s));
+        }
         bool retry_chunk = false;  // breakup room grown: the chunk runs again from the top
         KHIP_TRY(ctx, hipMemcpyAsync(ctx->snap_state, ctx->state[ctx->cur], sizeof(double) * 6 * c,
                                      hipMemcpyDeviceToDevice, s));
@@ -3729,6 +4201,25 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         const int cur0 = ctx->cur;
         const double t_chunk = ctx->t;
         const bool force0 = ctx->force_check;
+        // Drift bands of the chunk (the sample's drift carried to the
+        // chunk's end): events are kept up to threshold + band, searched for
+        // up to that plus the configured margin, and relative speeds down to
+        // the minimum encounter speed less the velocity band are kept.
+        const double elapsed = t_chunk - ctx->sample_t0, span = static_cast<double>(len) * ctx->dt;
+        const double band = follow ? drift_band(L.margin_factor, __builtin_bit_cast(double, dev0[0]), elapsed, span)
+                                   : 0.0;
+        const double band_v =
+            follow ? drift_band(L.margin_factor, __builtin_bit_cast(double, dev0[1]), elapsed, span) : 0.0;
+        const double thr = ctx->cfg.threshold_km + L.margin_km + band;
+        const double vmin_eff = std::max(0.0, ctx->cfg.min_encounter_speed_km_s - band_v);
+        const double vmin2 = vmin_eff * vmin_eff;
+        // The search tests a pair's speed at its window's start, the plain
+        // solve at its sub-step's: over a window of hw seconds the relative
+        // velocity of two objects d km apart changes by at most
+        // (2 mu / r^3) d hw, with 2 mu / r^3 <= 3.1e-6 s^-2 above the surface
+        // and d <= thr + vmin hw for a pair that closes to thr in the window.
+        const double vmin_search = std::max(0.0, vmin_eff - 3.1e-6 * (thr + vmin_eff * hw) * hw);
+        const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
         for (;;) {
             // Step table of the chunk and the host's copy of the check pattern.
             if (!ensure_steptab(ctx, len)) return false;
@@ -3766,7 +4257,29 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     g_slot = static_cast<int>(k * sub);
                     mark(ctx, 6);
                     const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
-                    if (pattern[static_cast<size_t>(k)]) {
+                    if (ctx->split_learned && pattern[static_cast<size_t>(k)]) {
+                        const Planes pw = dense_planes(ctx, 0);  // free until the screening's windows
+                        k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
+                        k_base_step<KIND_CHECK><<<grid_of(n), BLOCK, 0, s>>>(pc, pw, ctx->alive, nullptr,
+                                                                             ctx->counts, n, m13, lp);
+                        k_check_plain<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, m13, lp);
+                        k_net_step<true><<<std::min(grid_of(n), 2 * g_net), BLOCK, 0, s>>>(
+                            pc, pn, pw, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                        k_build_lists<<<std::max(1, std::min(grid_of(n), LIST_BLOCKS)), BLOCK, 0, s>>>(
+                            ctx->alive, ctx->mode, n, ctx->lists, static_cast<int>(c), ctx->counts);
+                    } else if (ctx->split_learned) {
+                        k_base_step<net::CLASS_LEARNED><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists,
+                                                                                ctx->counts, n, m13, lp);
+                        k_net_step<false><<<2 * g_net, BLOCK, 0, s>>>(
+                            pc, pn, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                        k_base_step<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists + 2 * c,
+                                                                             ctx->counts, n, m13, lp);
+                        k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
+                            pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                    } else if (pattern[static_cast<size_t>(k)]) {
                         k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
                         k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
                             pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
@@ -3785,6 +4298,17 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                     }
                     mark(ctx, 7);
+                    // The sample's plain step runs on the side stream beside
+                    // the screening (it reads the end-of-step states and
+                    // flags, and writes only its own arrays); joined before
+                    // anything writes the flags again.
+                    if (follow) {
+                        KHIP_TRY(ctx, hipEventRecord(ctx->ev_fork, s));
+                        KHIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                        k_sample_step<<<(ctx->sample_n + BLOCK - 1) / BLOCK, BLOCK, 0, ctx->side>>>(
+                            ctx->sample, ctx->sample_n, pn, ctx->alive, m13, h, sub, ctx->sample_dev);
+                        KHIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+                    }
                     if (screen) {
                         // Screening windows of tau / nw, each spw sub-steps long;
                         // up to FUSED_WINDOWS_MAX windows' states and reductions
@@ -3811,14 +4335,15 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                                 info, q, nw, spw,
                                                 fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
                                                 q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr,
-                                                shells ? q : -1))
+                                                shells ? q : -1, vmin_search))
                                 return false;
                             pending = mine;
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                                   static_cast<unsigned long long>(ctx->ev_cap), pc,
                                                                   ctx->radius, m13, info, h, sub,
-                                                                  ctx->cfg.threshold_km, vmin2, ctx->rstats);
+                                                                  ctx->cfg.threshold_km + band, vmin2, ctx->rstats);
+                        if (follow) KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
                         if (bk) {
                             // The step's confirmed collisions, in (t, i, j) order:
                             // parents at the collision instant from the step's
@@ -3844,6 +4369,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     } else {
                         k_finish<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->stats, 0, ctx->pair_parts, 0, 0, ctx->dense_parts,
                                                      0, ctx->exact_parts, 0, 0);
+                        if (follow) KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
                     }
                     cur = nxt;
                 }
@@ -3851,7 +4377,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             };
             if (ctx->use_graph && !ctx->timing && !bk) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
-                                            shells ? ctx->elaunch : 0};
+                                            shells ? ctx->elaunch : 0, __builtin_bit_cast(int64_t, thr),
+                                            __builtin_bit_cast(int64_t, vmin2), follow ? ctx->sample_n : -1};
                 key.insert(key.end(), pattern.begin(), pattern.end());
                 if (!launch_graph(ctx, key, enqueue_chunk)) return false;
             } else if (!enqueue_chunk()) {
@@ -3918,6 +4445,15 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 r.events_dropped += static_cast<int64_t>(rs.dropped);
                 r.events_unverified += static_cast<int64_t>(rs.kept_unverified);
                 r.steps += len;
+                r.band_km = band;
+                r.band_v_km_s = band_v;
+                if (follow) {
+                    unsigned long long dv[2] = {0, 0};
+                    KHIP_TRY(ctx, hipMemcpy(dv, ctx->sample_dev, sizeof(dv), hipMemcpyDeviceToHost));
+                    r.sample_dev_max_km = __builtin_bit_cast(double, dv[0]);
+                    r.sample_dev_v_max_km_s = __builtin_bit_cast(double, dv[1]);
+                    r.sample_objects = ctx->sample_n;
+                }
                 break;
             }
             ctx->tm.reruns += 1;
@@ -3930,6 +4466,11 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->counts, ctx->snap_lists + 3 * c, sizeof(int) * 3,
                                          hipMemcpyDeviceToDevice, s));
             KHIP_TRY(ctx, hipMemcpyAsync(ctx->cstats, &cs0, sizeof(cs0), hipMemcpyHostToDevice, s));
+            if (follow) {
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->sample, ctx->snap_sample, sizeof(SampleObj) * ctx->sample_n,
+                                             hipMemcpyDeviceToDevice, s));
+                KHIP_TRY(ctx, hipMemcpyAsync(ctx->sample_dev, dev0, sizeof(dev0), hipMemcpyHostToDevice, s));
+            }
             if (bk) {
                 KHIP_TRY(ctx, hipMemcpyAsync(ctx->mass, ctx->snap_mass, sizeof(double) * c, hipMemcpyDeviceToDevice, s));
                 KHIP_TRY(ctx, hipMemcpyAsync(ctx->bstate, ctx->snap_bstate, sizeof(bu::BreakupState),
@@ -3988,6 +4529,17 @@ bool autotune(khip_ctx* ctx, int64_t steps, bool allow_cell, double& base_ms, do
     const int cur0 = ctx->cur;
     const bool force0 = ctx->force_check, timing0 = ctx->timing, graph0 = ctx->use_graph;
     const khip_learned_report lrep0 = ctx->lrep;
+    // The followed sample, when one is running, goes back with the rest.
+    const bool sreset0 = ctx->sample_reset;
+    const int sn0 = ctx->sample_n;
+    const double sample_t00 = ctx->sample_t0;
+    SampleObj* sm = nullptr;
+    unsigned long long* sd = nullptr;
+    if (learned && !sreset0 && ctx->sample) {
+        if (!sc.get(ctx, &sm, SAMPLE_MAX) || !sc.get(ctx, &sd, 2)) return false;
+        KHIP_TRY(ctx, hipMemcpyAsync(sm, ctx->sample, sizeof(SampleObj) * SAMPLE_MAX, hipMemcpyDeviceToDevice, s));
+        KHIP_TRY(ctx, hipMemcpyAsync(sd, ctx->sample_dev, 2 * sizeof(*sd), hipMemcpyDeviceToDevice, s));
+    }
     const khip_timing tm0 = ctx->tm;
     std::vector<khip_event> ev0;
     std::vector<khip_stats> st0;
@@ -4009,6 +4561,13 @@ bool autotune(khip_ctx* ctx, int64_t steps, bool allow_cell, double& base_ms, do
         ctx->cur = cur0;
         ctx->force_check = force0;
         ctx->lrep = lrep0;
+        ctx->sample_reset = sreset0;
+        ctx->sample_n = sn0;
+        ctx->sample_t0 = sample_t00;
+        if (sm) {
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->sample, sm, sizeof(SampleObj) * SAMPLE_MAX, hipMemcpyDeviceToDevice, s));
+            KHIP_TRY(ctx, hipMemcpyAsync(ctx->sample_dev, sd, 2 * sizeof(*sd), hipMemcpyDeviceToDevice, s));
+        }
         ctx->host_events.clear();
         ctx->host_stats.clear();
         return true;
@@ -4094,6 +4653,7 @@ bool upload_objects(khip_ctx* ctx, int64_t n, const double* states, const uint8_
     ctx->n = n;
     ctx->n_launch = n;
     ctx->cur = 0;
+    ctx->sample_reset = true;
     ctx->host_size.assign(static_cast<size_t>(n), 0.8);
     if (n == 0) return true;
     const size_t c = static_cast<size_t>(ctx->cap);
@@ -4477,8 +5037,8 @@ constexpr size_t scr_bytes(int r) {
     case R_IN_SIZE: case R_RADIUS: return N * sizeof(double);
     case R_KEYS0: case R_KEYS1: case R_CELL_KEY: return N * sizeof(uint64_t);
     case R_IDX0: case R_IDX1: case R_CELL_BEGIN: case R_CELL_COUNT: case R_CELL_OF: return N * sizeof(int);
-    case R_HIST: return RADIX * SCR_TILES * sizeof(int);
-    case R_TOTALS: return RADIX * sizeof(int);
+    case R_HIST: return 2 * RADIX * SCR_TILES * sizeof(int);
+    case R_TOTALS: return MAX_PASSES * RADIX * sizeof(int);
     case R_BCOUNT: return SCR_TILES * sizeof(int);
     case R_RANGES: return NRANGES * N * sizeof(int2);
     case R_LREC: case R_REC: return 2 * N * sizeof(float4);
@@ -4670,6 +5230,10 @@ bool screen_static(khip_ctx* ctx, int64_t n, const double* before, const double*
         return false;
     }
     code = KHIP_ERR_STATE;
+    if (g.sort_fault) {
+        ctx->error = "khip_screen_static: a radix pass's look-back did not complete";
+        return false;
+    }
     if (g.surv_short || g.items_short || static_cast<int64_t>(g.events) > SCR_EVENTS) {
         ctx->error = "khip_screen_static: this screening exceeds the fixed buffers (" +
                      std::to_string(g.events) + " events, " + std::to_string(g.surv_max) +
@@ -4751,6 +5315,8 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     }
     khip_ctx* ctx = new khip_ctx;
     ctx->device = device;
+    if (const char* f = std::getenv("KHIP_LEARNED_FUSED")) ctx->split_learned = std::atoi(f) == 0;
+    if (const char* f = std::getenv("KHIP_SORT_CLASSIC")) ctx->onesweep = std::atoi(f) == 0;
     ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     char buf[512];
     std::snprintf(buf, sizeof(buf), "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName,
@@ -4783,9 +5349,13 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
 void khip_destroy(khip_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     free_all(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -4886,6 +5456,7 @@ int khip_set_time(khip_ctx* ctx, double epoch_jd, double t, double dt) {
     ctx->dt = dt;
     ctx->step = 0;
     ctx->gen += 1;
+    ctx->sample_reset = true;
     return KHIP_OK;
 }
 
@@ -4899,6 +5470,7 @@ int khip_get_time(const khip_ctx* ctx, double* t, int64_t* step) {
 int khip_run(khip_ctx* ctx, int64_t nsteps, int screen) {
     if (!ctx || nsteps < 0) return KHIP_ERR_ARG;
     if (ctx->precision == KHIP_PRECISION_LEARNED) return status(ctx, run_steps_learned(ctx, nsteps, screen));
+    ctx->sample_reset = true;  // the learned form's sample does not follow other forms
     return status(ctx, run_steps(ctx, nsteps, screen));
 }
 
@@ -5036,10 +5608,12 @@ int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg) {
         cfg->check_every < 1 || !(cfg->tol_km >= 0) || !(cfg->margin_km >= 0) || !(cfg->e_max >= 0) ||
         !(cfg->a_max_km >= 0) || !(cfg->tol_a_km >= 0) || cfg->windows < 0 ||
         (cfg->windows > 0 && cfg->substeps % cfg->windows != 0) ||
-        !(cfg->shell_km == 0 || (cfg->shell_km >= 1.0 && cfg->shell_km < 1e5)))
+        !(cfg->shell_km == 0 || (cfg->shell_km >= 1.0 && cfg->shell_km < 1e5)) ||
+        !(cfg->margin_factor >= 0 && cfg->margin_factor < 1e6))
         return KHIP_ERR_ARG;
     ctx->lcfg = *cfg;
     ctx->force_check = true;
+    ctx->sample_reset = true;
     ctx->gen += 1;
     return KHIP_OK;
 }

@@ -81,7 +81,7 @@ def trained(kamd):
     eng = kamd.Engine(0)
     orbits = kamd.generate(500000, seed=101)["state"]
     eng.set_time(2461253.9166666665, 0.0, 60.0)
-    rep, curve = eng.net_train(orbits, samples_per_orbit=10, spacing_steps=12, iterations=20000,
+    rep, curve = eng.net_train(orbits, samples_per_orbit=15, spacing_steps=12, iterations=20000,
                                batch=16384, lr_max=3e-3, seed=1)
     params, norm = eng.net_get()
     eng.close()

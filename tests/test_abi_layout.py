@@ -23,9 +23,10 @@ C_FIELDS = {
     "khip_timing": ["propagate_ms", "grid_ms", "pairs_ms", "pairs_sparse_ms", "pairs_dense_ms", "exact_ms",
                     "total_ms", "steps", "reruns", "rerun_reasons", "reverify_ms"],
     "khip_learned_config": ["substeps", "check_every", "tol_km", "margin_km", "e_max", "a_max_km", "tol_a_km",
-                            "windows", "shell_km"],
+                            "windows", "margin_factor", "shell_km"],
     "khip_learned_report": ["learned", "base", "plain", "rejected", "dev_max_km", "checks", "steps",
-                            "events_verified", "events_dropped", "events_unverified"],
+                            "events_verified", "events_dropped", "events_unverified", "sample_dev_max_km",
+                            "sample_dev_v_max_km_s", "band_km", "band_v_km_s", "sample_objects"],
     "khip_breakup_config": ["enabled", "reserve", "lc_m", "catastrophic_j_per_g", "seed"],
     "khip_breakup_event": ["t", "v_rel_km_s", "energy_j_per_g", "target_mass_kg", "projectile_mass_kg",
                            "sbm_mass_kg", "expected_fragments", "fragments_mass_kg", "position", "id", "target",

@@ -27,7 +27,7 @@ def scenarios(kamd):
 def net(kamd):
     eng = kamd.Engine(0)
     eng.set_time(EPOCH, 0.0, 60.0)
-    eng.net_train(kamd.generate(200000, seed=101)["state"], samples_per_orbit=10, spacing_steps=12,
+    eng.net_train(kamd.generate(200000, seed=101)["state"], samples_per_orbit=15, spacing_steps=12,
                   iterations=3000, batch=16384, seed=1)
     p, norm = eng.net_get()
     eng.close()

@@ -156,9 +156,10 @@ def test_learned_screening_recall_at_2km(engine, kamd, trained):
         worst_m = max(worst_m, abs(ml - m))
     # Precision: every learned event is a plain event (same pair, same or
     # adjacent sub-step), except pairs on the threshold's edge, where the
-    # learned and plain trajectories' metre-level difference decides.
+    # learned and plain trajectories' metre-level difference decides, and
+    # those within the drift band above it.
     extra = [(key, m) for key, (t, m) in kl.items()
-             if not any(k2 in kp for k2 in ((key[0], key[1], key[2] + d) for d in (0, -1, 1)))]
+             if m <= 2.0 and not any(k2 in kp for k2 in ((key[0], key[1], key[2] + d) for d in (0, -1, 1)))]
     below = int((el["miss_km"] < 2.0).sum())
     print(f"1 h at 2 km: plain {len(kp)} events, learned {len(el)} ({below} below 2 km), missing {len(missing)}, "
           f"extra {len(extra)} (miss {[round(m, 6) for _, m in extra[:5]]}), "
@@ -172,7 +173,7 @@ def test_learned_screening_recall_at_2km(engine, kamd, trained):
     # Events are plain-verified at the configured threshold itself (the
     # raised one serves the search only) and sorted by (step, t, i, j).
     assert rep["events_verified"] + rep["events_unverified"] >= len(el)
-    assert (el["miss_km"] <= 2.0).all()
+    assert (el["miss_km"] <= 2.0 + rep["band_km"]).all()
     order = np.lexsort((el["j"], el["i"], el["t"], el["step"]))
     assert np.array_equal(order, np.arange(len(el)))
 
@@ -235,3 +236,99 @@ def test_radial_shells_do_not_change_events(engine, kamd, trained):
             assert np.array_equal(st["propagated"], st0["propagated"])
             assert c < c0
         assert len(ev0) > 50
+
+
+def _keys(ev):
+    return (ev["i"].astype(np.int64) << 39) | (ev["j"].astype(np.int64) << 15) | ev["step"].astype(np.int64)
+
+
+def _found(e_ref, e_new):
+    """Events of e_ref present in e_new (same pair, same or adjacent sub-step)."""
+    kr, kn = _keys(e_ref), _keys(e_new)
+    return np.isin(kr, kn) | np.isin(kr - 1, kn) | np.isin(kr + 1, kn)
+
+
+def test_drift_band_follows_the_measured_drift(engine, kamd, trained):
+    """The drift band comes from the measured learned-vs-plain drift. The
+    followed sample's plain twins step exactly as a plain run does, so its
+    D is at least the sampled objects' drift against the plain run at the
+    end; a degraded corrector (weights perturbed by 10%, tolerances loosened
+    so its objects keep the learned method) drifts more, and D and the band
+    grow with it; the band stays at least the pair drift the whole
+    population reached (twice its largest drift); with the band, every event
+    of the plain run is reported (recall 1.0) for both correctors, and the
+    events the plain run does not have lie within the drift's reach of the
+    threshold. Without the band (margin_factor 0) the degraded run's recall
+    is printed for contrast."""
+    _, _, params, norm = trained
+    pop = kamd.generate(60000, seed=10, mix=MIX_DENSE)
+    hours = 6
+    s64, a64, e64, _ = _run(engine, pop, 0, hours * 360, 10.0, True)
+    rng = np.random.default_rng(3)
+    bad = (params * (1.0 + 0.10 * rng.standard_normal(params.shape))).astype(np.float32)
+    loose = dict(tol_km=5e-3, tol_a_km=1e-5)
+    runs = {}
+    try:
+        for name, p, kw in (("trained", params, {}), ("degraded", bad, loose),
+                            ("degraded, no band", bad, dict(loose, margin_factor=0.0))):
+            engine.net_set(p, norm)
+            sl, al, el, _ = _run(engine, pop, 2, hours * 60, 60.0, True, **kw)
+            runs[name] = (sl, al, el, engine.learned_report())
+    finally:
+        engine.net_set(params, norm)
+        engine.set_learned()
+    ns = min(8192, len(pop))
+    idx = np.arange(ns) * max(1, len(pop) // 8192)
+    assert len(e64) > 500
+    out = {}
+    for name, (sl, al, el, rep) in runs.items():
+        both = (a64 == 1) & (al == 1)
+        drift = np.linalg.norm(sl[:, :3] - s64[:, :3], axis=1)
+        d_sample = drift[idx][both[idx]].max()
+        found = _found(e64, el)
+        extra = ~_found(el, e64)
+        out[name] = (rep, d_sample, drift[both].max(), found)
+        print(f"{name}: D {rep['sample_dev_max_km'] * 1e3:.3f} m (sampled objects at the end "
+              f"{d_sample * 1e3:.3f} m, population {drift[both].max() * 1e3:.3f} m), band "
+              f"{rep['band_km'] * 1e3:.3f} m; {len(e64)} plain events, learned {len(el)}, recall "
+              f"{found.mean():.6f}, not in the plain run {int(extra.sum())} "
+              f"(miss {el['miss_km'][extra].min() if extra.any() else 0:.4f} to "
+              f"{el['miss_km'][extra].max() if extra.any() else 0:.4f} km); methods {rep['learned']} learned, "
+              f"{rep['plain']} plain")
+        if name == "degraded, no band":
+            assert rep["band_km"] == 0.0
+            continue
+        assert rep["sample_objects"] == ns
+        assert rep["sample_dev_max_km"] >= d_sample
+        assert found.all(), name
+        assert (el["miss_km"] <= 2.0 + rep["band_km"]).all()
+        assert (el["miss_km"][extra] > 2.0 - 2 * drift[both].max() - 1e-3).all()
+        assert rep["band_km"] >= 2 * drift[both].max()
+    (rt, _, _, _), (rd, _, _, _) = out["trained"], out["degraded"]
+    assert rd["sample_dev_max_km"] > 3 * rt["sample_dev_max_km"]
+    assert rd["band_km"] > rt["band_km"]
+
+
+def test_split_kernels_equal_fused(kamd, trained, monkeypatch):
+    """The learned step with its FP64 work in full-occupancy kernels and the
+    corrector alone in the MFMA kernel gives the one-kernel-per-method
+    step's results bit for bit (states, flags, events, statistics and
+    method counts) over checks, corrected, base and plain steps."""
+    _, _, params, norm = trained
+    pop = kamd.generate(60000, seed=11, mix=MIX_DENSE)
+    out = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("KHIP_LEARNED_FUSED", fused)
+        eng = kamd.Engine(0)
+        try:
+            eng.net_set(params, norm)
+            s, a, ev, st = _run(eng, pop, 2, 45, 60.0, True)
+            out[fused] = (s, a, ev, st, eng.learned_report())
+        finally:
+            eng.close()
+    (s0, a0, e0, t0, r0), (s1, a1, e1, t1, r1) = out["0"], out["1"]
+    print(f"split vs fused: {len(e0)} events, methods {r0['learned']}/{r0['base']}/{r0['plain']}")
+    assert np.array_equal(s0, s1) and np.array_equal(a0, a1)
+    assert e0.tobytes() == e1.tobytes() and t0.tobytes() == t1.tobytes()
+    assert r0 == r1
+    assert len(e0) > 50 and r0["learned"] > 0 and r0["base"] > 0 and r0["plain"] > 0
