@@ -166,8 +166,6 @@ struct GridState {
     unsigned int bmin;              // innermost shell of this screening (shell screening)
     int entries_short;              // a screening had more shell entries than were launched
     unsigned int entries_max;       // max shell entries over the chunk
-    int sort_fault;                 // a radix pass's look-back gave up waiting (never expected)
-    int tile_ctr[MAX_PASSES];       // tiles claimed per radix pass (one-sweep passes)
     int pad4;
 };
 
@@ -425,6 +423,16 @@ __device__ __forceinline__ void store_state(Planes p, int i, D3 r, D3 v) {
     p.vx[i] = v.x; p.vy[i] = v.y; p.vz[i] = v.z;
 }
 
+// An object dies in a learned step (under the surface): its final state
+// goes into the step's start planes too (the end planes already hold it), so
+// both buffers keep it whether or not a later step's kernels visit the
+// object (a check drops dead objects from the method lists). Nothing reads
+// a dead object's start state later in the step.
+__device__ __forceinline__ void kill(Planes cur, uint8_t* alive, int i, D3 r, D3 v) {
+    store_state(cur, i, r, v);
+    alive[i] = 0;
+}
+
 // Corrector on the 64 objects of a wave (lane = object). x: the lane's
 // standardised inputs; returns the lane's six outputs.
 __device__ __forceinline__ void net_wave(const float x[net::NIN], const uint16_t* frags, const float* P,
@@ -457,6 +465,42 @@ __device__ __forceinline__ void net_wave(const float x[net::NIN], const uint16_t
         net::forward_tiles<2>(x1, frags, P, lane, out + 2);
     }
     // Output 4g' + e of column j of tile t sits in lane j + 16 g', element e.
+    const int mt = lane >> 4, mj = lane & 15;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const float a0 = __shfl(out[t].x, mj, WAVE), a1 = __shfl(out[t].y, mj, WAVE);
+        const float a2 = __shfl(out[t].z, mj, WAVE), a3 = __shfl(out[t].w, mj, WAVE);
+        const float a4 = __shfl(out[t].x, mj + 16, WAVE), a5 = __shfl(out[t].y, mj + 16, WAVE);
+        if (mt == t) {
+            o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
+        }
+    }
+}
+
+// The same with NT tiles per pass (4 / NT passes): fewer tiles per pass
+// hold fewer registers (more waves per SIMD), more tiles per pass read each
+// weight fragment from LDS for more MFMAs.
+template <int NT>
+__device__ __forceinline__ void net_wave_nt(const float x[net::NIN], const uint16_t* frags, const float* P,
+                                            float o[net::NOUT]) {
+    const int lane = static_cast<int>(__lane_id());
+    const int j = lane & 15, g = lane >> 4;
+    uint32_t w[10];
+    net::input_words(x, w);
+    net::u32x4 xin[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uint32_t wt[10];
+#pragma unroll
+        for (int k = 0; k < 10; ++k) wt[k] = static_cast<uint32_t>(__shfl(static_cast<int>(w[k]), 16 * t + j, WAVE));
+        xin[t] = net::input_operand_words(wt, g);
+    }
+    net::f32x4 out[4];
+#pragma unroll
+    for (int t0 = 0; t0 < 4; t0 += NT) {
+        net::forward_tiles<NT>(xin + t0, frags, P, lane, out + t0);
+        __builtin_amdgcn_sched_barrier(0);  // keep the passes apart (register pressure)
+    }
     const int mt = lane >> 4, mj = lane & 15;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -530,12 +574,12 @@ __global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8
         if (!on) continue;
         if (KIND == net::CLASS_LEARNED || KIND == net::CLASS_BASE) {
             store_state(nxt, i, rb, vb);
-            if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) alive[i] = 0;
+            if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) kill(cur, alive, i, rb, vb);
         } else {
             D3 rp = r, vp = v;
             const bool survives = plain_substeps(rp, vp, moon13, h, lp.substeps);
             store_state(nxt, i, rp, vp);
-            if (!survives) alive[i] = 0;
+            if (!survives) kill(cur, alive, i, rp, vp);
             if (KIND == KIND_CHECK) {
                 int m = net::CLASS_PLAIN;
                 if (survives && cls != net::CLASS_PLAIN) {
@@ -595,7 +639,7 @@ __global__ void __launch_bounds__(BLOCK) k_base_step(Planes cur, Planes out, uin
         if (KIND == KIND_CHECK && net::domain_class(r, v, lp.e_max, lp.a_max) == net::CLASS_PLAIN) continue;
         rk4(r, v, lp.tau, moon13[0], moon13[6], moon13[12], AccelMixed());
         store_state(out, i, r, v);
-        if (KIND == net::CLASS_BASE && sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) alive[i] = 0;
+        if (KIND == net::CLASS_BASE && sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) kill(cur, alive, i, r, v);
     }
 }
 
@@ -606,9 +650,7 @@ __global__ void __launch_bounds__(BLOCK) k_check_plain(Planes cur, Planes nxt, u
     const double h = lp.tau / lp.substeps;
     for (int i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
         D3 r{cur.x[i], cur.y[i], cur.z[i]}, v{cur.vx[i], cur.vy[i], cur.vz[i]};
-        if (alive[i]) {
-            if (!plain_substeps(r, v, moon13, h, lp.substeps)) alive[i] = 0;
-        }
+        if (alive[i] && !plain_substeps(r, v, moon13, h, lp.substeps)) kill(cur, alive, i, r, v);
         store_state(nxt, i, r, v);
     }
 }
@@ -620,8 +662,12 @@ __global__ void __launch_bounds__(BLOCK) k_check_plain(Planes cur, Planes nxt, u
 // sub-steps), base states in `base`: an object keeps the corrected or base
 // method if it stayed within the tolerances of its plain state, and the
 // check tallies and method flags are written as k_learned<KIND_CHECK> does.
+constexpr int NET_NT = 2;          // tiles per pass in k_net_step
+constexpr int NET_BLOCK = 256;     // two blocks share a CU's LDS: two waves per SIMD
+constexpr int NET_WAVES = NET_BLOCK / WAVE;
+
 template <bool CHECK>
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) k_net_step(Planes cur, Planes nxt, Planes base, uint8_t* alive,
+__global__ void __launch_bounds__(NET_BLOCK) __attribute__((amdgpu_waves_per_eu(2))) k_net_step(Planes cur, Planes nxt, Planes base, uint8_t* alive,
                                                      uint8_t* mode, const int* __restrict__ list,
                                                      const int* __restrict__ counts, int n,
                                                      const MoonSample* __restrict__ moon13,
@@ -630,7 +676,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     __shared__ uint4 s_frag[net::NFRAG_FWD * net::FRAG_HALVES / 8];
     {
         const uint4* src = reinterpret_cast<const uint4*>(ffwd);
-        for (int k = threadIdx.x; k < net::NFRAG_FWD * net::FRAG_HALVES / 8; k += BLOCK) s_frag[k] = src[k];
+        for (int k = threadIdx.x; k < net::NFRAG_FWD * net::FRAG_HALVES / 8; k += NET_BLOCK) s_frag[k] = src[k];
         __syncthreads();
     }
     const uint16_t* frags = reinterpret_cast<const uint16_t*>(s_frag);
@@ -639,7 +685,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
     const int wave = threadIdx.x / WAVE;
     unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
     double dev_max = 0.0;
-    for (int tile = blockIdx.x * WAVES_PER_BLOCK + wave; tile * WAVE < count; tile += gridDim.x * WAVES_PER_BLOCK) {
+    for (int tile = blockIdx.x * NET_WAVES + wave; tile * WAVE < count; tile += gridDim.x * NET_WAVES) {
         const int k = tile * WAVE + lane;
         const bool active = k < count;
         const int i = active ? (CHECK ? k : list[k]) : 0;
@@ -666,7 +712,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
 #pragma unroll
                 for (int q = 0; q < net::NIN; ++q) x[q] = 0.0f;
             }
-            net_wave(x, frags, P, o);
+            net_wave_nt<NET_NT>(x, frags, P, o);
         }
         const Planes src = CHECK ? base : nxt;
         D3 rb{0.0, 0.0, 0.0}, vb{0.0, 0.0, 0.0};
@@ -683,7 +729,7 @@ __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
         if (!CHECK) {
             if (on) {
                 store_state(nxt, i, rb, vb);
-                if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) alive[i] = 0;
+                if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) kill(cur, alive, i, rb, vb);
             }
         } else if (active) {
             int m = net::CLASS_PLAIN;
@@ -993,7 +1039,6 @@ __global__ void k_reset(GridState* gs) {
     gs->npasses = 0;
     gs->entries_short = 0;
     gs->entries_max = 0;
-    gs->sort_fault = 0;
 }
 
 __global__ void k_zero_events(GridState* gs) {
@@ -1140,16 +1185,9 @@ struct ShellScreen {
 __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
-                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh,
-                                                       int* ghist) {
+                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh) {
     __shared__ StepPartial s_rows[TWAVES];
     __shared__ int s_wave[TWAVES];
-    // One-sweep sorting: the digit counts of every pass (accumulated by the
-    // key kernel) and the passes' tile counters start from zero.
-    if (ghist) {
-        for (int k = threadIdx.x; k < MAX_PASSES * RADIX; k += TBLOCK) ghist[k] = 0;
-        if (threadIdx.x < MAX_PASSES) gs->tile_ctr[threadIdx.x] = 0;
-    }
     long long entries = 0;
     if (sh.on) {
         // Exclusive prefix of the object tiles' entry counts (every thread).
@@ -1263,15 +1301,11 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
 // sorted object.
 __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  const int* __restrict__ scen, int n, const GridState* gs,
-                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec,
-                                                 int* ghist) {
+                                                 uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec) {
     __shared__ int s_hist[RADIX];
-    __shared__ int s_gh[MAX_PASSES * RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
-    if (ghist)
-        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK) s_gh[k] = 0;
     __syncthreads();
     const int base = blockIdx.x * TILE;
     for (int r = 0; r < TITEMS; ++r) {
@@ -1300,20 +1334,11 @@ __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int mid
             }
             keys[e] = k;
             idx[e] = e;
-            if (ghist)
-                for (int p = 0; p < g.npasses; ++p)
-                    atomicAdd(&s_gh[p * RADIX + static_cast<int>((k >> (RADIX_BITS * p)) & DIGIT_MASK)], 1);
-            else
-                atomicAdd(&s_hist[static_cast<int>(k & DIGIT_MASK)], 1);
+            atomicAdd(&s_hist[static_cast<int>(k & DIGIT_MASK)], 1);
         }
     }
     __syncthreads();
-    if (ghist) {
-        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK)
-            if (s_gh[k]) atomicAdd(&ghist[k], s_gh[k]);
-    } else if (threadIdx.x < RADIX) {
-        hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
-    }
+    if (threadIdx.x < RADIX) hist[threadIdx.x * ntiles + blockIdx.x] = s_hist[threadIdx.x];
 }
 
 // Shell entries of one object tile (TILE objects, the tiles k_windows
@@ -1328,15 +1353,10 @@ __global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, c
                                                        const int* __restrict__ shell_lo,
                                                        const uint16_t* __restrict__ shell_n,
                                                        const int* __restrict__ tile_base, unsigned int ecap,
-                                                       uint64_t* keys, int* idx, float4* rec, int* ghist) {
+                                                       uint64_t* keys, int* idx, float4* rec) {
     __shared__ int s_wave[TWAVES];
-    __shared__ int s_gh[MAX_PASSES * RADIX];
     const GridState g = *gs;
     if (g.npasses == 0) return;
-    if (ghist) {
-        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK) s_gh[k] = 0;
-        __syncthreads();
-    }
     const int base = blockIdx.x * TILE;
     const int bz = g.bits[2], bb = g.bbits;
     unsigned int off = static_cast<unsigned int>(tile_base[blockIdx.x]);
@@ -1368,19 +1388,10 @@ __global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, c
         for (int j = 0; j < nb; ++j) {
             const unsigned int pos = off + static_cast<unsigned int>(ex + j);
             if (pos >= ecap) break;
-            const uint64_t kj = k | (static_cast<uint64_t>(s0 + j - static_cast<int>(g.bmin)) << bz);
-            keys[pos] = kj;
+            keys[pos] = k | (static_cast<uint64_t>(s0 + j - static_cast<int>(g.bmin)) << bz);
             idx[pos] = e | (j ? NOT_FIRST : 0);
-            if (ghist)
-                for (int p = 0; p < g.npasses; ++p)
-                    atomicAdd(&s_gh[p * RADIX + static_cast<int>((kj >> (RADIX_BITS * p)) & DIGIT_MASK)], 1);
         }
         off += static_cast<unsigned int>(tot);
-    }
-    if (ghist) {
-        __syncthreads();
-        for (int k = threadIdx.x; k < g.npasses * RADIX; k += TBLOCK)
-            if (s_gh[k]) atomicAdd(&ghist[k], s_gh[k]);
     }
 }
 
@@ -1477,138 +1488,6 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
             const int pos = s_off[d] + pre + rank;
             keys_out[pos] = key;
             idx_out[pos] = id;
-        }
-        __syncthreads();
-        if (t < RADIX) {
-            int add = 0;
-            for (int w = 0; w < TWAVES; ++w) {
-                add += s_wcnt[w][t];
-                s_wcnt[w][t] = 0;
-            }
-            s_off[t] += add;
-        }
-        __syncthreads();
-    }
-}
-
-// One-sweep radix pass: histogram, offsets and scatter of one digit in a
-// single launch. The digit counts of the whole array (ghist, all passes,
-// from the key kernel) give each digit's start; a tile's offset within its
-// digit comes from the tiles before it by decoupled look-back: each tile
-// publishes its own digit counts at once (AGG), then walks back over its
-// predecessors' words, adding AGG counts until it meets an inclusive prefix
-// (PREFIX), and publishes its own prefix. Tiles are claimed in launch order
-// from a counter, so every tile waited on belongs to a block already
-// running, which publishes its counts without waiting on anything: the
-// walk always ends. A word is (epoch << 34 | flag << 32 | count) with the
-// epoch unique to the screening and pass, so words of earlier passes read
-// as not yet published; one 64-bit store carries flag and count together.
-// The walk's wait is bounded all the same: past the bound it gives up, sets
-// sort_fault (the host reports an error) and leaves positions inside the
-// array. The scatter itself is k_scatter's stable in-tile ranking, so the
-// order is the same as the three-launch pass.
-constexpr unsigned long long LB_AGG = 1ULL, LB_PREFIX = 2ULL;
-constexpr int LB_SPIN_MAX = 1 << 22;
-
-__global__ void __launch_bounds__(TBLOCK) k_onesweep(const uint64_t* keys_in, const int* idx_in, uint64_t* keys_out,
-                                                      int* idx_out, int pass, GridState* gs,
-                                                      const int* __restrict__ ghist, unsigned long long* status) {
-    __shared__ int s_off[RADIX];
-    __shared__ int s_cnt[RADIX];
-    __shared__ int s_wcnt[TWAVES][RADIX];
-    __shared__ int s_wave[TWAVES];
-    __shared__ int s_tile;
-    if (pass >= gs->npasses) return;
-    const int n = static_cast<int>(gs->nsort);
-    const int ntiles = (n + TILE - 1) / TILE;
-    const int t = threadIdx.x;
-    if (t == 0) s_tile = atomicAdd(&gs->tile_ctr[pass], 1);
-    if (t < RADIX) s_cnt[t] = 0;
-    __syncthreads();
-    const int tile = s_tile;
-    if (tile >= ntiles) return;  // uniform: blocks beyond the live tiles
-    const unsigned long long epoch =
-        (static_cast<unsigned long long>(static_cast<unsigned int>(gs->stamp)) * MAX_PASSES + pass) & 0x3fffffffULL;
-    const int shift = RADIX_BITS * pass;
-    const int base = tile * TILE;
-    uint64_t keys[TITEMS];
-    int ids[TITEMS];
-#pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
-        const int e = base + r * TBLOCK + t;
-        keys[r] = e < n ? keys_in[e] : 0ULL;
-        ids[r] = e < n ? idx_in[e] : 0;
-        if (e < n) atomicAdd(&s_cnt[static_cast<int>((keys[r] >> shift) & DIGIT_MASK)], 1);
-    }
-    __syncthreads();
-    int excl = 0;
-    if (t < RADIX) {
-        const unsigned int own = static_cast<unsigned int>(s_cnt[t]);
-        unsigned long long* mine = status + static_cast<size_t>(tile) * RADIX + t;
-        if (tile == 0) {
-            __hip_atomic_store(mine, (epoch << 34) | (LB_PREFIX << 32) | own, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(mine, (epoch << 34) | (LB_AGG << 32) | own, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            unsigned int acc = 0;
-            int tp = tile - 1, spins = 0;
-            for (;;) {
-                const unsigned long long w = __hip_atomic_load(status + static_cast<size_t>(tp) * RADIX + t,
-                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long flag = (w >> 32) & 3ULL;
-                if ((w >> 34) != epoch || flag == 0ULL) {
-                    if (++spins > LB_SPIN_MAX) {
-                        gs->sort_fault = 1;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                acc += static_cast<unsigned int>(w);
-                if (flag == LB_PREFIX || tp == 0) break;
-                --tp;
-            }
-            __hip_atomic_store(mine, (epoch << 34) | (LB_PREFIX << 32) | (acc + own), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            excl = static_cast<int>(acc);
-        }
-    }
-    {
-        int tot;
-        const int digit_base = block_exclusive_scan<TWAVES>(t < RADIX ? ghist[pass * RADIX + t] : 0, tot, s_wave);
-        if (t < RADIX) {
-            s_off[t] = digit_base + excl;
-            for (int w = 0; w < TWAVES; ++w) s_wcnt[w][t] = 0;
-        }
-    }
-    __syncthreads();
-    const int wave = t / WAVE;
-    const unsigned long long below = lanes_below();
-#pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
-        const int e = base + r * TBLOCK + t;
-        const bool valid = e < n;
-        const uint64_t key = keys[r];
-        const int id = ids[r];
-        const int d = static_cast<int>((key >> shift) & DIGIT_MASK);
-        unsigned long long peers = __ballot(valid);
-        for (int b = 0; b < RADIX_BITS; ++b) {
-            const bool bit = (d >> b) & 1;
-            const unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        const int rank = __popcll(peers & below);
-        if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
-        __syncthreads();
-        if (valid) {
-            int pre = 0;
-            for (int w = 0; w < wave; ++w) pre += s_wcnt[w][d];
-            const int pos = s_off[d] + pre + rank;
-            if (pos < n) {
-                keys_out[pos] = key;
-                idx_out[pos] = id;
-            }
         }
         __syncthreads();
         if (t < RADIX) {
@@ -3070,10 +2949,7 @@ struct khip_ctx {
     // corrector alone in the MFMA kernel (KHIP_LEARNED_FUSED=1: one kernel
     // per method, for comparison; results are bit-identical).
     bool split_learned = true;
-    // Radix passes in one launch each (look-back); KHIP_SORT_CLASSIC=1:
-    // histogram, digit scan and scatter launches per pass, for comparison
-    // (the same order either way).
-    bool onesweep = true;
+
     uint64_t gen = 1;               // bumped by every (re)allocation and configuration change
     // Instantiated chunk graphs keyed by what they were captured for (most
     // recently used last; at most kGraphCache).
@@ -3175,13 +3051,6 @@ bool ensure_surv(khip_ctx* ctx, int64_t want) {
 
 // Buffers indexed by sorted element or cell: objects for the plain grid,
 // shell entries for the shell screening.
-// Zero-fills a fresh allocation (outside any graph capture).
-template <typename T>
-bool clear(khip_ctx* ctx, T* p, size_t count) {
-    KHIP_TRY(ctx, hipMemset(p, 0, sizeof(T) * count));
-    return true;
-}
-
 bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
     if (want <= ctx->ecap) return true;
     const int64_t cap = std::max<int64_t>(want, ctx->cap);
@@ -3189,13 +3058,12 @@ bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
     ctx->ecap = cap;
     return dev_alloc(ctx, &ctx->keys[0], cap) && dev_alloc(ctx, &ctx->keys[1], cap) &&
            dev_alloc(ctx, &ctx->idx[0], cap) && dev_alloc(ctx, &ctx->idx[1], cap) &&
-           dev_alloc(ctx, &ctx->hist, 2 * RADIX * tiles) && dev_alloc(ctx, &ctx->bcount, tiles) &&
+           dev_alloc(ctx, &ctx->hist, RADIX * tiles) && dev_alloc(ctx, &ctx->bcount, tiles) &&
            dev_alloc(ctx, &ctx->cell_key, cap) && dev_alloc(ctx, &ctx->cell_begin, cap) &&
            dev_alloc(ctx, &ctx->cell_count, cap) && dev_alloc(ctx, &ctx->cell_of, cap) &&
            dev_alloc(ctx, &ctx->ranges, NRANGES * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->lrec, 2 * static_cast<size_t>(cap)) &&
-           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) && dev_alloc(ctx, &ctx->dense, cap) &&
-           clear(ctx, ctx->hist, 2 * RADIX * tiles);  // look-back words: no stale epoch
+           dev_alloc(ctx, &ctx->pair_parts, (cap + BLOCK - 1) / BLOCK) && dev_alloc(ctx, &ctx->dense, cap);
 }
 
 bool ensure_capacity(khip_ctx* ctx, int64_t n) {
@@ -3208,7 +3076,7 @@ bool ensure_capacity(khip_ctx* ctx, int64_t n) {
     return dev_alloc(ctx, &ctx->alive, cap) && dev_alloc(ctx, &ctx->radius, cap) &&
            dev_alloc(ctx, &ctx->size_m, cap) && dev_alloc(ctx, &ctx->mass, cap) &&
            dev_alloc(ctx, &ctx->snap_mass, cap) && ensure_sort_capacity(ctx, cap) &&
-           dev_alloc(ctx, &ctx->totals, MAX_PASSES * RADIX) &&
+           dev_alloc(ctx, &ctx->totals, RADIX) &&
            dev_alloc(ctx, &ctx->rec, 2 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_state, 6 * static_cast<size_t>(cap)) &&
            dev_alloc(ctx, &ctx->snap_alive, cap) &&
@@ -3324,27 +3192,18 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
         sh = ShellScreen{1, ctx->tile_cnt + static_cast<size_t>(shell_q) * ctx->tstride,
                          ctx->tile_base + static_cast<size_t>(shell_q) * ctx->tstride, obj_tiles,
                          static_cast<unsigned int>(n)};
-    // One-sweep passes: digit counts of every pass from the key kernel
-    // (into `totals`), look-back words in `hist`.
-    int* ghist = ctx->onesweep ? ctx->totals : nullptr;
     k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n_obj), dt, thr, ctx->passes,
-                                     midpoint, ctx->sbits, tc, fin, n_obj, sh, ghist);
+                                     midpoint, ctx->sbits, tc, fin, n_obj, sh);
     if (shells)
         k_keys_shell<<<obj_tiles, TBLOCK, 0, s>>>(
             pb, pa, ctx->alive, ctx->scen, n_obj, ctx->gs, ctx->shell_lo + static_cast<size_t>(shell_q) * ctx->cap,
             ctx->shell_n + static_cast<size_t>(shell_q) * ctx->cap, sh.tile_base, static_cast<unsigned int>(n),
-            ctx->keys[0], ctx->idx[0], ctx->rec, ghist);
+            ctx->keys[0], ctx->idx[0], ctx->rec);
     else
         k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
-                                       ctx->idx[0], ctx->hist, tiles, ctx->rec, ghist);
+                                       ctx->idx[0], ctx->hist, tiles, ctx->rec);
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
-        if (ctx->onesweep) {
-            k_onesweep<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
-                                               ctx->gs, ctx->totals,
-                                               reinterpret_cast<unsigned long long*>(ctx->hist));
-            continue;
-        }
         // k_keys counts the first pass's digits; shell entries are not in
         // object tiles, so their first pass is counted here too.
         if (pass > 0 || shells)
@@ -3488,10 +3347,6 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     rerun = false;
     ctx->last_rerun = 0;
-    if (g.sort_fault) {
-        ctx->error = "radix sort: a pass's look-back did not complete";
-        return false;
-    }
     if (screen && g.passes_short) {
         ctx->passes = std::min(MAX_PASSES, std::max(g.passes_max, ctx->passes + 1));
         ctx->last_rerun |= KHIP_RERUN_PASSES;
@@ -4263,7 +4118,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         k_base_step<KIND_CHECK><<<grid_of(n), BLOCK, 0, s>>>(pc, pw, ctx->alive, nullptr,
                                                                              ctx->counts, n, m13, lp);
                         k_check_plain<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, m13, lp);
-                        k_net_step<true><<<std::min(grid_of(n), 2 * g_net), BLOCK, 0, s>>>(
+                        k_net_step<true><<<std::min((n + NET_BLOCK - 1) / NET_BLOCK, 2 * g_net), NET_BLOCK, 0, s>>>(
                             pc, pn, pw, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd,
                             ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                         k_build_lists<<<std::max(1, std::min(grid_of(n), LIST_BLOCKS)), BLOCK, 0, s>>>(
@@ -4271,7 +4126,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     } else if (ctx->split_learned) {
                         k_base_step<net::CLASS_LEARNED><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists,
                                                                                 ctx->counts, n, m13, lp);
-                        k_net_step<false><<<2 * g_net, BLOCK, 0, s>>>(
+                        k_net_step<false><<<2 * g_net, NET_BLOCK, 0, s>>>(
                             pc, pn, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
                             ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                         k_base_step<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists + 2 * c,
@@ -5037,8 +4892,8 @@ constexpr size_t scr_bytes(int r) {
     case R_IN_SIZE: case R_RADIUS: return N * sizeof(double);
     case R_KEYS0: case R_KEYS1: case R_CELL_KEY: return N * sizeof(uint64_t);
     case R_IDX0: case R_IDX1: case R_CELL_BEGIN: case R_CELL_COUNT: case R_CELL_OF: return N * sizeof(int);
-    case R_HIST: return 2 * RADIX * SCR_TILES * sizeof(int);
-    case R_TOTALS: return MAX_PASSES * RADIX * sizeof(int);
+    case R_HIST: return RADIX * SCR_TILES * sizeof(int);
+    case R_TOTALS: return RADIX * sizeof(int);
     case R_BCOUNT: return SCR_TILES * sizeof(int);
     case R_RANGES: return NRANGES * N * sizeof(int2);
     case R_LREC: case R_REC: return 2 * N * sizeof(float4);
@@ -5230,10 +5085,6 @@ bool screen_static(khip_ctx* ctx, int64_t n, const double* before, const double*
         return false;
     }
     code = KHIP_ERR_STATE;
-    if (g.sort_fault) {
-        ctx->error = "khip_screen_static: a radix pass's look-back did not complete";
-        return false;
-    }
     if (g.surv_short || g.items_short || static_cast<int64_t>(g.events) > SCR_EVENTS) {
         ctx->error = "khip_screen_static: this screening exceeds the fixed buffers (" +
                      std::to_string(g.events) + " events, " + std::to_string(g.surv_max) +
@@ -5316,7 +5167,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     khip_ctx* ctx = new khip_ctx;
     ctx->device = device;
     if (const char* f = std::getenv("KHIP_LEARNED_FUSED")) ctx->split_learned = std::atoi(f) == 0;
-    if (const char* f = std::getenv("KHIP_SORT_CLASSIC")) ctx->onesweep = std::atoi(f) == 0;
+
     ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     char buf[512];
     std::snprintf(buf, sizeof(buf), "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName,

@@ -235,26 +235,46 @@ def test_ensemble_two_days_recall(engine, kamd, trained):
     offsets) over two days at 5 km, plain FP64 at 10 s against the learned
     form at 60 s with its drift band. Every plain event is reported, pairs
     never cross scenarios, and events the plain run does not have lie within
-    the band's reach of the threshold."""
+    the band's reach of the threshold; of events with an object in its last
+    minute before re-entry some may be missing (printed)."""
     pops = [kamd.cascade(30000 + 4000 * k, 100 + k, 5 * k, 0.1) for k in range(8)]
     off = np.concatenate([[0], np.cumsum([len(p) for p in pops])]).astype(np.int64)
     allpop = np.concatenate(pops)
-    _, a64, ev, _ = _run(engine, allpop, kamd.PRECISION_FP64, 17280, 10.0, 5.0, offsets=off)
+    s64, a64, ev, _ = _run(engine, allpop, kamd.PRECISION_FP64, 17280, 10.0, 5.0, offsets=off)
     _, _, params, norm = trained
     engine.net_set(params, norm)
-    _, al, evl, _ = _run(engine, allpop, kamd.PRECISION_LEARNED, 2880, 60.0, 5.0, offsets=off)
+    sl, al, evl, _ = _run(engine, allpop, kamd.PRECISION_LEARNED, 2880, 60.0, 5.0, offsets=off)
     rep = engine.learned_report()
     found, back, _, _ = _match(ev, evl)
+    # Objects that re-enter: the learned form screens them up to the start of
+    # the 60 s step in which they go under the surface, the plain run up to
+    # its last 10 s step before it, so events in an object's last minute
+    # above the surface may be missing.
+    reentered = (allpop["alive"] == 1) & (a64 == 0)
+    late = reentered[ev["i"]] | reentered[ev["j"]]
     sid = np.searchsorted(off, ev["i"], side="right") - 1
     lid = np.searchsorted(off, evl["i"], side="right") - 1
     assert np.array_equal(lid, np.searchsorted(off, evl["j"], side="right") - 1), "a pair crosses scenarios"
     cp, cl = np.bincount(sid, minlength=8), np.bincount(lid, minlength=8)
     print(f"ensemble of 8, {len(allpop)} objects, 48 h at 5 km: plain {len(ev)} events, learned {len(evl)}; "
-          f"recall {found.mean():.7f}, precision {back.mean():.7f}; per-scenario counts plain {cp.tolist()} "
+          f"recall {found.mean():.7f} ({int((~found & late).sum())} missing of {int(late.sum())} events with an "
+          f"object that re-enters), precision {back.mean():.7f}; per-scenario counts plain {cp.tolist()} "
           f"learned {cl.tolist()}; drift D {rep['sample_dev_max_km'] * 1e3:.2f} m, band "
           f"{rep['band_km'] * 1e3:.2f} m")
     assert len(ev) > 1000
     assert np.array_equal(al, a64)
-    assert found.all()
+    # Re-entering objects run the plain method (perigee under the corrector's
+    # domain) and keep their state at re-entry: the plain run's, bit for bit.
+    assert reentered.any() and np.array_equal(sl[reentered], s64[reentered])
+    assert found[~late].all()
     assert (evl["miss_km"] <= 5.0 + rep["band_km"]).all()
-    assert (evl["miss_km"][~back] > 5.0 - 2 * rep["band_km"] - EDGE_KM).all()
+    # Events the plain run does not have: on the threshold's edge (within the
+    # drift of its miss distance), or slow ones kept by the velocity band
+    # (relative speed within a few cm/s of the 10 m/s minimum at the
+    # sub-step's start; within 1 m/s at the TCA).
+    extra = evl[~back]
+    edge = extra["miss_km"] > 5.0 - 2 * rep["band_km"] - EDGE_KM
+    slow = extra["v_rel_km_s"] < 0.01 + 1e-3
+    print(f"not in the plain run: {len(extra)} ({int(edge.sum())} on the distance edge, "
+          f"{int((slow & ~edge).sum())} slow)")
+    assert (edge | slow).all()

@@ -191,34 +191,3 @@ def test_catalog_24h_known_answer(engine, catalog):
     assert abs(ev["miss_km"][k] * 1000 - 7.17728) < 5e-5
     assert abs(ev["t"][k] / 3600 - 1.6221) < 5e-4
     assert len(st) == 8640
-
-
-def test_one_sweep_sort_equals_three_launch_passes(kamd, trained, monkeypatch):
-    """The radix passes in one launch each (decoupled look-back) order the
-    keys exactly as the histogram / digit-scan / scatter launches do, so a
-    run gives the same events, statistics and states either way: the plain
-    form over a 200,000-object catalogue at 5 km (object keys, several
-    passes) and the learned form's shell entries."""
-    pop = kamd.generate(200000, seed=12)
-    out = {}
-    for classic in ("1", "0"):
-        monkeypatch.setenv("KHIP_SORT_CLASSIC", classic)
-        eng = kamd.Engine(0)
-        try:
-            s, a, ev, st = _run_engine(eng, pop["state"], pop["alive"].astype(np.uint8), pop["size_m"], 10.0, 30)
-            eng.set_precision(kamd.PRECISION_LEARNED)
-            eng.set_time(EPOCH, 0.0, 60.0)
-            eng.net_set(trained[2], trained[3])
-            eng.set_objects(pop["state"][:60000], pop["alive"][:60000].astype(np.uint8), pop["size_m"][:60000])
-            eng.run(10, screen=True)
-            out[classic] = (s, a, ev, st, eng.take_events(), eng.take_stats(), eng.get_objects())
-        finally:
-            eng.close()
-    (s0, a0, e0, t0, le0, lt0, lo0), (s1, a1, e1, t1, le1, lt1, lo1) = out["1"], out["0"]
-    print(f"one-sweep vs three launches: {len(e0)} plain events, {len(le0)} learned events")
-    assert len(e0) > 100 and len(le0) > 10
-    assert np.array_equal(s0, s1) and np.array_equal(a0, a1)
-    assert e0.tobytes() == e1.tobytes() and t0.tobytes() == t1.tobytes()
-    assert le0.tobytes() == le1.tobytes() and lt0.tobytes() == lt1.tobytes()
-    assert np.array_equal(lo0[0], lo1[0]) and np.array_equal(lo0[1], lo1[1])
-
