@@ -289,14 +289,14 @@ def recall_against(e_ref, e_new, thr=None):
 
 
 # HBM bytes per launch of the pair kernels in the learned workload (two 30 s
-# screening windows with 32 km radial shells), measured by two rocprofv3 PMC
+# screening windows with 16 km radial shells), measured by two rocprofv3 PMC
 # passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from copy kernels of
 # known size in the same runs); None until measured at this configuration.
 PAIR_TRAFFIC_B = {}
 # Screening windows per 60 s learned step (30 s each; propagation and the
 # plain re-verification keep 10 s sub-steps) and the radial shell width.
 WINDOWS = 2
-SHELL_KM = 32.0
+SHELL_KM = 16.0
 
 # BF16 matrix-core flops of one corrector evaluation (10-128-128-128-6;
 # the input and output layers count once although they run hi/lo splits).
@@ -410,8 +410,9 @@ def learned_workload(kamd, args):
         "ms_per_step": 1e3 * lrn["wall"] / steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16 corrector + f64 state + f32 perturbations", "data": "synthetic",
         "config": {"workload": ("learned corrector: 1,000,000 synthetic objects (seed 0, LEO shells x1.5), "
-                                f"60 s steps, {steps * 60 / 3600:g} h, screening 2 km (+25 m margin, two 30 s "
-                                "windows, 32 km radial shells), object-steps counted in 10 s steps"),
+                                f"60 s steps, {steps * 60 / 3600:g} h, screening 2 km (+25 m margin + measured "
+                                "drift band, two 30 s windows, 16 km radial shells), object-steps counted in "
+                                "10 s steps"),
                    "objects": int(len(pop)), "dt_s": 60.0, "threshold_km": thr},
         "plain_fp64_value": v_p, "learned_over_plain_fp64": v_l / v_p,
         "object_seconds_per_s": v_l * DT,

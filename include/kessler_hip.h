@@ -91,7 +91,7 @@ typedef struct {
     double propagate_ms;    /* k_propagate: fused RK4 step */
     double grid_ms;         /* keys, radix sort, cell table, neighbour ranges, work items */
     double pairs_ms;        /* the three kernels below */
-    double pairs_sparse_ms; /* k_pairs: objects of sparse cells, FP32 pre-filter */
+    double pairs_sparse_ms; /* k_pairs_lanes: objects of sparse cells, FP32 pre-filter */
     double pairs_dense_ms;  /* k_pairs_dense: dense-cell work items, FP32 pre-filter */
     double exact_ms;        /* k_exact: FP64 filter, TCA, radius test, event append */
     double total_ms;        /* whole steps, launch gaps included */
@@ -258,7 +258,7 @@ typedef struct {
     double shell_km;     /* radial shell width of the screening (>= 1 km, with at most 4 windows):
                             objects are binned by their radius range over the window as well
                             as by cell, and pairs are enumerated within shells only (see
-                            KHIP_PRECISION_LEARNED); 0 screens on cells alone. Default 32. */
+                            KHIP_PRECISION_LEARNED); 0 screens on cells alone. Default 16. */
 } khip_learned_config;
 
 typedef struct {
@@ -340,7 +340,7 @@ int khip_take_breakups(khip_ctx* ctx, khip_breakup_event* out, int64_t cap, int6
 typedef struct {
     double cell_mult;     /* >= 1 (default 1: the reference's cell) */
     int32_t dense_min;    /* cells with at least this many objects use work items (128) */
-    int32_t long_range;   /* sparse-cell ranges longer than this use work items (256) */
+    int32_t long_range;   /* sparse-cell ranges longer than this use work items (256; 1 to 16384) */
     int32_t seg;          /* partners per work item (256) */
     int32_t dense_blocks; /* persistent grid of the dense-cell pair kernel (1024, <= 4096) */
     int32_t exact_blocks; /* grid of the exact FP64 kernel (512, <= 2048) */

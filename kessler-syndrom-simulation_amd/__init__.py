@@ -370,7 +370,7 @@ class Engine:
         return grad, loss.value
 
     def set_learned(self, substeps=6, check_every=10, tol_km=1e-5, margin_km=0.025, e_max=0.1,
-                    a_max_km=12000.0, tol_a_km=5e-8, windows=0, shell_km=32.0, margin_factor=4.0):
+                    a_max_km=12000.0, tol_a_km=5e-8, windows=0, shell_km=16.0, margin_factor=4.0):
         cfg = LearnedConfig(substeps, check_every, tol_km, margin_km, e_max, a_max_km, tol_a_km, windows,
                             margin_factor, shell_km)
         self._check(self.lib.khip_set_learned(self.h, C.byref(cfg)), "khip_set_learned")

@@ -34,16 +34,19 @@
 //                 the cell's own tail), found through the column table;
 //                 dense cells registered for work items.
 //   k_items       work items of dense cells and long runs.
-//   k_pairs / k_pairs_dense
+//   k_pairs_lanes / k_pairs_dense
 //                 the FP32 pre-filter on every candidate pair (sparse cells:
-//                 (object, run) units ranked by length, a lane per unit;
+//                 a lane per partner against the own cell's records in LDS;
 //                 dense cells: a wave per item); survivors go to a queue.
 //   k_exact       the queue: FP64 filter_pair, Hermite + golden-section TCA,
 //                 radius test; events appended with wave-aggregated atomics.
 //   k_finish      the step's statistics record, accumulators reset.
 //
 // The learned form (KHIP_PRECISION_LEARNED) replaces k_propagate by the
-// corrector / base / plain-method kernels (k_learned) over per-method lists,
+// method kernels over per-method lists (k_base_step: the mixed 60 s step at
+// full occupancy; k_net_step: the corrector on MFMA; k_learned<PLAIN>: the
+// plain sub-steps; at checks k_check_plain for everyone), follows a sample of
+// objects on plain twins beside the screening (k_sample_step, side stream),
 // screens each step in windows on the step's cubic dense output (k_windows:
 // window boundary states, reductions and, with radial shells, each object's
 // shells; k_keys_shell: one sorted element per (object, shell)), and solves
@@ -107,8 +110,8 @@ constexpr int MAX_PASSES = 8;
 constexpr int DENSE_MIN = 128;  // defaults; the tuned values live in GridState
 // A sparse cell's range longer than LONG_RANGE (a neighbouring cloud) is
 // also handed to the work items instead of being walked lane by lane.
-// (Measured at 1M objects, 2 km: the length-ranked lane walk of k_pairs
-// beats the work items for everything but real clouds.)
+// (Measured at 1M objects, 2 km: the sparse-cell kernel beats the work items
+// for everything but real clouds.)
 constexpr int LONG_RANGE = 256;
 constexpr int CHUNK = 64;
 constexpr int SEG = 256;
@@ -153,7 +156,7 @@ struct GridState {
     unsigned long long surv_max;    // max n_surv over the chunk
     int surv_short;                 // a step had more survivors than the queue holds
     int pad2;
-    // Accumulated by k_pairs.
+    // Accumulated by the pair kernels.
     unsigned long long cand, refined, co;
     unsigned int max_cell;
     unsigned int pad1;
@@ -2087,116 +2090,140 @@ struct SurvivorSlots {
     }
 };
 
-// Sparse cells. A unit is one (own object, neighbour range) pair: the run
-// of partners the object is tested against (range 0 from the position after
-// its own). A block takes PAIR_GROUP consecutive sorted objects, ranks their
-// units by length in LDS (counting sort, longest first) and hands them to
-// its waves in rounds of 64, so the lanes of a round walk runs of nearly
-// equal length. A lane loads PAIR_BATCH partner records at a time into the
-// registers that test them (no copies of loads in flight, which would make
-// every iteration wait for its own loads). Survivors are few here: they are
-// appended one wave-aggregated atomic at a time (a per-wave reservation
-// would need wave-uniform loops and leave unused slots for k_exact).
-constexpr int PAIR_GROUP = BLOCK;   // sorted objects per block, one per thread
-constexpr int UNIT_BUCKETS = 64;    // unit length classes, longest first (unit_bucket)
-constexpr int UNIT_OBJ_BITS = 8;    // log2(PAIR_GROUP)
-constexpr int PAIR_BATCH = 4;
-static_assert((1 << UNIT_OBJ_BITS) == PAIR_GROUP && UNIT_BUCKETS == WAVE, "unit packing");
+constexpr int PAIR_GROUP = BLOCK;  // sorted elements per block of the sparse-cell pair kernel
 
-// Length class of a unit, longest first: lengths 1-32 one class each
-// (buckets 63..32), then classes 8 long (33-40 -> 31, ..., 273-280 -> 1),
-// longer -> 0.
-__device__ __forceinline__ int unit_bucket(int l) {
-    if (l <= 32) return UNIT_BUCKETS - l;
-    const int c = 31 - ((l - 33) >> 3);
-    return c > 0 ? c : 0;
-}
+// Sparse cells, partners on the lanes. A block takes PAIR_GROUP consecutive
+// sorted elements; their compact records go to LDS. The elements of one cell
+// inside the block form a segment, and a unit is (segment, neighbour range):
+// every partner of the range against every own element of the segment
+// (range 0: the partners after each own element, q > p). Each lane takes
+// one partner of one unit, holds its record in registers (the partners of a
+// unit are consecutive: coalesced loads) and walks the segment's own
+// records from LDS, where all lanes of a unit read the same record at once.
+// Per candidate that is one LDS broadcast instead of a partner-record load
+// per lane. Units and lanes are numbered by (element, range), so the lane
+// work is the same every run. Dense cells (dense_min objects or more) and
+// ranges longer than long_range go to the work items (k_pairs_dense).
+// Against one own object per lane walking its partners (global loads per
+// candidate), this measured 30% faster at 1M objects, 2 km.
+constexpr int MAX_UNITS = PAIR_GROUP * NRANGES;
+constexpr int OWN_BATCH = 4;
 
-// SHELL: the sorted elements are shell entries; a pair is tested in the
-// shell where one of its two entries is its object's first.
 template <bool SHELL>
-__global__ void __launch_bounds__(BLOCK) k_pairs(GridState* gs, PairParams P, PairPartial* parts) {
+__global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams P, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
-    __shared__ int s_hist[UNIT_BUCKETS];
-    __shared__ int s_total;
-    __shared__ int2 s_unit[PAIR_GROUP * NRANGES];
+    __shared__ float4 s_rec[2 * PAIR_GROUP];
+    __shared__ int s_cell[PAIR_GROUP];
+    __shared__ int s_lane0[PAIR_GROUP + 1];  // first lane of each element's units
+    __shared__ int s_wave[WAVES_PER_BLOCK];
     const int A = static_cast<int>(gs->nlive);
     const int g0 = blockIdx.x * PAIR_GROUP;
-    const int lane = static_cast<int>(__lane_id());
-    const int wave = threadIdx.x / WAVE;
+    const int t = threadIdx.x;
     unsigned int cand = 0;
     const float lim_max2 = pair_lim_max2(gs, P);
     if (gs->npasses > 0 && g0 < A) {  // uniform over the block
-        if (threadIdx.x < UNIT_BUCKETS) s_hist[threadIdx.x] = 0;
-        __syncthreads();
-        const int long_range = gs->long_range, dmin = gs->dense_min;
-        const int p = g0 + threadIdx.x;
-        const int c = p < A ? P.cell_of[p] : 0;
-        const bool own = p < A && P.cell_count[c] < dmin;
-        int q0[NRANGES], len[NRANGES], rank[NRANGES];
-#pragma unroll
-        for (int r = 0; r < NRANGES; ++r) {
-            int2 run = make_int2(0, 0);
-            if (own) run = P.ranges[c * NRANGES + r];
-            const int first = r == 0 ? p + 1 : run.x;
-            const int l = run.y - run.x > long_range ? 0 : run.y - first;  // long ranges: work items
-            q0[r] = first;
-            len[r] = l > 0 ? l : 0;
-            rank[r] = 0;
-            if (l > 0) rank[r] = atomicAdd(&s_hist[unit_bucket(l)], 1);
+        const int gend = min(A, g0 + PAIR_GROUP);
+        const int p = g0 + t;
+        const int c = p < A ? P.cell_of[p] : -1;
+        s_cell[t] = c;
+        if (p < A) {
+            s_rec[2 * t] = P.lrec[2 * p];
+            s_rec[2 * t + 1] = P.lrec[2 * p + 1];
         }
         __syncthreads();
-        // Exclusive scan of the bucket counts (one wave, one bucket per lane).
-        if (wave == 0) {
-            const int v = s_hist[lane];
-            int incl = v;
-            for (int d = 1; d < WAVE; d <<= 1) {
-                const int t = __shfl_up(incl, d, WAVE);
-                if (lane >= d) incl += t;
+        // Segment heads: the block's first element of each cell.
+        const bool head = p < A && (t == 0 || s_cell[t - 1] != c);
+        int m = 0;
+        int2 run[NRANGES];
+        int lanes[NRANGES];
+        int lane_sum = 0;
+        if (head && P.cell_count[c] < gs->dense_min) {
+            while (t + m < gend - g0 && s_cell[t + m] == c) ++m;
+#pragma unroll
+            for (int r = 0; r < NRANGES; ++r) {
+                run[r] = P.ranges[c * NRANGES + r];
+                int k = 0;
+                if (run[r].y - run[r].x <= gs->long_range) k = r == 0 ? run[r].y - (p + 1) : run[r].y - run[r].x;
+                lanes[r] = k > 0 ? k : 0;
+                lane_sum += lanes[r];
             }
-            s_hist[lane] = incl - v;
-            if (lane == WAVE - 1) s_total = incl;
-        }
-        __syncthreads();
-        const int total = s_total;
+        } else {
 #pragma unroll
-        for (int r = 0; r < NRANGES; ++r) {
-            const int l = len[r];
-            if (l == 0) continue;
-            const int bucket = unit_bucket(l);
-            s_unit[s_hist[bucket] + rank[r]] =
-                make_int2(q0[r], static_cast<int>(threadIdx.x) | (r << UNIT_OBJ_BITS) | (l << (UNIT_OBJ_BITS + 3)));
+            for (int r = 0; r < NRANGES; ++r) {
+                run[r] = make_int2(0, 0);
+                lanes[r] = 0;
+            }
+        }
+        int total;
+        const int lane0 = block_exclusive_scan(lane_sum, total, s_wave);
+        s_lane0[t] = lane0;
+        if (t == 0) s_lane0[PAIR_GROUP] = total;
+        // Unit descriptors stay in the head thread's registers; lanes find
+        // their element by binary search over s_lane0 and read the rest
+        // from LDS copies written here.
+        // Per (element, range): first partner, own count | lanes << 16 (own
+        // counts are at most PAIR_GROUP, lanes at most long_range).
+        __shared__ int2 s_unit[MAX_UNITS];
+        if (head) {
+#pragma unroll
+            for (int r = 0; r < NRANGES; ++r)
+                s_unit[t * NRANGES + r] = make_int2(r == 0 ? p + 1 : run[r].x, m | (lanes[r] << 16));
         }
         __syncthreads();
         const float cellf = static_cast<float>(gs->cell);
-        for (int u = wave * WAVE + lane; u - lane < total; u += BLOCK) {
-            if (u >= total) continue;
-            const int2 d = s_unit[u];
-            const int po = g0 + (d.y & (PAIR_GROUP - 1));
-            const int r = (d.y >> UNIT_OBJ_BITS) & 7;
-            const int ql = d.x + (d.y >> (UNIT_OBJ_BITS + 3)) - 1;  // last partner
+        const int wave = t / WAVE;
+        const int lane = static_cast<int>(__lane_id());
+        for (int l0 = wave * WAVE; l0 < total; l0 += BLOCK) {
+            const int l = l0 + lane;
+            if (l >= total) continue;
+            // Element e whose units hold lane l: the last e with s_lane0[e] <= l.
+            int lo = 0, hi = PAIR_GROUP;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_lane0[mid] <= l) lo = mid;
+                else hi = mid;
+            }
+            const int e = lo;
+            int j = l - s_lane0[e];
+            int r = 0;
+            int2 u = s_unit[e * NRANGES];
+            while (j >= (u.y >> 16)) {
+                j -= u.y >> 16;
+                ++r;
+                u = s_unit[e * NRANGES + r];
+            }
+            const int q = u.x + j;
+            // Own elements of the segment [e, e + own) that pair with q.
+            const int m_seg = u.y & 0xffff;
+            const int own = r == 0 ? min(m_seg, q - (g0 + e)) : m_seg;
             int dx, dy;
             range_offset(r, dx, dy);
-            const float4 la = P.lrec[2 * po], va = P.lrec[2 * po + 1];
-            const float ax = la.x - static_cast<float>(dx) * cellf;
-            const float ay = la.y - static_cast<float>(dy) * cellf;
-            cand += static_cast<unsigned int>(ql - d.x + 1);
-            for (int q = d.x; q <= ql; q += PAIR_BATCH) {
-                float4 b[PAIR_BATCH], w[PAIR_BATCH];
+            const float4 b = P.lrec[2 * q], w = P.lrec[2 * q + 1];
+            const int ib = __float_as_int(b.w);
+            const int wz = __float_as_int(w.w);
+            const float fdx = static_cast<float>(dx) * cellf, fdy = static_cast<float>(dy) * cellf;
+            cand += static_cast<unsigned int>(own);
+            // OWN_BATCH own records read at a time (their LDS reads in flight
+            // together), then tested.
+            for (int k0 = 0; k0 < own; k0 += OWN_BATCH) {
+                float4 la[OWN_BATCH], va[OWN_BATCH];
 #pragma unroll
-                for (int k = 0; k < PAIR_BATCH; ++k) {
-                    const int qq = q + k < ql ? q + k : ql;
-                    b[k] = P.lrec[2 * qq];
-                    w[k] = P.lrec[2 * qq + 1];
+                for (int u = 0; u < OWN_BATCH; ++u) {
+                    const int kk = e + min(k0 + u, own - 1);
+                    la[u] = s_rec[2 * kk];
+                    va[u] = s_rec[2 * kk + 1];
                 }
 #pragma unroll
-                for (int k = 0; k < PAIR_BATCH; ++k) {
-                    if (q + k > ql) continue;
-                    const int ia = __float_as_int(la.w), ib = __float_as_int(b[k].w);
+                for (int u = 0; u < OWN_BATCH; ++u) {
+                    if (k0 + u >= own) continue;
+                    const int ia = __float_as_int(la[u].w);
                     if (SHELL && (ia & ib & NOT_FIRST)) continue;  // tested in an inner common shell
-                    const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
-                    if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
-                                            w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f, lim_max2))
+                    const float ax = la[u].x - fdx;
+                    const float ay = la[u].y - fdy;
+                    const float oz = static_cast<float>(wz - __float_as_int(va[u].w)) * cellf;
+                    if (prefilter_keep_fast(b.x - ax, b.y - ay, oz + (b.z - la[u].z), w.x - va[u].x,
+                                            w.y - va[u].y, w.z - va[u].z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f,
+                                            lim_max2))
                         push_survivor(gs, P, ia & OBJ_MASK, ib & OBJ_MASK);
                 }
             }
@@ -2905,7 +2932,7 @@ struct khip_ctx {
 
     // Learned form: per-object method, per-method lists, dense-output
     // states of the sub-step boundaries, check and re-verification tallies.
-    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0, 4.0, 32.0};
+    khip_learned_config lcfg{6, 10, 1e-5, 0.025, 0.1, 12000.0, 5e-8, 0, 4.0, 16.0};
     int64_t lrn_cap = 0;
     uint8_t* mode = nullptr;
     uint8_t* snap_mode = nullptr;
@@ -2949,6 +2976,7 @@ struct khip_ctx {
     // corrector alone in the MFMA kernel (KHIP_LEARNED_FUSED=1: one kernel
     // per method, for comparison; results are bit-identical).
     bool split_learned = true;
+
 
     uint64_t gen = 1;               // bumped by every (re)allocation and configuration change
     // Instantiated chunk graphs keyed by what they were captured for (most
@@ -3144,7 +3172,7 @@ constexpr int PART_BLOCKS = 1024;
 inline int part_grid(int64_t n) { return std::max(1, std::min(grid_of(n), PART_BLOCKS)); }
 
 // Timing marks of step slot `slot` in the current chunk: 0 before the
-// propagation, 1 after it, 2 before the pair kernels, 3 after k_pairs,
+// propagation, 1 after it, 2 before the pair kernels, 3 after k_pairs_lanes,
 // 4 after k_pairs_dense, 5 after k_exact.
 // The learned form uses one slot per sub-step: marks 1-5 around that
 // sub-step's screening (1 before its reduction), and, in the slot of the
@@ -3253,9 +3281,9 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
     const int pair_blocks = std::max(1, (n + PAIR_GROUP - 1) / PAIR_GROUP);
     if (shells)
-        k_pairs<true><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+        k_pairs_lanes<true><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     else
-        k_pairs<false><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+        k_pairs_lanes<false><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
     mark(ctx, 3);
     if (shells)
         k_pairs_dense<true><<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
@@ -5168,6 +5196,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     ctx->device = device;
     if (const char* f = std::getenv("KHIP_LEARNED_FUSED")) ctx->split_learned = std::atoi(f) == 0;
 
+
     ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     char buf[512];
     std::snprintf(buf, sizeof(buf), "%s (%s, %d CUs, %.1f GB)", prop.name, prop.gcnArchName,
@@ -5519,6 +5548,7 @@ int khip_take_breakups(khip_ctx* ctx, khip_breakup_event* out, int64_t cap, int6
 int khip_set_tuning(khip_ctx* ctx, const khip_tuning* t) {
     static_assert(sizeof(khip_tuning) == sizeof(TileCfg), "tuning layout");
     if (!ctx || !t || !(t->cell_mult >= 1.0) || t->cell_mult > 4.0 || t->dense_min < 2 || t->long_range < 1 ||
+        t->long_range > 16384 ||
         t->seg < 1 || t->dense_blocks < 1 || t->dense_blocks > MAX_DENSE_BLOCKS || t->exact_blocks < 1 ||
         t->exact_blocks > MAX_EXACT_BLOCKS)
         return KHIP_ERR_ARG;
