@@ -409,6 +409,60 @@ struct SampleObj {
     int on;   // 0: dropped (the object or its plain twin died)
 };
 
+// The thresholds of one learned step's screening, from the sample's drift
+// so far (k_bands writes them once per step on the device; the screening
+// kernels read them, so captured graphs and chunk boundaries do not depend
+// on the values).
+struct Bands {
+    double band, band_v;              // drift bands (km, km/s)
+    double thr_search, thr_report;    // search: threshold + margin + band; report: threshold + band
+    double vmin2_search, vmin2_report;
+    float thr_f, base_f, vmin2_f, pad;
+};
+
+struct BandCfg {
+    double factor;     // margin_factor (0: no band)
+    double threshold;  // configured threshold (km)
+    double margin;     // configured search margin (km)
+    double vmin;       // configured minimum encounter speed (km/s)
+    double window;     // screening window (s)
+    double dt;         // learned step (s)
+    double t0;         // time the sample started
+};
+
+// The drift band for the step starting at t: the sample's drift D (position
+// or velocity) measured up to t, carried to the step's end as
+// D ((T + dt) / T)^2 with T = t - t0 (an error in the semi-major axis grows
+// the along-track drift quadratically in time), times `factor` (both objects
+// of a pair drift, and the sample's maximum stands for the population's).
+// The search's minimum speed is lowered further by the change of a pair's
+// relative velocity over a window of w seconds: (2 mu / r^3) d w with
+// 2 mu / r^3 <= 3.1e-6 s^-2 above the surface and d <= thr + vmin w for a
+// pair that closes to thr in the window (the plain solve tests the speed at
+// its sub-step's start, the search at its window's).
+__host__ __device__ inline Bands make_bands(const BandCfg& c, double dev, double dev_v, double t) {
+    Bands b{};
+    const double T = t - c.t0;
+    double g2 = 0.0;
+    if (c.factor > 0.0 && T > 0.0) {
+        const double g = (T + c.dt) / T;
+        g2 = g * g;
+    }
+    b.band = c.factor * dev * g2;
+    b.band_v = c.factor * dev_v * g2;
+    b.thr_report = c.threshold + b.band;
+    b.thr_search = b.thr_report + c.margin;
+    const double vmin = c.vmin - b.band_v > 0.0 ? c.vmin - b.band_v : 0.0;
+    b.vmin2_report = vmin * vmin;
+    double vs = vmin - 3.1e-6 * (b.thr_search + vmin * c.window) * c.window;
+    vs = vs > 0.0 ? vs : 0.0;
+    b.vmin2_search = vs * vs;
+    b.thr_f = static_cast<float>(b.thr_search);
+    b.vmin2_f = static_cast<float>(b.vmin2_search);
+    b.base_f = b.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * static_cast<float>(c.window);
+    return b;
+}
+
 // One object through `substeps` FP64 RK4 steps; moon13 holds the Moon every
 // tau / 12 over the step. Stops at the sub-step whose end is under the
 // surface (the object dies there, as a plain step of tau / substeps would).
@@ -947,7 +1001,8 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
                                                     int nw, double* dense, size_t cap, StepPartial* partials,
                                                     GridState* gs, unsigned long long ev_cap, int shells,
                                                     ShellCfg sc, int* shell_lo, uint16_t* shell_n, int* tile_cnt,
-                                                    int tstride) {
+                                                    int tstride, const Bands* bands) {
+    if (bands) sc.half = 0.5 * bands->thr_search + 1e-6;
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     if (blockIdx.x == 0 && threadIdx.x == 0) gs->ev_mark = gs->events < ev_cap ? gs->events : ev_cap;
     StepPartial acc[FUSED_WINDOWS_MAX];
@@ -1188,8 +1243,10 @@ struct ShellScreen {
 __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const StepPartial* partials,
                                                        int nparts, double dt, double thr,
                                                        int launched_passes, int midpoint, int sbits,
-                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh) {
+                                                       TileCfg tc, FinishArgs fin, int nsort, ShellScreen sh,
+                                                       const Bands* bands) {
     __shared__ StepPartial s_rows[TWAVES];
+    if (bands) thr = bands->thr_search;
     __shared__ int s_wave[TWAVES];
     long long entries = 0;
     if (sh.on) {
@@ -1903,6 +1960,7 @@ struct StepInfo {
 
 struct PairParams {
     Planes before, after;
+    const Bands* bands;     // learned form with a drift band: thresholds of the step (else null)
     const double* radius;   // collision radius (km) per object
     const int* cell_of;
     const int* cell_begin;
@@ -1928,6 +1986,19 @@ struct PairParams {
 // (neighbour cells), so |dr|_1 <= 6 cell, and |dv|_1 <= sqrt(3) |dv| <=
 // 2 sqrt(3) vmax (vmax: the largest speed of the velocities in the records);
 // lim grows with both. Safety factors cover FP32 rounding.
+// The step's thresholds from the device (learned form with a drift band).
+__device__ __forceinline__ PairParams with_bands(PairParams P) {
+    if (P.bands) {
+        const Bands b = *P.bands;
+        P.thr = b.thr_search;
+        P.vmin2 = b.vmin2_search;
+        P.thr_f = b.thr_f;
+        P.vmin2_f = b.vmin2_f;
+        P.base_f = b.base_f;
+    }
+    return P;
+}
+
 __device__ __forceinline__ float pair_lim_max2(const GridState* gs, const PairParams& P) {
     const double vmax = __longlong_as_double(static_cast<long long>(gs->vmax_bits));
     const double dvn = 2.0 * 1.7320508075688772 * vmax * 1.001;
@@ -2110,7 +2181,8 @@ constexpr int MAX_UNITS = PAIR_GROUP * NRANGES;
 constexpr int OWN_BATCH = 4;
 
 template <bool SHELL>
-__global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams P, PairPartial* parts) {
+__global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams Pin, PairPartial* parts) {
+    const PairParams P = with_bands(Pin);
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     __shared__ float4 s_rec[2 * PAIR_GROUP];
     __shared__ int s_cell[PAIR_GROUP];
@@ -2246,11 +2318,12 @@ constexpr int PARTNER_BATCH = 4;
 //     scalar loads.
 // Both evaluate the same expression on the same operands, own object first.
 template <bool SHELL>
-__global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams P,
+__global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams Pin,
                                                         const PairItem* __restrict__ items,
                                                         const float4* __restrict__ lrec,
                                                         unsigned int item_cap, PairPartial* parts) {
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
+    const PairParams P = with_bands(Pin);
     const int lane = static_cast<int>(__lane_id());
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const unsigned int total_waves = gridDim.x * WAVES_PER_BLOCK;
@@ -2356,7 +2429,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_dense(GridState* gs, PairParams
 
 // FP64 exact filter, TCA and radius test of the pre-filter survivors: one
 // lane per survivor (grid-stride), coherent FP64 work.
-__global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams P, PairPartial* parts) {
+__global__ void __launch_bounds__(BLOCK) k_exact(GridState* gs, PairParams Pin, PairPartial* parts) {
+    const PairParams P = with_bands(Pin);
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     const unsigned long long n = gs->n_surv < P.surv_cap ? gs->n_surv : P.surv_cap;
     if (blockIdx.x == 0 && threadIdx.x == 0 && gs->n_surv > P.surv_cap) gs->surv_short = 1;
@@ -2457,7 +2531,11 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
                                                      Planes s0, const double* radius,
                                                      const MoonSample* __restrict__ moon13, const StepInfo* info,
                                                      double h, int sub, double thr_report, double vmin2,
-                                                     ReverifyStats* rs) {
+                                                     ReverifyStats* rs, const Bands* bands) {
+    if (bands) {
+        thr_report = bands->thr_report;
+        vmin2 = bands->vmin2_report;
+    }
     const double t_step = info->t;
     const long long sub0 = info->macro * sub;
     const unsigned long long e0 = gs->ev_mark;
@@ -2960,6 +3038,7 @@ struct khip_ctx {
     // largest drift; the twins step on a side stream beside the screening.
     SampleObj* sample = nullptr;
     SampleObj* snap_sample = nullptr;
+    Bands* bands = nullptr;     // the current learned step's thresholds (drift band)
     unsigned long long* sample_dev = nullptr;
     int sample_n = 0;
     bool sample_reset = true;   // start the sample afresh at the next learned chunk
@@ -3058,7 +3137,7 @@ void free_all(khip_ctx* ctx) {
                     ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
                     ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates,
                     ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tile_base, ctx->sample, ctx->snap_sample,
-                    ctx->sample_dev};
+                    ctx->sample_dev, ctx->bands};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
 }
@@ -3204,7 +3283,7 @@ void mark(khip_ctx* ctx, int k) {
 bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step, int stat_slot, double dt,
                     double thr, int midpoint, const StepInfo* info = nullptr, int sub_q = 0, int sub_n = 1,
                     int sub_per = 1, const StepPartial* parts = nullptr, const FinishArgs* prev = nullptr,
-                    FinishArgs* defer = nullptr, int shell_q = -1, double vmin_km_s = -1.0) {
+                    FinishArgs* defer = nullptr, int shell_q = -1, const Bands* bands = nullptr) {
     const int n_obj = static_cast<int>(ctx->n_launch);
     const int obj_tiles = ctx->ntiles();
     const bool shells = shell_q >= 0;
@@ -3221,7 +3300,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
                          ctx->tile_base + static_cast<size_t>(shell_q) * ctx->tstride, obj_tiles,
                          static_cast<unsigned int>(n)};
     k_grid_setup<<<1, TBLOCK, 0, s>>>(ctx->gs, parts ? parts : ctx->step_parts, part_grid(n_obj), dt, thr, ctx->passes,
-                                     midpoint, ctx->sbits, tc, fin, n_obj, sh);
+                                     midpoint, ctx->sbits, tc, fin, n_obj, sh, bands);
     if (shells)
         k_keys_shell<<<obj_tiles, TBLOCK, 0, s>>>(
             pb, pa, ctx->alive, ctx->scen, n_obj, ctx->gs, ctx->shell_lo + static_cast<size_t>(shell_q) * ctx->cap,
@@ -3272,8 +3351,8 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.sub_per = sub_per;
     P.dt = dt;
     P.thr = thr;
-    const double vmin = vmin_km_s >= 0.0 ? vmin_km_s : ctx->cfg.min_encounter_speed_km_s;
-    P.vmin2 = vmin * vmin;
+    P.vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
+    P.bands = bands;
     P.dt_f = static_cast<float>(P.dt);
     P.thr_f = static_cast<float>(P.thr);
     P.vmin2_f = static_cast<float>(P.vmin2);
@@ -3917,20 +3996,13 @@ __global__ void __launch_bounds__(BLOCK) k_sample_step(SampleObj* s, int ns, Pla
     if (__lane_id() == 0 && mv) atomicMax(dev + 1, mv);
 }
 
-// The drift band for a chunk of `span` seconds, with the sample's drift D
-// (position, km, or velocity, km/s) measured `elapsed` seconds after its
-// start: an error in the semi-major axis grows the along-track drift
-// quadratically in time, so D is carried to the chunk's end as
-// D ((elapsed + span) / elapsed)^2, times `factor` (both objects of a pair
-// drift, and the sample's maximum stands for the population's), rounded up
-// to a ladder of 2^(1/4) steps (1e-3 x 2^(k/4) for integer k) so the chunk
-// graphs come in few sizes.
-double drift_band(double factor, double dev_km, double elapsed, double span) {
-    if (!(factor > 0) || !(dev_km > 0) || !(elapsed > 0)) return 0.0;
-    const double g = (elapsed + span) / elapsed;
-    const double want = factor * dev_km * g * g;
-    const double level = std::ceil(4.0 * std::log2(want / 1e-3)) / 4.0;
-    return 1e-3 * std::exp2(level);
+// One learned step's thresholds (one thread): read before the step's sample
+// update is forked, so every run computes them from the same D.
+__global__ void k_bands(const unsigned long long* dev, const StepInfo* info, BandCfg c, Bands* out) {
+    if (threadIdx.x != 0) return;
+    const double d = dev ? __longlong_as_double(static_cast<long long>(dev[0])) : 0.0;
+    const double dv = dev ? __longlong_as_double(static_cast<long long>(dev[1])) : 0.0;
+    *out = make_bands(c, d, dv, info->t);
 }
 
 bool ensure_sample(khip_ctx* ctx) {
@@ -3939,7 +4011,7 @@ bool ensure_sample(khip_ctx* ctx) {
     if (!ctx->ev_fork) KHIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
     if (!ctx->ev_join) KHIP_TRY(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     return dev_alloc(ctx, &ctx->sample, SAMPLE_MAX) && dev_alloc(ctx, &ctx->snap_sample, SAMPLE_MAX) &&
-           dev_alloc(ctx, &ctx->sample_dev, 2);
+           dev_alloc(ctx, &ctx->sample_dev, 2) && dev_alloc(ctx, &ctx->bands, 1);
 }
 
 bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
@@ -4084,24 +4156,14 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         const int cur0 = ctx->cur;
         const double t_chunk = ctx->t;
         const bool force0 = ctx->force_check;
-        // Drift bands of the chunk (the sample's drift carried to the
-        // chunk's end): events are kept up to threshold + band, searched for
-        // up to that plus the configured margin, and relative speeds down to
-        // the minimum encounter speed less the velocity band are kept.
-        const double elapsed = t_chunk - ctx->sample_t0, span = static_cast<double>(len) * ctx->dt;
-        const double band = follow ? drift_band(L.margin_factor, __builtin_bit_cast(double, dev0[0]), elapsed, span)
-                                   : 0.0;
-        const double band_v =
-            follow ? drift_band(L.margin_factor, __builtin_bit_cast(double, dev0[1]), elapsed, span) : 0.0;
-        const double thr = ctx->cfg.threshold_km + L.margin_km + band;
-        const double vmin_eff = std::max(0.0, ctx->cfg.min_encounter_speed_km_s - band_v);
-        const double vmin2 = vmin_eff * vmin_eff;
-        // The search tests a pair's speed at its window's start, the plain
-        // solve at its sub-step's: over a window of hw seconds the relative
-        // velocity of two objects d km apart changes by at most
-        // (2 mu / r^3) d hw, with 2 mu / r^3 <= 3.1e-6 s^-2 above the surface
-        // and d <= thr + vmin hw for a pair that closes to thr in the window.
-        const double vmin_search = std::max(0.0, vmin_eff - 3.1e-6 * (thr + vmin_eff * hw) * hw);
+        // With a drift band, each step's thresholds are computed on the
+        // device (k_bands) from the sample's drift; the host's values below
+        // are then the band-free ones the layout starts from.
+        const double thr = ctx->cfg.threshold_km + L.margin_km;
+        const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
+        const BandCfg bcfg{follow ? L.margin_factor : 0.0, ctx->cfg.threshold_km, L.margin_km,
+                           ctx->cfg.min_encounter_speed_km_s, hw, ctx->dt, ctx->sample_t0};
+        const Bands* bands = follow ? ctx->bands : nullptr;
         const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
         for (;;) {
             // Step table of the chunk and the host's copy of the check pattern.
@@ -4181,6 +4243,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                     }
                     mark(ctx, 7);
+                    // The step's thresholds, from the drift so far (before
+                    // this step's sample update is forked).
+                    if (follow) k_bands<<<1, 64, 0, s>>>(ctx->sample_dev, info, bcfg, ctx->bands);
                     // The sample's plain step runs on the side stream beside
                     // the screening (it reads the end-of-step states and
                     // flags, and writes only its own arrays); joined before
@@ -4212,20 +4277,20 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                 k_windows<<<part_grid(n), BLOCK, 0, s>>>(
                                     pc, pn, ctx->alive, n, ctx->dt, nw, ctx->substates, c, ctx->win_parts, ctx->gs,
                                     static_cast<unsigned long long>(ctx->ev_cap), shells ? 1 : 0, shell_cfg,
-                                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride);
+                                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride, bands);
                             FinishArgs mine{};
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw,
                                                 fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
                                                 q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr,
-                                                shells ? q : -1, vmin_search))
+                                                shells ? q : -1, bands))
                                 return false;
                             pending = mine;
                         }
                         k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                                   static_cast<unsigned long long>(ctx->ev_cap), pc,
                                                                   ctx->radius, m13, info, h, sub,
-                                                                  ctx->cfg.threshold_km + band, vmin2, ctx->rstats);
+                                                                  ctx->cfg.threshold_km, vmin2, ctx->rstats, bands);
                         if (follow) KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
                         if (bk) {
                             // The step's confirmed collisions, in (t, i, j) order:
@@ -4260,8 +4325,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             };
             if (ctx->use_graph && !ctx->timing && !bk) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
-                                            shells ? ctx->elaunch : 0, __builtin_bit_cast(int64_t, thr),
-                                            __builtin_bit_cast(int64_t, vmin2), follow ? ctx->sample_n : -1};
+                                            shells ? ctx->elaunch : 0, follow ? ctx->sample_n : -1,
+                                            __builtin_bit_cast(int64_t, ctx->sample_t0)};
                 key.insert(key.end(), pattern.begin(), pattern.end());
                 if (!launch_graph(ctx, key, enqueue_chunk)) return false;
             } else if (!enqueue_chunk()) {
@@ -4328,9 +4393,11 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 r.events_dropped += static_cast<int64_t>(rs.dropped);
                 r.events_unverified += static_cast<int64_t>(rs.kept_unverified);
                 r.steps += len;
-                r.band_km = band;
-                r.band_v_km_s = band_v;
                 if (follow) {
+                    Bands lb;
+                    KHIP_TRY(ctx, hipMemcpy(&lb, ctx->bands, sizeof(lb), hipMemcpyDeviceToHost));
+                    r.band_km = lb.band;
+                    r.band_v_km_s = lb.band_v;
                     unsigned long long dv[2] = {0, 0};
                     KHIP_TRY(ctx, hipMemcpy(dv, ctx->sample_dev, sizeof(dv), hipMemcpyDeviceToHost));
                     r.sample_dev_max_km = __builtin_bit_cast(double, dv[0]);
