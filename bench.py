@@ -291,8 +291,9 @@ def recall_against(e_ref, e_new, thr=None):
 # HBM bytes per launch of the pair kernels in the learned workload (two 30 s
 # screening windows with 16 km radial shells), measured by two rocprofv3 PMC
 # passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from copy kernels of
-# known size in the same runs); None until measured at this configuration.
-PAIR_TRAFFIC_B = {}
+# known size in the same runs): k_pairs_lanes<true> 79,314 + 619 KiB,
+# k_pairs_dense<true> 26 + 32 KiB.
+PAIR_TRAFFIC_B = {"k_pairs": 163068928, "k_pairs_dense": 86016}
 # Screening windows per 60 s learned step (30 s each; propagation and the
 # plain re-verification keep 10 s sub-steps) and the radial shell width.
 WINDOWS = 2
@@ -369,7 +370,7 @@ def learned_workload(kamd, args):
     rec = recall_against(plain["events"], lrn["events"], thr)
     tm = lrn["timing"]
     launches_sub = max(1, int(tm["steps"]) * WINDOWS)
-    kernels = {"k_learned (propagation)": float(tm["propagate_ms"]), "grid (11-13 kernels)": float(tm["grid_ms"]),
+    kernels = {"k_learned (propagation)": float(tm["propagate_ms"]), "grid (15-19 kernels)": float(tm["grid_ms"]),
                "k_pairs": float(tm["pairs_sparse_ms"]), "k_pairs_dense": float(tm["pairs_dense_ms"]),
                "k_exact": float(tm["exact_ms"]), "k_reverify": float(tm["reverify_ms"])}
     cand = int(lrn["stats"]["candidate_pairs"].sum())
