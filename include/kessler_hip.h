@@ -344,7 +344,9 @@ typedef struct {
     int32_t seg;          /* partners per work item (256) */
     int32_t dense_blocks; /* persistent grid of the dense-cell pair kernel (1024, <= 4096) */
     int32_t exact_blocks; /* grid of the exact FP64 kernel (512, <= 2048) */
-    int32_t pad;
+    int32_t sparse_kernel; /* sparse-cell pair kernel: 0 (default) one partner per lane against the
+                              cell's own objects in LDS (catalogues); 1 one own object per lane
+                              walking its partners (the crowded cells of fragment clouds) */
 } khip_tuning;
 int khip_set_tuning(khip_ctx* ctx, const khip_tuning* t);
 int khip_get_tuning(const khip_ctx* ctx, khip_tuning* t);

@@ -109,7 +109,7 @@ class BreakupConfig(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("cell_mult", _D), ("dense_min", C.c_int32), ("long_range", C.c_int32), ("seg", C.c_int32),
-                ("dense_blocks", C.c_int32), ("exact_blocks", C.c_int32), ("pad", C.c_int32)]
+                ("dense_blocks", C.c_int32), ("exact_blocks", C.c_int32), ("sparse_kernel", C.c_int32)]
 
 
 class ScreenConfig(C.Structure):
@@ -418,13 +418,13 @@ class Engine:
     def tuning(self) -> dict:
         t = Tuning()
         self._check(self.lib.khip_get_tuning(self.h, C.byref(t)), "khip_get_tuning")
-        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad"}
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
         t = Tuning(cur["cell_mult"], cur["dense_min"], cur["long_range"], cur["seg"], cur["dense_blocks"],
-                   cur["exact_blocks"], 0)
+                   cur["exact_blocks"], cur["sparse_kernel"])
         self._check(self.lib.khip_set_tuning(self.h, C.byref(t)), "khip_set_tuning")
 
     def autotune(self, steps=4, allow_cell=False):
@@ -433,7 +433,7 @@ class Engine:
         b0, b1 = _D(), _D()
         self._check(self.lib.khip_autotune(self.h, int(steps), 1 if allow_cell else 0, C.byref(t), C.byref(b0),
                                            C.byref(b1)), "khip_autotune")
-        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad"}, b0.value, b1.value
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}, b0.value, b1.value
 
     def learned_report(self):
         out = np.zeros(1, dtype=LEARNED_REPORT_DTYPE)

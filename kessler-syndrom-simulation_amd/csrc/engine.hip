@@ -125,7 +125,8 @@ constexpr int MAX_EXACT_BLOCKS = 2048;
 // changes the candidate count only.
 struct TileCfg {
     double cell_mult;
-    int dense_min, long_range, seg, dense_blocks, exact_blocks, pad;
+    int dense_min, long_range, seg, dense_blocks, exact_blocks;
+    int sparse;  // sparse-cell pair kernel: 0 partners on the lanes, 1 one own object per lane
 };
 
 // Device-resident grid parameters and per-step accumulators.
@@ -2161,7 +2162,126 @@ struct SurvivorSlots {
     }
 };
 
-constexpr int PAIR_GROUP = BLOCK;  // sorted elements per block of the sparse-cell pair kernel
+// Sparse cells, one own object per lane (the tuning's sparse_kernel 1). A
+// unit is one (own object, neighbour range) pair: the run
+// of partners the object is tested against (range 0 from the position after
+// its own). A block takes PAIR_GROUP consecutive sorted objects, ranks their
+// units by length in LDS (counting sort, longest first) and hands them to
+// its waves in rounds of 64, so the lanes of a round walk runs of nearly
+// equal length. A lane loads PAIR_BATCH partner records at a time into the
+// registers that test them (no copies of loads in flight, which would make
+// every iteration wait for its own loads). Survivors are few here: they are
+// appended one wave-aggregated atomic at a time (a per-wave reservation
+// would need wave-uniform loops and leave unused slots for k_exact).
+constexpr int PAIR_GROUP = BLOCK;   // sorted objects per block, one per thread
+constexpr int UNIT_BUCKETS = 64;    // unit length classes, longest first (unit_bucket)
+constexpr int UNIT_OBJ_BITS = 8;    // log2(PAIR_GROUP)
+constexpr int PAIR_BATCH = 4;
+static_assert((1 << UNIT_OBJ_BITS) == PAIR_GROUP && UNIT_BUCKETS == WAVE, "unit packing");
+
+// Length class of a unit, longest first: lengths 1-32 one class each
+// (buckets 63..32), then classes 8 long (33-40 -> 31, ..., 273-280 -> 1),
+// longer -> 0.
+__device__ __forceinline__ int unit_bucket(int l) {
+    if (l <= 32) return UNIT_BUCKETS - l;
+    const int c = 31 - ((l - 33) >> 3);
+    return c > 0 ? c : 0;
+}
+
+// SHELL: the sorted elements are shell entries; a pair is tested in the
+// shell where one of its two entries is its object's first.
+template <bool SHELL>
+__global__ void __launch_bounds__(BLOCK) k_pairs_units(GridState* gs, PairParams Pin, PairPartial* parts) {
+    const PairParams P = with_bands(Pin);
+    __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
+    __shared__ int s_hist[UNIT_BUCKETS];
+    __shared__ int s_total;
+    __shared__ int2 s_unit[PAIR_GROUP * NRANGES];
+    const int A = static_cast<int>(gs->nlive);
+    const int g0 = blockIdx.x * PAIR_GROUP;
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
+    unsigned int cand = 0;
+    const float lim_max2 = pair_lim_max2(gs, P);
+    if (gs->npasses > 0 && g0 < A) {  // uniform over the block
+        if (threadIdx.x < UNIT_BUCKETS) s_hist[threadIdx.x] = 0;
+        __syncthreads();
+        const int long_range = gs->long_range, dmin = gs->dense_min;
+        const int p = g0 + threadIdx.x;
+        const int c = p < A ? P.cell_of[p] : 0;
+        const bool own = p < A && P.cell_count[c] < dmin;
+        int q0[NRANGES], len[NRANGES], rank[NRANGES];
+#pragma unroll
+        for (int r = 0; r < NRANGES; ++r) {
+            int2 run = make_int2(0, 0);
+            if (own) run = P.ranges[c * NRANGES + r];
+            const int first = r == 0 ? p + 1 : run.x;
+            const int l = run.y - run.x > long_range ? 0 : run.y - first;  // long ranges: work items
+            q0[r] = first;
+            len[r] = l > 0 ? l : 0;
+            rank[r] = 0;
+            if (l > 0) rank[r] = atomicAdd(&s_hist[unit_bucket(l)], 1);
+        }
+        __syncthreads();
+        // Exclusive scan of the bucket counts (one wave, one bucket per lane).
+        if (wave == 0) {
+            const int v = s_hist[lane];
+            int incl = v;
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const int t = __shfl_up(incl, d, WAVE);
+                if (lane >= d) incl += t;
+            }
+            s_hist[lane] = incl - v;
+            if (lane == WAVE - 1) s_total = incl;
+        }
+        __syncthreads();
+        const int total = s_total;
+#pragma unroll
+        for (int r = 0; r < NRANGES; ++r) {
+            const int l = len[r];
+            if (l == 0) continue;
+            const int bucket = unit_bucket(l);
+            s_unit[s_hist[bucket] + rank[r]] =
+                make_int2(q0[r], static_cast<int>(threadIdx.x) | (r << UNIT_OBJ_BITS) | (l << (UNIT_OBJ_BITS + 3)));
+        }
+        __syncthreads();
+        const float cellf = static_cast<float>(gs->cell);
+        for (int u = wave * WAVE + lane; u - lane < total; u += BLOCK) {
+            if (u >= total) continue;
+            const int2 d = s_unit[u];
+            const int po = g0 + (d.y & (PAIR_GROUP - 1));
+            const int r = (d.y >> UNIT_OBJ_BITS) & 7;
+            const int ql = d.x + (d.y >> (UNIT_OBJ_BITS + 3)) - 1;  // last partner
+            int dx, dy;
+            range_offset(r, dx, dy);
+            const float4 la = P.lrec[2 * po], va = P.lrec[2 * po + 1];
+            const float ax = la.x - static_cast<float>(dx) * cellf;
+            const float ay = la.y - static_cast<float>(dy) * cellf;
+            cand += static_cast<unsigned int>(ql - d.x + 1);
+            for (int q = d.x; q <= ql; q += PAIR_BATCH) {
+                float4 b[PAIR_BATCH], w[PAIR_BATCH];
+#pragma unroll
+                for (int k = 0; k < PAIR_BATCH; ++k) {
+                    const int qq = q + k < ql ? q + k : ql;
+                    b[k] = P.lrec[2 * qq];
+                    w[k] = P.lrec[2 * qq + 1];
+                }
+#pragma unroll
+                for (int k = 0; k < PAIR_BATCH; ++k) {
+                    if (q + k > ql) continue;
+                    const int ia = __float_as_int(la.w), ib = __float_as_int(b[k].w);
+                    if (SHELL && (ia & ib & NOT_FIRST)) continue;  // tested in an inner common shell
+                    const float oz = static_cast<float>(__float_as_int(w[k].w) - __float_as_int(va.w)) * cellf;
+                    if (prefilter_keep_fast(b[k].x - ax, b[k].y - ay, oz + (b[k].z - la.z), w[k].x - va.x,
+                                            w[k].y - va.y, w[k].z - va.z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f, lim_max2))
+                        push_survivor(gs, P, ia & OBJ_MASK, ib & OBJ_MASK);
+                }
+            }
+        }
+        __syncthreads();  // s_cnt is reused below
+    }
+    store_counters(parts, PairCounters{cand, 0, 0}, s_cnt);
+}
 
 // Sparse cells, partners on the lanes. A block takes PAIR_GROUP consecutive
 // sorted elements; their compact records go to LDS. The elements of one cell
@@ -3359,10 +3479,16 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     P.gdt3_f = static_cast<float>(GRAVITY_GRADIENT) * P.dt_f * P.dt_f * P.dt_f;
     P.base_f = P.thr_f + 1e-3f + DR_ERR_KM + DV_ERR_KM_S * P.dt_f;
     const int pair_blocks = std::max(1, (n + PAIR_GROUP - 1) / PAIR_GROUP);
-    if (shells)
+    if (tc.sparse == 1) {
+        if (shells)
+            k_pairs_units<true><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+        else
+            k_pairs_units<false><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    } else if (shells) {
         k_pairs_lanes<true><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
-    else
+    } else {
         k_pairs_lanes<false><<<pair_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->pair_parts);
+    }
     mark(ctx, 3);
     if (shells)
         k_pairs_dense<true><<<tc.dense_blocks, BLOCK, 0, s>>>(ctx->gs, P, ctx->items, ctx->lrec, icap,
@@ -4543,6 +4669,19 @@ bool autotune(khip_ctx* ctx, int64_t steps, bool allow_cell, double& base_ms, do
     };
     // The dense-cell threshold and the long-range cut act together (a large
     // threshold pays only with a long cut): they are searched as pairs.
+    // The sparse-cell kernel first: partners on the lanes suits catalogues,
+    // one own object per lane the crowded cells of fresh fragment clouds.
+    for (int v : {0, 1}) {
+        if (!ok || bestc.sparse == v) continue;
+        TileCfg cand = bestc;
+        cand.sparse = v;
+        double t = 0.0;
+        ok = trial(cand, t);
+        if (ok && t < 0.98 * bt) {
+            bestc = cand;
+            bt = t;
+        }
+    }
     const std::pair<int, int> split[] = {{32, 64}, {64, 128}, {128, 192}, {128, 256}, {256, 256}, {256, 512}};
     for (const auto& dl : split) {
         if (!ok || (bestc.dense_min == dl.first && bestc.long_range == dl.second)) continue;
@@ -5617,7 +5756,7 @@ int khip_set_tuning(khip_ctx* ctx, const khip_tuning* t) {
     if (!ctx || !t || !(t->cell_mult >= 1.0) || t->cell_mult > 4.0 || t->dense_min < 2 || t->long_range < 1 ||
         t->long_range > 16384 ||
         t->seg < 1 || t->dense_blocks < 1 || t->dense_blocks > MAX_DENSE_BLOCKS || t->exact_blocks < 1 ||
-        t->exact_blocks > MAX_EXACT_BLOCKS)
+        t->exact_blocks > MAX_EXACT_BLOCKS || t->sparse_kernel < 0 || t->sparse_kernel > 1)
         return KHIP_ERR_ARG;
     std::memcpy(&ctx->tile, t, sizeof(TileCfg));
     ctx->gen += 1;

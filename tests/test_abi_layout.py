@@ -31,7 +31,8 @@ C_FIELDS = {
     "khip_breakup_event": ["t", "v_rel_km_s", "energy_j_per_g", "target_mass_kg", "projectile_mass_kg",
                            "sbm_mass_kg", "expected_fragments", "fragments_mass_kg", "position", "id", "target",
                            "projectile", "catastrophic", "drawn", "fragments", "escaped", "first_index"],
-    "khip_tuning": ["cell_mult", "dense_min", "long_range", "seg", "dense_blocks", "exact_blocks", "pad"],
+    "khip_tuning": ["cell_mult", "dense_min", "long_range", "seg", "dense_blocks", "exact_blocks",
+                    "sparse_kernel"],
     "khip_net_norm": ["fmu", "fsig", "omu", "osig", "tau"],
     "khip_train_config": ["samples_per_orbit", "spacing_steps", "iterations", "batch", "lr_max", "e_max",
                           "a_max_km", "tau", "seed"],

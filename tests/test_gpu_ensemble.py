@@ -129,9 +129,12 @@ def test_autotune_keeps_results(engine, kamd, scenarios):
     # Deliberately different tilings give the same results too: tiny dense
     # cells and short ranges; and large sparse cells (more objects than a
     # wave has lanes) whose long ranges go to work items chunk by chunk.
-    for tiling in (dict(dense_min=8, long_range=16, seg=64, dense_blocks=300, exact_blocks=100),
-                   dict(dense_min=160, long_range=96, seg=128, dense_blocks=512, exact_blocks=256)):
+    # Both sparse-cell kernels.
+    for tiling in (dict(dense_min=8, long_range=16, seg=64, dense_blocks=300, exact_blocks=100, sparse_kernel=1),
+                   dict(dense_min=160, long_range=96, seg=128, dense_blocks=512, exact_blocks=256, sparse_kernel=0),
+                   dict(dense_min=160, long_range=96, seg=128, dense_blocks=512, exact_blocks=256, sparse_kernel=1)):
         engine.set_tuning(**tiling)
         s2, a2, ev2, st2 = _run(engine, allpop, 20, 10.0, kamd.PRECISION_FP64, offsets=off)
         assert np.array_equal(s2, ref[0]) and _key(ev2) == _key(ref[2]) and _same_stats(st2, ref[3]), tiling
-    engine.set_tuning(dense_min=128, long_range=256, seg=256, dense_blocks=1024, exact_blocks=512, cell_mult=1.0)
+    engine.set_tuning(dense_min=128, long_range=256, seg=256, dense_blocks=1024, exact_blocks=512, cell_mult=1.0,
+                      sparse_kernel=0)
