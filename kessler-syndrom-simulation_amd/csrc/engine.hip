@@ -2299,6 +2299,8 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_units(GridState* gs, PairParams
 // candidate), this measured 30% faster at 1M objects, 2 km.
 constexpr int MAX_UNITS = PAIR_GROUP * NRANGES;
 constexpr int OWN_BATCH = 4;
+constexpr int LANE_MAP = 2048;  // lanes mapped to their unit per round of the block
+static_assert(PAIR_GROUP <= 256 && NRANGES <= 8, "lane map code: element | range << 8");
 
 template <bool SHELL>
 __global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams Pin, PairPartial* parts) {
@@ -2306,7 +2308,9 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams
     __shared__ unsigned long long s_cnt[WAVES_PER_BLOCK][3];
     __shared__ float4 s_rec[2 * PAIR_GROUP];
     __shared__ int s_cell[PAIR_GROUP];
-    __shared__ int s_lane0[PAIR_GROUP + 1];  // first lane of each element's units
+    __shared__ int s_own[PAIR_GROUP];
+    __shared__ int s_first[MAX_UNITS], s_start[MAX_UNITS];
+    __shared__ uint16_t s_map[LANE_MAP];
     __shared__ int s_wave[WAVES_PER_BLOCK];
     const int A = static_cast<int>(gs->nlive);
     const int g0 = blockIdx.x * PAIR_GROUP;
@@ -2348,75 +2352,75 @@ __global__ void __launch_bounds__(BLOCK) k_pairs_lanes(GridState* gs, PairParams
         }
         int total;
         const int lane0 = block_exclusive_scan(lane_sum, total, s_wave);
-        s_lane0[t] = lane0;
-        if (t == 0) s_lane0[PAIR_GROUP] = total;
-        // Unit descriptors stay in the head thread's registers; lanes find
-        // their element by binary search over s_lane0 and read the rest
-        // from LDS copies written here.
-        // Per (element, range): first partner, own count | lanes << 16 (own
-        // counts are at most PAIR_GROUP, lanes at most long_range).
-        __shared__ int2 s_unit[MAX_UNITS];
+        // Per (element, range) of a head: first partner and first lane; per
+        // element its own count. Lanes find their unit through a lane map
+        // (element | range << 8 per lane), LANE_MAP lanes at a time.
         if (head) {
+            int st = lane0;
 #pragma unroll
-            for (int r = 0; r < NRANGES; ++r)
-                s_unit[t * NRANGES + r] = make_int2(r == 0 ? p + 1 : run[r].x, m | (lanes[r] << 16));
+            for (int r = 0; r < NRANGES; ++r) {
+                s_first[t * NRANGES + r] = r == 0 ? p + 1 : run[r].x;
+                s_start[t * NRANGES + r] = st;
+                st += lanes[r];
+            }
         }
-        __syncthreads();
+        s_own[t] = m;
         const float cellf = static_cast<float>(gs->cell);
         const int wave = t / WAVE;
         const int lane = static_cast<int>(__lane_id());
-        for (int l0 = wave * WAVE; l0 < total; l0 += BLOCK) {
-            const int l = l0 + lane;
-            if (l >= total) continue;
-            // Element e whose units hold lane l: the last e with s_lane0[e] <= l.
-            int lo = 0, hi = PAIR_GROUP;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_lane0[mid] <= l) lo = mid;
-                else hi = mid;
-            }
-            const int e = lo;
-            int j = l - s_lane0[e];
-            int r = 0;
-            int2 u = s_unit[e * NRANGES];
-            while (j >= (u.y >> 16)) {
-                j -= u.y >> 16;
-                ++r;
-                u = s_unit[e * NRANGES + r];
-            }
-            const int q = u.x + j;
-            // Own elements of the segment [e, e + own) that pair with q.
-            const int m_seg = u.y & 0xffff;
-            const int own = r == 0 ? min(m_seg, q - (g0 + e)) : m_seg;
-            int dx, dy;
-            range_offset(r, dx, dy);
-            const float4 b = P.lrec[2 * q], w = P.lrec[2 * q + 1];
-            const int ib = __float_as_int(b.w);
-            const int wz = __float_as_int(w.w);
-            const float fdx = static_cast<float>(dx) * cellf, fdy = static_cast<float>(dy) * cellf;
-            cand += static_cast<unsigned int>(own);
-            // OWN_BATCH own records read at a time (their LDS reads in flight
-            // together), then tested.
-            for (int k0 = 0; k0 < own; k0 += OWN_BATCH) {
-                float4 la[OWN_BATCH], va[OWN_BATCH];
+        for (int w0 = 0; w0 < total; w0 += LANE_MAP) {
+            const int w1 = min(total, w0 + LANE_MAP);
+            __syncthreads();  // the previous window's map is read out
+            if (head && lane_sum > 0) {
+                int st = lane0;
 #pragma unroll
-                for (int u = 0; u < OWN_BATCH; ++u) {
-                    const int kk = e + min(k0 + u, own - 1);
-                    la[u] = s_rec[2 * kk];
-                    va[u] = s_rec[2 * kk + 1];
+                for (int r = 0; r < NRANGES; ++r) {
+                    const int a = max(st, w0), b = min(st + lanes[r], w1);
+                    const uint16_t code = static_cast<uint16_t>(t | (r << 8));
+                    for (int l = a; l < b; ++l) s_map[l - w0] = code;
+                    st += lanes[r];
                 }
+            }
+            __syncthreads();
+            for (int l0 = w0 + wave * WAVE; l0 < w1; l0 += BLOCK) {
+                const int l = l0 + lane;
+                if (l >= w1) continue;
+                const int code = s_map[l - w0];
+                const int e = code & 0xff, r = code >> 8;
+                const int q = s_first[e * NRANGES + r] + (l - s_start[e * NRANGES + r]);
+                // Own elements of the segment [e, e + own) that pair with q.
+                const int m_seg = s_own[e];
+                const int own = r == 0 ? min(m_seg, q - (g0 + e)) : m_seg;
+                int dx, dy;
+                range_offset(r, dx, dy);
+                const float4 b = P.lrec[2 * q], w = P.lrec[2 * q + 1];
+                const int ib = __float_as_int(b.w);
+                const int wz = __float_as_int(w.w);
+                // The partner's position in the own cell's frame (one cell
+                // edge per offset; one rounding, inside DR_ERR_KM).
+                const float bx = b.x + static_cast<float>(dx) * cellf, by = b.y + static_cast<float>(dy) * cellf;
+                cand += static_cast<unsigned int>(own);
+                // OWN_BATCH own records read at a time (their LDS reads in
+                // flight together), then tested.
+                for (int k0 = 0; k0 < own; k0 += OWN_BATCH) {
+                    float4 la[OWN_BATCH], va[OWN_BATCH];
 #pragma unroll
-                for (int u = 0; u < OWN_BATCH; ++u) {
-                    if (k0 + u >= own) continue;
-                    const int ia = __float_as_int(la[u].w);
-                    if (SHELL && (ia & ib & NOT_FIRST)) continue;  // tested in an inner common shell
-                    const float ax = la[u].x - fdx;
-                    const float ay = la[u].y - fdy;
-                    const float oz = static_cast<float>(wz - __float_as_int(va[u].w)) * cellf;
-                    if (prefilter_keep_fast(b.x - ax, b.y - ay, oz + (b.z - la[u].z), w.x - va[u].x,
-                                            w.y - va[u].y, w.z - va[u].z, P.dt_f, P.gdt3_f, P.base_f, P.vmin2_f,
-                                            lim_max2))
-                        push_survivor(gs, P, ia & OBJ_MASK, ib & OBJ_MASK);
+                    for (int u = 0; u < OWN_BATCH; ++u) {
+                        const int kk = e + min(k0 + u, own - 1);
+                        la[u] = s_rec[2 * kk];
+                        va[u] = s_rec[2 * kk + 1];
+                    }
+#pragma unroll
+                    for (int u = 0; u < OWN_BATCH; ++u) {
+                        if (k0 + u >= own) continue;
+                        const int ia = __float_as_int(la[u].w);
+                        if (SHELL && (ia & ib & NOT_FIRST)) continue;  // tested in an inner common shell
+                        const float oz = static_cast<float>(wz - __float_as_int(va[u].w)) * cellf;
+                        if (prefilter_keep_fast(bx - la[u].x, by - la[u].y, oz + (b.z - la[u].z), w.x - va[u].x,
+                                                w.y - va[u].y, w.z - va[u].z, P.dt_f, P.gdt3_f, P.base_f,
+                                                P.vmin2_f, lim_max2))
+                            push_survivor(gs, P, ia & OBJ_MASK, ib & OBJ_MASK);
+                    }
                 }
             }
         }
