@@ -948,40 +948,52 @@ struct ShellCfg {
     double r0, w, r1, w2;
     int n1, nmax;
     double half;  // half the search threshold plus rounding slack (km)
+    float r0f, r1f, inv_w, inv_w2;  // FP32 copies for shell_of
 };
 
-__device__ __forceinline__ int shell_of(double rho, const ShellCfg& c) {
-    if (rho < c.r1) return rho <= c.r0 ? 0 : static_cast<int>((rho - c.r0) / c.w);
-    const double k = (rho - c.r1) / c.w2;
-    return k >= static_cast<double>(c.nmax - c.n1) ? c.nmax : c.n1 + static_cast<int>(k);
+// Shell index of radius rho, in FP32 (reciprocal widths): a radius within
+// SHELL_SLACK of a boundary may land on either side, which the callers'
+// widening by SHELL_SLACK covers.
+__device__ __forceinline__ int shell_of(float rho, const ShellCfg& c) {
+    if (rho < c.r1f) return rho <= c.r0f ? 0 : static_cast<int>((rho - c.r0f) * c.inv_w);
+    const float k = (rho - c.r1f) * c.inv_w2;
+    return k >= static_cast<float>(c.nmax - c.n1) ? c.nmax : c.n1 + static_cast<int>(k);
 }
+
+// FP32 error allowance of the radius bounds and shell indices below: 2e-6
+// relative (the few FP32 operations err by well under 1e-6) plus 0.1 m.
+__device__ __forceinline__ float shell_slack(float r) { return 2e-6f * r + 1e-4f; }
 
 // Radius range of a window's cubic (the Hermite of its end states p0, v0
 // and p1, v1 over h seconds). The curve lies in the convex hull of its
 // Bezier control points P0 = p0, P1 = p0 + h v0 / 3, P2 = p1 - h v1 / 3,
 // P3 = p1, so |p(t)| <= max |Pk|, and |p(t)| >= p(t).u >= min Pk.u for the
-// unit vector u along P0 + P3.
+// unit vector u along P0 + P3. The control points are formed in FP64, the
+// bounds in FP32 and widened by shell_slack.
 __device__ __forceinline__ void radius_range(const double p0[3], const double v0[3], const double p1[3],
-                                             const double v1[3], double h, double& lo, double& hi) {
-    double P[4][3];
+                                             const double v1[3], double h, float& lo, float& hi) {
+    float P[4][3];
     for (int k = 0; k < 3; ++k) {
-        P[0][k] = p0[k];
-        P[1][k] = p0[k] + h * v0[k] * (1.0 / 3.0);
-        P[2][k] = p1[k] - h * v1[k] * (1.0 / 3.0);
-        P[3][k] = p1[k];
+        P[0][k] = static_cast<float>(p0[k]);
+        P[1][k] = static_cast<float>(p0[k] + h * v0[k] * (1.0 / 3.0));
+        P[2][k] = static_cast<float>(p1[k] - h * v1[k] * (1.0 / 3.0));
+        P[3][k] = static_cast<float>(p1[k]);
     }
-    const double ux = P[0][0] + P[3][0], uy = P[0][1] + P[3][1], uz = P[0][2] + P[3][2];
-    const double un = sqrt(ux * ux + uy * uy + uz * uz);
-    const double inv = un > 0.0 ? 1.0 / un : 0.0;
-    double h2 = 0.0, dmin = 1e300;
+    const float ux = P[0][0] + P[3][0], uy = P[0][1] + P[3][1], uz = P[0][2] + P[3][2];
+    const float un2 = ux * ux + uy * uy + uz * uz;
+    const float inv = un2 > 0.0f ? __builtin_amdgcn_rsqf(un2) : 0.0f;
+    float h2 = 0.0f, dmin = 3.0e38f;
     for (int j = 0; j < 4; ++j) {
-        const double r2 = P[j][0] * P[j][0] + P[j][1] * P[j][1] + P[j][2] * P[j][2];
+        const float r2 = P[j][0] * P[j][0] + P[j][1] * P[j][1] + P[j][2] * P[j][2];
         h2 = r2 > h2 ? r2 : h2;
-        const double d = (P[j][0] * ux + P[j][1] * uy + P[j][2] * uz) * inv;
+        const float d = (P[j][0] * ux + P[j][1] * uy + P[j][2] * uz) * inv;
         dmin = d < dmin ? d : dmin;
     }
-    hi = sqrt(h2);
-    lo = un > 0.0 ? dmin : 0.0;
+    hi = __builtin_sqrtf(h2);
+    lo = un2 > 0.0f ? dmin : 0.0f;
+    const float sl = shell_slack(hi);
+    hi += sl;
+    lo -= sl;
 }
 
 // Every screening window of a learned step in one pass over its start (a)
@@ -998,17 +1010,21 @@ __device__ __forceinline__ void radius_range(const double p0[3], const double v0
 // tile_cnt + q * tstride (k_grid_setup reads and clears them).
 constexpr int FUSED_WINDOWS_MAX = 4;
 
+// NW: the windows (1 .. FUSED_WINDOWS_MAX), a template argument so that only
+// NW windows' partial reductions are held in registers.
+template <int NW>
 __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uint8_t* alive, int n, double dt,
-                                                    int nw, double* dense, size_t cap, StepPartial* partials,
+                                                    double* dense, size_t cap, StepPartial* partials,
                                                     GridState* gs, unsigned long long ev_cap, int shells,
                                                     ShellCfg sc, int* shell_lo, uint16_t* shell_n, int* tile_cnt,
                                                     int tstride, const Bands* bands) {
     if (bands) sc.half = 0.5 * bands->thr_search + 1e-6;
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     if (blockIdx.x == 0 && threadIdx.x == 0) gs->ev_mark = gs->events < ev_cap ? gs->events : ev_cap;
-    StepPartial acc[FUSED_WINDOWS_MAX];
+    constexpr int nw = NW;
+    StepPartial acc[NW];
 #pragma unroll
-    for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) acc[q] = empty_step_partial();
+    for (int q = 0; q < NW; ++q) acc[q] = empty_step_partial();
     const double hw = dt / nw;
     // Block-uniform trip count: the per-wave tile sums below need every lane.
     for (int i0 = blockIdx.x * BLOCK; i0 < n; i0 += gridDim.x * BLOCK) {
@@ -1019,9 +1035,10 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
         const double x1[3] = {b.x[ii], b.y[ii], b.z[ii]}, v1[3] = {b.vx[ii], b.vy[ii], b.vz[ii]};
         const bool on = in && alive[ii] != 0;
         double ps[3] = {x0[0], x0[1], x0[2]}, ws[3] = {v0[0], v0[1], v0[2]};
+        // |v| at the window's start: the previous window's end value.
+        double va = on ? sqrt(ws[0] * ws[0] + ws[1] * ws[1] + ws[2] * ws[2]) : 0.0;
 #pragma unroll
-        for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) {
-            if (q >= nw) break;
+        for (int q = 0; q < NW; ++q) {
             double pe[3], we[3];
             if (q == nw - 1) {
                 for (int k = 0; k < 3; ++k) pe[k] = x1[k], we[k] = v1[k];
@@ -1038,17 +1055,18 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
                 rx = 0.5 * (ps[0] + pe[0]);
                 ry = 0.5 * (ps[1] + pe[1]);
                 rz = 0.5 * (ps[2] + pe[2]);
-                const double va = sqrt(ws[0] * ws[0] + ws[1] * ws[1] + ws[2] * ws[2]);
                 const double vb = sqrt(we[0] * we[0] + we[1] * we[1] + we[2] * we[2]);
                 vn = va > vb ? va : vb;
+                va = vb;
             }
             StepPartial op = object_partial(on, on, vn, rx, ry, rz);
             if (shells) {
                 int nb = 0;
                 if (on) {
-                    double lo, hi;
+                    float lo, hi;
                     radius_range(ps, ws, pe, we, hw, lo, hi);
-                    const int s0 = shell_of(lo - sc.half, sc), s1 = shell_of(hi + sc.half, sc);
+                    const float half = static_cast<float>(sc.half);
+                    const int s0 = shell_of(lo - half, sc), s1 = shell_of(hi + half, sc);
                     nb = s1 - s0 + 1;
                     shell_lo[static_cast<size_t>(q) * cap + i] = s0;
                     shell_n[static_cast<size_t>(q) * cap + i] = static_cast<uint16_t>(nb);
@@ -1066,8 +1084,7 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
         }
     }
 #pragma unroll
-    for (int q = 0; q < FUSED_WINDOWS_MAX; ++q) {
-        if (q >= nw) break;
+    for (int q = 0; q < NW; ++q) {
         const StepPartial t = block_reduce(acc[q], s_rows);
         if (threadIdx.x == 0) partials[q * gridDim.x + blockIdx.x] = t;
         __syncthreads();  // s_rows is reused by the next window
@@ -4035,7 +4052,11 @@ ShellCfg shell_layout(double shell_km, double thr) {
     c.r1 = c.r0 + c.n1 * c.w;
     c.w2 = std::max(256.0, shell_km);
     c.nmax = c.n1 + static_cast<int>(std::ceil(std::max(0.0, 60000.0 - c.r1) / c.w2));
-    c.half = 0.5 * thr + 1e-6;  // + rounding of the radius bounds (~1e-12 relative)
+    c.half = 0.5 * thr + 1e-6;
+    c.r0f = static_cast<float>(c.r0);
+    c.r1f = static_cast<float>(c.r1);
+    c.inv_w = static_cast<float>(1.0 / c.w);
+    c.inv_w2 = static_cast<float>(1.0 / c.w2);
     return c;
 }
 
@@ -4403,11 +4424,14 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             if (!fused)
                                 k_substep<<<part_grid(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, ctx->dt, q, nw, pa,
                                                                        ctx->step_parts);
-                            else if (q == 0)
-                                k_windows<<<part_grid(n), BLOCK, 0, s>>>(
-                                    pc, pn, ctx->alive, n, ctx->dt, nw, ctx->substates, c, ctx->win_parts, ctx->gs,
+                            else if (q == 0) {
+                                auto kw = nw == 1 ? k_windows<1> : nw == 2 ? k_windows<2> : nw == 3 ? k_windows<3>
+                                                                                                   : k_windows<4>;
+                                kw<<<part_grid(n), BLOCK, 0, s>>>(
+                                    pc, pn, ctx->alive, n, ctx->dt, ctx->substates, c, ctx->win_parts, ctx->gs,
                                     static_cast<unsigned long long>(ctx->ev_cap), shells ? 1 : 0, shell_cfg,
                                     ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride, bands);
+                            }
                             FinishArgs mine{};
                             if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
                                                 info, q, nw, spw,

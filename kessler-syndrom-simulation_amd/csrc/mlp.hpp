@@ -163,21 +163,22 @@ __device__ __forceinline__ u32x4 frag_load(const uint16_t* frags, int f, int lan
     return *reinterpret_cast<const u32x4*>(frags + f * FRAG_HALVES + lane * 8);
 }
 
-// SiLU and its derivative with the hardware exp2 and reciprocal.
-__device__ __forceinline__ float sigmoid_fast(float z) {
-    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+// Hard swish, z clamp(z / 6 + 1 / 2, 0, 1), and its derivative: an FMA, a
+// three-way median and a product per activation (SiLU needs an exp2 and a
+// reciprocal, two quarter-rate instructions).
+__device__ __forceinline__ float hswish(float z) {
+    return z * __builtin_amdgcn_fmed3f(fmaf(z, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
 }
-__device__ __forceinline__ f32x4 silu4(f32x4 z) {
+__device__ __forceinline__ f32x4 act4(f32x4 z) {
     f32x4 h;
-    h.x = z.x * sigmoid_fast(z.x);
-    h.y = z.y * sigmoid_fast(z.y);
-    h.z = z.z * sigmoid_fast(z.z);
-    h.w = z.w * sigmoid_fast(z.w);
+    h.x = hswish(z.x);
+    h.y = hswish(z.y);
+    h.z = hswish(z.z);
+    h.w = hswish(z.w);
     return h;
 }
-__device__ __forceinline__ float dsilu(float z) {
-    const float s = sigmoid_fast(z);
-    return s * (1.0f + z * (1.0f - s));
+__device__ __forceinline__ float dact(float z) {
+    return z < -3.0f ? 0.0f : (z > 3.0f ? 1.0f : fmaf(z, 1.0f / 3.0f, 0.5f));
 }
 
 __device__ __forceinline__ f32x4 bias4(const float* b, int mt, int g) {
@@ -328,7 +329,7 @@ __device__ __forceinline__ void forward_tiles(const u32x4 xin[NT], const uint16_
         for (int t = 0; t < NT; ++t) {
             const f32x4 z0 = mfma(a0, xin[t], b0);
             const f32x4 z1 = mfma(a1, xin[t], b1);
-            hb[p][t] = pack_tiles(silu4(z0), silu4(z1));
+            hb[p][t] = pack_tiles(act4(z0), act4(z1));
         }
     }
     // Two hidden layers; the output layer (hi/lo split of weights and
@@ -371,13 +372,13 @@ __device__ __forceinline__ void forward_tiles(const u32x4 xin[NT], const uint16_
             }
             if (layer == 0) {
 #pragma unroll
-                for (int t = 0; t < NT; ++t) nb[p][t] = pack_tiles(silu4(z0[t]), silu4(z1[t]));
+                for (int t = 0; t < NT; ++t) nb[p][t] = pack_tiles(act4(z0[t]), act4(z1[t]));
             } else {
                 const u32x4 ah = frag_load(frags, FR_L4H + p, lane);
                 const u32x4 al = frag_load(frags, FR_L4L + p, lane);
 #pragma unroll
                 for (int t = 0; t < NT; ++t) {
-                    const f32x4 h0 = silu4(z0[t]), h1 = silu4(z1[t]);
+                    const f32x4 h0 = act4(z0[t]), h1 = act4(z1[t]);
                     const u32x4 hi = pack_tiles(h0, h1), lo = pack_tiles_lo(h0, h1, hi);
                     out[t] = mfma(ah, hi, out[t]);
                     out[t] = mfma(ah, lo, out[t]);
@@ -500,7 +501,7 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) z1[mt] = mfma(frag_load(ffwd, FR_L1 + mt, lane), xin, bias4(P + OFF_B1, mt, g));
 #pragma unroll
-    for (int p = 0; p < 4; ++p) hb1[p] = pack_tiles(silu4(z1[2 * p]), silu4(z1[2 * p + 1]));
+    for (int p = 0; p < 4; ++p) hb1[p] = pack_tiles(act4(z1[2 * p]), act4(z1[2 * p + 1]));
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         f32x4 z = bias4(P + OFF_B2, mt, g);
@@ -509,7 +510,7 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
         z2[mt] = z;
     }
 #pragma unroll
-    for (int p = 0; p < 4; ++p) hb2[p] = pack_tiles(silu4(z2[2 * p]), silu4(z2[2 * p + 1]));
+    for (int p = 0; p < 4; ++p) hb2[p] = pack_tiles(act4(z2[2 * p]), act4(z2[2 * p + 1]));
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         f32x4 z = bias4(P + OFF_B3, mt, g);
@@ -519,7 +520,7 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        const f32x4 h0 = silu4(z3[2 * p]), h1 = silu4(z3[2 * p + 1]);
+        const f32x4 h0 = act4(z3[2 * p]), h1 = act4(z3[2 * p + 1]);
         hb3[p] = pack_tiles(h0, h1);
         hb3lo[p] = pack_tiles_lo(h0, h1, hb3[p]);
     }
@@ -565,16 +566,16 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
         zero.x = zero.y = zero.z = zero.w = 0.0f;
         const f32x4 dh = mfma(frag_load(fbwd, FR_T4 + mt, lane), gb4, zero);
         f32x4 r;
-        r.x = dh.x * dsilu(z3[mt].x);
-        r.y = dh.y * dsilu(z3[mt].y);
-        r.z = dh.z * dsilu(z3[mt].z);
-        r.w = dh.w * dsilu(z3[mt].w);
+        r.x = dh.x * dact(z3[mt].x);
+        r.y = dh.y * dact(z3[mt].y);
+        r.z = dh.z * dact(z3[mt].z);
+        r.w = dh.w * dact(z3[mt].w);
         gz[mt] = r;
     }
     // Layer-4 input activations and layer-3 gradients.
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
-        store_tile(T.h3, B, mt, g, col, silu4(z3[mt]));
+        store_tile(T.h3, B, mt, g, col, act4(z3[mt]));
         store_tile(T.g3, B, mt, g, col, gz[mt]);
     }
     // dh2 = W3^T g3.
@@ -587,11 +588,11 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
         dh.x = dh.y = dh.z = dh.w = 0.0f;
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) dh = mfma(frag_load(fbwd, FR_T3 + mt * 4 + ks, lane), gb[ks], dh);
-        gz[mt].x = dh.x * dsilu(z2[mt].x);
-        gz[mt].y = dh.y * dsilu(z2[mt].y);
-        gz[mt].z = dh.z * dsilu(z2[mt].z);
-        gz[mt].w = dh.w * dsilu(z2[mt].w);
-        store_tile(T.h2, B, mt, g, col, silu4(z2[mt]));
+        gz[mt].x = dh.x * dact(z2[mt].x);
+        gz[mt].y = dh.y * dact(z2[mt].y);
+        gz[mt].z = dh.z * dact(z2[mt].z);
+        gz[mt].w = dh.w * dact(z2[mt].w);
+        store_tile(T.h2, B, mt, g, col, act4(z2[mt]));
         store_tile(T.g2, B, mt, g, col, gz[mt]);
     }
     // dh1 = W2^T g2.
@@ -604,11 +605,11 @@ __global__ void __launch_bounds__(256) k_net_fwdbwd(const float* __restrict__ X,
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) dh = mfma(frag_load(fbwd, FR_T2 + mt * 4 + ks, lane), gb[ks], dh);
         f32x4 r;
-        r.x = dh.x * dsilu(z1[mt].x);
-        r.y = dh.y * dsilu(z1[mt].y);
-        r.z = dh.z * dsilu(z1[mt].z);
-        r.w = dh.w * dsilu(z1[mt].w);
-        store_tile(T.h1, B, mt, g, col, silu4(z1[mt]));
+        r.x = dh.x * dact(z1[mt].x);
+        r.y = dh.y * dact(z1[mt].y);
+        r.z = dh.z * dact(z1[mt].z);
+        r.w = dh.w * dact(z1[mt].w);
+        store_tile(T.h1, B, mt, g, col, act4(z1[mt]));
         store_tile(T.g1, B, mt, g, col, r);
     }
 }

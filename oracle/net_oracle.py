@@ -16,8 +16,13 @@ restates is the library's own definition of the perceptron, in two forms:
   at every point where the device rounds (operands of the matrix-core
   products; the input and output layers split operands into BF16 high and
   low parts and add hi*hi + hi*lo + lo*hi), FP64 accumulation. The device
-  accumulates in FP32 in another order and uses hardware exp2 / reciprocal
-  in SiLU, so comparisons carry a tolerance stated in the tests.
+  accumulates in FP32 in another order and evaluates the activation in FP32
+  (one fused multiply-add, a clamp and a product), so comparisons carry a
+  tolerance stated in the tests.
+
+The activation is the hard swish z * clamp(z / 6 + 1 / 2, 0, 1) (Howard et
+al., "Searching for MobileNetV3", 2019), whose derivative is 0 below -3, 1
+above 3 and z / 3 + 1 / 2 between.
 
 Parameter vector layout (35,206 floats): W1[128][10], b1[128], W2[128][128],
 b2[128], W3[128][128], b3[128], W4[6][128], b4[6].
@@ -62,13 +67,12 @@ def init_params(seed=0):
     return pack(*parts).astype(np.float32)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def act(z):
+    return z * np.clip(z / 6.0 + 0.5, 0.0, 1.0)
 
 
-def dsilu(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def dact(z):
+    return np.where(z < -3.0, 0.0, np.where(z > 3.0, 1.0, z / 3.0 + 0.5))
 
 
 # ---------------------------------------------------------------------------
@@ -79,11 +83,11 @@ def forward_exact(p, x):
     W1, b1, W2, b2, W3, b3, W4, b4 = unpack(p)
     x = np.asarray(x, dtype=np.float64)
     z1 = x @ W1.T + b1
-    h1 = silu(z1)
+    h1 = act(z1)
     z2 = h1 @ W2.T + b2
-    h2 = silu(z2)
+    h2 = act(z2)
     z3 = h2 @ W3.T + b3
-    h3 = silu(z3)
+    h3 = act(z3)
     return h3 @ W4.T + b4, (x, z1, h1, z2, h2, z3, h3)
 
 
@@ -95,9 +99,9 @@ def loss_and_grad_exact(p, x, y):
     d = o - np.asarray(y, dtype=np.float64)
     loss = float((d * d).sum() / (6.0 * B))
     g4 = 2.0 * d / (6.0 * B)
-    g3 = (g4 @ W4) * dsilu(z3)
-    g2 = (g3 @ W3) * dsilu(z2)
-    g1 = (g2 @ W2) * dsilu(z1)
+    g3 = (g4 @ W4) * dact(z3)
+    g2 = (g3 @ W3) * dact(z2)
+    g1 = (g2 @ W2) * dact(z1)
     grad = pack(g1.T @ x, g1.sum(0), g2.T @ h1, g2.sum(0), g3.T @ h2, g3.sum(0), g4.T @ h3, g4.sum(0))
     return loss, grad
 
@@ -128,11 +132,11 @@ def forward_device(p, x):
     xh, xl = split(x)
     W1h, W1l = split(W1)
     z1 = xh @ W1h.T + xl @ W1h.T + xh @ W1l.T + b1
-    h1 = silu(z1.astype(np.float32).astype(np.float64))
+    h1 = act(z1.astype(np.float32).astype(np.float64))
     z2 = bf16(h1) @ bf16(W2).T + b2
-    h2 = silu(z2.astype(np.float32).astype(np.float64))
+    h2 = act(z2.astype(np.float32).astype(np.float64))
     z3 = bf16(h2) @ bf16(W3).T + b3
-    h3 = silu(z3.astype(np.float32).astype(np.float64))
+    h3 = act(z3.astype(np.float32).astype(np.float64))
     h3h, h3l = split(h3)
     W4h, W4l = split(W4)
     o = h3h @ W4h.T + h3l @ W4h.T + h3h @ W4l.T + b4
@@ -148,9 +152,9 @@ def grad_device(p, x, y):
     d = o - np.asarray(y, dtype=np.float64)
     loss = float((d * d).sum() / (6.0 * B))
     g4 = bf16(2.0 * d / (6.0 * B))
-    g3 = bf16((g4 @ bf16(W4)) * dsilu(z3))
-    g2 = bf16((g3 @ bf16(W3)) * dsilu(z2))
-    g1 = bf16((g2 @ bf16(W2)) * dsilu(z1))
+    g3 = bf16((g4 @ bf16(W4)) * dact(z3))
+    g2 = bf16((g3 @ bf16(W3)) * dact(z2))
+    g1 = bf16((g2 @ bf16(W2)) * dact(z1))
     grad = pack(g1.T @ xh, g1.sum(0), g2.T @ bf16(h1), g2.sum(0), g3.T @ bf16(h2), g3.sum(0),
                 g4.T @ bf16(h3), g4.sum(0))
     return loss, grad
