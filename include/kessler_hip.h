@@ -283,8 +283,9 @@ int khip_set_learned(khip_ctx* ctx, const khip_learned_config* cfg);
  * captured once into a HIP graph and replayed for every later chunk of the
  * same shape (length, check pattern, radix passes, buffers); per-step times
  * and indices come from a device table, so replays need no re-capture. All
- * forms; not with in-run breakups, whose chunks launch directly. Results
- * are identical to direct launches. No reference counterpart. */
+ * forms, in-run breakups included (a chunk after the population grew is
+ * captured again). Results are identical to direct launches. No reference
+ * counterpart. */
 int khip_set_graph(khip_ctx* ctx, int enabled);
 
 /* ---- In-run breakups (cascade mode) ---------------------------------------

@@ -2756,9 +2756,13 @@ __global__ void k_mark_events(GridState* gs, unsigned long long cap) {
 
 __global__ void __launch_bounds__(BLOCK) k_bu_events(const GridState* gs, const DevEvent* __restrict__ events,
                                                       unsigned long long cap, Planes pb, Planes pa, uint8_t* alive,
-                                                      double* mass, const double* size_m, const int* scen, double t0,
-                                                      double dt, bu::BreakupCfg cfg, bu::BreakupState* st,
-                                                      bu::Job* jobs, bu::LogRecord* logs, int log_cap) {
+                                                      double* mass, const double* size_m, const int* scen, double t_arg,
+                                                      const StepInfo* info, double dt, bu::BreakupCfg cfg,
+                                                      bu::BreakupState* st, bu::Job* jobs, bu::LogRecord* logs,
+                                                      int log_cap) {
+    // The step's start time: from the step table when one is given (captured
+    // chunks), else the argument.
+    const double t0 = info ? info->t : t_arg;
     __shared__ int s_n;
     __shared__ unsigned long long s_idx[bu::MAX_JOBS];
     if (threadIdx.x == 0) s_n = 0;
@@ -2994,8 +2998,10 @@ __global__ void __launch_bounds__(BLOCK) k_bu_write(const bu::BreakupState* st, 
                                                      const bu::Sample* __restrict__ samples, int n0, int reserve,
                                                      Planes pa, uint8_t* alive, double* radius, double* size_m,
                                                      double* mass, int* scen, const MoonSample* __restrict__ moon,
-                                                     int mstride, double t0, double dt, bu::BreakupCfg cfg,
-                                                     uint8_t* mode, int* plain_list, int* counts, Planes pb) {
+                                                     int mstride, double t_arg, const StepInfo* info, double dt,
+                                                     bu::BreakupCfg cfg, uint8_t* mode, int* plain_list, int* counts,
+                                                     Planes pb) {
+    const double t0 = info ? info->t : t_arg;  // as in k_bu_events
     const int ns = st->nsamples, nj = st->njobs;
     const MoonSample moon3[3] = {moon[0], moon[mstride], moon[2 * mstride]};
     for (int f = blockIdx.x * BLOCK + threadIdx.x; f < ns; f += gridDim.x * BLOCK) {
@@ -3815,10 +3821,9 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             bc.seed = ctx->bcfg.seed;
             bc.reserve = reserve;
             bc.sample_cap = ctx->sample_cap;
-            // Without breakups (whose kernels take the step's time as an
-            // argument) a chunk may run as a captured graph: the per-step
-            // time and index then come from the step table.
-            const bool graphed = ctx->use_graph && !ctx->timing && !bk && screen;
+            // A chunk may run as a captured graph: the per-step time and
+            // index then come from the step table (breakups included).
+            const bool graphed = ctx->use_graph && !ctx->timing && screen;
             if (graphed) {
                 if (!ensure_steptab(ctx, len)) return false;
                 std::vector<StepInfo> tab(static_cast<size_t>(len));
@@ -3860,9 +3865,10 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                     if (bk) {
                         const Planes pb = ctx->planes(cur), pa = ctx->planes(nxt);
                         int* sc = ctx->sbits ? ctx->scen : nullptr;
+                        const StepInfo* si = graphed ? ctx->steptab + k : nullptr;
                         k_bu_events<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                         static_cast<unsigned long long>(ctx->ev_cap), pb, pa,
-                                                        ctx->alive, ctx->mass, ctx->size_m, sc, t, ctx->dt, bc,
+                                                        ctx->alive, ctx->mass, ctx->size_m, sc, t, si, ctx->dt, bc,
                                                         ctx->bstate, ctx->jobs, ctx->blogs, ctx->blog_cap);
                         k_bu_sample<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples, bc);
                         k_bu_budget<<<bu::MAX_JOBS / 64, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
@@ -3871,7 +3877,7 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                         k_bu_write<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
                                                                   static_cast<int>(n0), reserve, pa, ctx->alive,
                                                                   ctx->radius, ctx->size_m, ctx->mass, sc,
-                                                                  ctx->moon + 2 * k, 1, t, ctx->dt, bc, nullptr,
+                                                                  ctx->moon + 2 * k, 1, t, si, ctx->dt, bc, nullptr,
                                                                   nullptr, nullptr, pb);
                     }
                 } else {
@@ -3886,7 +3892,8 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             };
             if (graphed) {
                 const std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
-                                                  ctx->precision, -1};
+                                                  ctx->precision, -1, bk ? n0 : -1, bk ? reserve : -1,
+                                                  bk ? ctx->sample_cap : -1};
                 if (!launch_graph(ctx, key, enqueue_chunk)) return false;
             } else if (!enqueue_chunk()) {
                 return false;
@@ -4456,16 +4463,17 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             int* sc = ctx->sbits ? ctx->scen : nullptr;
                             k_bu_events<<<1, BLOCK, 0, s>>>(ctx->gs, ctx->events,
                                                             static_cast<unsigned long long>(ctx->ev_cap), pc, pn,
-                                                            ctx->alive, ctx->mass, ctx->size_m, sc, t_step, ctx->dt,
-                                                            bc, ctx->bstate, ctx->jobs, ctx->blogs, ctx->blog_cap);
+                                                            ctx->alive, ctx->mass, ctx->size_m, sc, t_step, info,
+                                                            ctx->dt, bc, ctx->bstate, ctx->jobs, ctx->blogs,
+                                                            ctx->blog_cap);
                             k_bu_sample<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples, bc);
                             k_bu_budget<<<bu::MAX_JOBS / 64, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->samples,
                                                                           ctx->blogs);
                             k_bu_bases<<<1, 64, 0, s>>>(ctx->bstate, ctx->jobs, ctx->blogs, reserve);
                             k_bu_write<<<EXACT_BLOCKS, BLOCK, 0, s>>>(
                                 ctx->bstate, ctx->jobs, ctx->samples, static_cast<int>(n0), reserve, pn, ctx->alive,
-                                ctx->radius, ctx->size_m, ctx->mass, sc, m13, 6, t_step, ctx->dt, bc, ctx->mode,
-                                ctx->lists + c, ctx->counts, pc);
+                                ctx->radius, ctx->size_m, ctx->mass, sc, m13, 6, t_step, info, ctx->dt, bc,
+                                ctx->mode, ctx->lists + c, ctx->counts, pc);
                         }
                         mark(ctx, 0);
                     } else {
@@ -4477,10 +4485,11 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 }
                 return true;
             };
-            if (ctx->use_graph && !ctx->timing && !bk) {
+            if (ctx->use_graph && !ctx->timing) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
                                             shells ? ctx->elaunch : 0, follow ? ctx->sample_n : -1,
-                                            __builtin_bit_cast(int64_t, ctx->sample_t0)};
+                                            __builtin_bit_cast(int64_t, ctx->sample_t0), bk ? n0 : -1,
+                                            bk ? reserve : -1, bk ? ctx->sample_cap : -1};
                 key.insert(key.end(), pattern.begin(), pattern.end());
                 if (!launch_graph(ctx, key, enqueue_chunk)) return false;
             } else if (!enqueue_chunk()) {

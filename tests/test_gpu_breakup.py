@@ -57,10 +57,11 @@ def _scene(npairs=200, seed=3, masses_kg=(1000.0, 500.0)):
 
 
 def _run(engine, steps=3, chunk=256, reserve=65536, seed=9, npairs=200, masses_kg=(1000.0, 500.0), precision=0,
-         dt=10.0):
+         dt=10.0, graph=False):
     s, size, mass = _scene(npairs, masses_kg=masses_kg)
     engine.set_precision(precision)
     engine.set_chunk(chunk)
+    engine.set_graph(graph)
     engine.set_objects(s, np.ones(len(s), np.uint8), size)
     engine.set_masses(mass)
     engine.set_config(5.0, 1000.0, 0.01)
@@ -71,6 +72,7 @@ def _run(engine, steps=3, chunk=256, reserve=65536, seed=9, npairs=200, masses_k
     finally:
         engine.set_breakup(False)
         engine.set_chunk(256)
+        engine.set_graph(False)
     states, alive = engine.get_objects()
     return (states, alive, engine.take_events(), engine.take_breakups(), engine.sizes(), engine.masses(),
             engine.take_stats())
@@ -203,3 +205,22 @@ def test_learned_form_breakups_deterministic_and_chunk_independent(engine, train
                                                      np.sort(b[2], order=["step", "t", "i", "j"]))
     assert np.array_equal(a[3], b[3])
     assert np.array_equal(a[6]["candidate_pairs"], b[6]["candidate_pairs"])
+
+
+@pytest.mark.parametrize("precision,dt", [(0, 10.0), (2, 60.0)])
+def test_breakups_in_captured_graphs_equal_eager(engine, trained, precision, dt):
+    """Chunks with in-run breakups replayed from captured graphs (the step's
+    time from the step table) give the eager run's states, events and
+    breakup records bit for bit, in the plain and the learned form."""
+    _, _, params, norm = trained
+    engine.net_set(params, norm)
+    a = _run(engine, steps=8, chunk=2, reserve=65536, precision=precision, dt=dt)
+    c0, l0 = engine.graph_counts()
+    b = _run(engine, steps=8, chunk=2, reserve=65536, precision=precision, dt=dt, graph=True)
+    c1, l1 = engine.graph_counts()
+    print(f"precision {precision}: {len(a[3])} breakups, {len(a[2])} events; graph captures {c1 - c0}, "
+          f"launches {l1 - l0}")
+    assert l1 - l0 >= 4 and len(a[3]) > 0
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert a[2].tobytes() == b[2].tobytes() and np.array_equal(a[3], b[3])
+    assert np.array_equal(a[4], b[4]) and np.array_equal(a[5], b[5])
