@@ -2745,6 +2745,26 @@ __global__ void k_zero_reverify(ReverifyStats* rs) {
     }
 }
 
+// Appends the events a second screening stream found (its own buffer and
+// counter) to the context's events, and empties the second buffer. The
+// count is carried in full, so an overflow of either buffer shows as an
+// overflow of the context's and the chunk runs again with both grown.
+__global__ void __launch_bounds__(TBLOCK) k_merge_events(GridState* gs, DevEvent* events, unsigned long long cap,
+                                                         GridState* gs2, const DevEvent* __restrict__ ev2,
+                                                         unsigned long long cap2) {
+    const unsigned long long base = gs->events;
+    const unsigned long long n2 = gs2->events;
+    const unsigned long long m = n2 < cap2 ? n2 : cap2;
+    for (unsigned long long i = threadIdx.x; i < m; i += TBLOCK)
+        if (base + i < cap) events[base + i] = ev2[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        gs->events = base + n2;
+        gs2->events = 0ULL;
+        gs2->ev_mark = 0ULL;
+    }
+}
+
 __global__ void k_mark_events(GridState* gs, unsigned long long cap) {
     if (threadIdx.x == 0) gs->ev_mark = gs->events < cap ? gs->events : cap;
 }
@@ -3067,6 +3087,38 @@ __global__ void k_fp64_check(int n, const double* a, const double* b, double* ou
 // Context
 // ---------------------------------------------------------------------------
 
+// Per-screening device buffers of a second screening stream (swapped into
+// the context while its kernels are enqueued; sizes follow the context's).
+struct ScreenUnit {
+    GridState* gs = nullptr;
+    uint64_t* keys[2] = {nullptr, nullptr};
+    int* idx[2] = {nullptr, nullptr};
+    int* hist = nullptr;
+    int* totals = nullptr;
+    int* bcount = nullptr;
+    uint64_t* cell_key = nullptr;
+    int* cell_begin = nullptr;
+    int* cell_count = nullptr;
+    int* cell_of = nullptr;
+    int2* ranges = nullptr;
+    float4* lrec = nullptr;
+    float4* rec = nullptr;
+    int4* coltab = nullptr;
+    PairPartial* pair_parts = nullptr;
+    PairPartial* dense_parts = nullptr;
+    PairPartial* exact_parts = nullptr;
+    uint2* surv = nullptr;
+    int64_t surv_cap = 0;
+    DenseCell* dense = nullptr;
+    PairItem* items = nullptr;
+    int64_t item_cap = 0;
+    DevEvent* events = nullptr;
+    int64_t ev_cap = 0;
+    int64_t ecap = 0, cap = 0;  // element and object capacities the buffers were sized for
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_go = nullptr, ev_done = nullptr;
+};
+
 struct khip_ctx {
     int device = 0;
     std::string name, error;
@@ -3203,6 +3255,15 @@ struct khip_ctx {
     // per method, for comparison; results are bit-identical).
     bool split_learned = true;
 
+
+    // A second set of screening buffers and its stream: the learned form
+    // screens its windows on two streams at once (window q on unit q % 2),
+    // so one window's latency-bound grid kernels run beside the other's
+    // pair kernels. KHIP_WINDOW_STREAMS=0: one stream.
+    ScreenUnit unit2;
+    bool window_streams = true;
+    bool unit2_used = false;  // the chunk being drained screened on unit2
+    bool in_unit2 = false;    // unit2's buffers are swapped in (no timing marks)
 
     uint64_t gen = 1;               // bumped by every (re)allocation and configuration change
     // Instantiated chunk graphs keyed by what they were captured for (most
@@ -3355,6 +3416,102 @@ bool ensure_stats(khip_ctx* ctx, int64_t want) {
     return true;
 }
 
+// Exchanges the context's per-screening buffers (and its stream) with the
+// second unit's: screenings enqueued in between run on unit2.
+void swap_unit2(khip_ctx* c) {
+    using std::swap;
+    ScreenUnit& u = c->unit2;
+    swap(c->gs, u.gs);
+    swap(c->keys[0], u.keys[0]);
+    swap(c->keys[1], u.keys[1]);
+    swap(c->idx[0], u.idx[0]);
+    swap(c->idx[1], u.idx[1]);
+    swap(c->hist, u.hist);
+    swap(c->totals, u.totals);
+    swap(c->bcount, u.bcount);
+    swap(c->cell_key, u.cell_key);
+    swap(c->cell_begin, u.cell_begin);
+    swap(c->cell_count, u.cell_count);
+    swap(c->cell_of, u.cell_of);
+    swap(c->ranges, u.ranges);
+    swap(c->lrec, u.lrec);
+    swap(c->rec, u.rec);
+    swap(c->coltab, u.coltab);
+    swap(c->pair_parts, u.pair_parts);
+    swap(c->dense_parts, u.dense_parts);
+    swap(c->exact_parts, u.exact_parts);
+    swap(c->surv, u.surv);
+    swap(c->surv_cap, u.surv_cap);
+    swap(c->dense, u.dense);
+    swap(c->items, u.items);
+    swap(c->item_cap, u.item_cap);
+    swap(c->events, u.events);
+    swap(c->ev_cap, u.ev_cap);
+    swap(c->stream, u.stream);
+    c->in_unit2 = !c->in_unit2;
+}
+
+// Sizes unit2's buffers like the context's (after any growth of those).
+bool sync_unit2(khip_ctx* ctx) {
+    ScreenUnit& u = ctx->unit2;
+    if (!u.gs) {
+        KHIP_TRY(ctx, hipStreamCreateWithFlags(&u.stream, hipStreamNonBlocking));
+        KHIP_TRY(ctx, hipEventCreateWithFlags(&u.ev_go, hipEventDisableTiming));
+        KHIP_TRY(ctx, hipEventCreateWithFlags(&u.ev_done, hipEventDisableTiming));
+        if (!dev_alloc(ctx, &u.gs, 1) || !dev_alloc(ctx, &u.coltab, size_t{1} << COL_BITS) ||
+            !dev_alloc(ctx, &u.totals, RADIX) || !dev_alloc(ctx, &u.dense_parts, MAX_DENSE_BLOCKS) ||
+            !dev_alloc(ctx, &u.exact_parts, MAX_EXACT_BLOCKS))
+            return false;
+        KHIP_TRY(ctx, hipMemset(u.gs, 0, sizeof(GridState)));
+        KHIP_TRY(ctx, hipMemset(u.coltab, 0, sizeof(int4) << COL_BITS));
+        KHIP_TRY(ctx, hipDeviceSynchronize());
+    }
+    if (u.ecap != ctx->ecap) {
+        const int64_t cap = ctx->ecap;
+        const int64_t tiles = (cap + TILE - 1) / TILE;
+        if (!dev_alloc(ctx, &u.keys[0], cap) || !dev_alloc(ctx, &u.keys[1], cap) || !dev_alloc(ctx, &u.idx[0], cap) ||
+            !dev_alloc(ctx, &u.idx[1], cap) || !dev_alloc(ctx, &u.hist, RADIX * tiles) ||
+            !dev_alloc(ctx, &u.bcount, tiles) || !dev_alloc(ctx, &u.cell_key, cap) ||
+            !dev_alloc(ctx, &u.cell_begin, cap) || !dev_alloc(ctx, &u.cell_count, cap) ||
+            !dev_alloc(ctx, &u.cell_of, cap) || !dev_alloc(ctx, &u.ranges, NRANGES * static_cast<size_t>(cap)) ||
+            !dev_alloc(ctx, &u.lrec, 2 * static_cast<size_t>(cap)) ||
+            !dev_alloc(ctx, &u.pair_parts, (cap + BLOCK - 1) / BLOCK) || !dev_alloc(ctx, &u.dense, cap))
+            return false;
+        u.ecap = cap;
+    }
+    if (u.cap != ctx->cap) {
+        if (!dev_alloc(ctx, &u.rec, 2 * static_cast<size_t>(ctx->cap))) return false;
+        u.cap = ctx->cap;
+    }
+    if (u.surv_cap != ctx->surv_cap) {
+        if (!dev_alloc(ctx, &u.surv, ctx->surv_cap)) return false;
+        u.surv_cap = ctx->surv_cap;
+    }
+    if (u.item_cap != ctx->item_cap) {
+        if (!dev_alloc(ctx, &u.items, ctx->item_cap)) return false;
+        u.item_cap = ctx->item_cap;
+    }
+    if (u.ev_cap != ctx->ev_cap) {
+        if (!dev_alloc(ctx, &u.events, ctx->ev_cap)) return false;
+        u.ev_cap = ctx->ev_cap;
+    }
+    return true;
+}
+
+void free_unit2(khip_ctx* ctx) {
+    ScreenUnit& u = ctx->unit2;
+    if (u.stream) (void)hipStreamSynchronize(u.stream);
+    void* ptrs[] = {u.gs, u.keys[0], u.keys[1], u.idx[0], u.idx[1], u.hist, u.totals, u.bcount, u.cell_key,
+                    u.cell_begin, u.cell_count, u.cell_of, u.ranges, u.lrec, u.rec, u.coltab, u.pair_parts,
+                    u.dense_parts, u.exact_parts, u.surv, u.dense, u.items, u.events};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (u.ev_go) (void)hipEventDestroy(u.ev_go);
+    if (u.ev_done) (void)hipEventDestroy(u.ev_done);
+    if (u.stream) (void)hipStreamDestroy(u.stream);
+    u = ScreenUnit{};
+}
+
 // Moon samples for steps [0, nsteps) starting at time t: entry 2k is the
 // Moon at the start of step k, 2k+1 at its midpoint; entry 2k+2 (the end of
 // step k) is the start of step k+1, bit for bit, since both are evaluated at
@@ -3417,7 +3574,7 @@ bool ensure_event_pool(khip_ctx* ctx, int64_t steps) {
 }
 
 void mark(khip_ctx* ctx, int k) {
-    if (ctx->timing) (void)hipEventRecord(ctx->ev_pool[MARKS * g_slot + k], ctx->stream);
+    if (ctx->timing && !ctx->in_unit2) (void)hipEventRecord(ctx->ev_pool[MARKS * g_slot + k], ctx->stream);
 }
 
 // Enqueues the screening kernels for the step whose start / end states are
@@ -3605,6 +3762,20 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     GridState g;
     KHIP_TRY(ctx, hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->unit2_used) {
+        // The second screening stream's needs count as the context's (the
+        // two are sized alike; its events are already merged).
+        GridState g2;
+        KHIP_TRY(ctx, hipMemcpy(&g2, ctx->unit2.gs, sizeof(g2), hipMemcpyDeviceToHost));
+        g.passes_short |= g2.passes_short;
+        g.passes_max = std::max(g.passes_max, g2.passes_max);
+        g.items_short |= g2.items_short;
+        g.items_max = std::max(g.items_max, g2.items_max);
+        g.surv_short |= g2.surv_short;
+        g.surv_max = std::max(g.surv_max, g2.surv_max);
+        g.entries_short |= g2.entries_short;
+        g.entries_max = std::max(g.entries_max, g2.entries_max);
+    }
     rerun = false;
     ctx->last_rerun = 0;
     if (screen && g.passes_short) {
@@ -4323,7 +4494,13 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                            ctx->cfg.min_encounter_speed_km_s, hw, ctx->dt, ctx->sample_t0};
         const Bands* bands = follow ? ctx->bands : nullptr;
         const ShellCfg shell_cfg = shell_layout(L.shell_km > 0 ? L.shell_km : 16.0, thr);
+        // Windows on two streams (window q on unit q % 2) when their states
+        // all come from the fused window pass.
+        // (Not in a run with kernel marks: those time one stream.)
+        const bool two = screen && ctx->window_streams && !ctx->timing && nw >= 2 && nw <= FUSED_WINDOWS_MAX;
         for (;;) {
+            if (two && !sync_unit2(ctx)) return false;
+            ctx->unit2_used = two;
             // Step table of the chunk and the host's copy of the check pattern.
             if (!ensure_steptab(ctx, len)) return false;
             std::vector<StepInfo> tab(static_cast<size_t>(len));
@@ -4352,6 +4529,10 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 k_reset<<<1, 64, 0, s>>>(ctx->gs);
                 k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
                 k_zero_reverify<<<1, 64, 0, s>>>(ctx->rstats);
+                if (two) {
+                    k_reset<<<1, 64, 0, s>>>(ctx->unit2.gs);
+                    k_zero_events<<<1, 64, 0, s>>>(ctx->unit2.gs);
+                }
                 int cur = cur0;
                 for (int64_t k = 0; k < len; ++k) {
                     const int nxt = cur ^ 1;
@@ -4438,20 +4619,46 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                     pc, pn, ctx->alive, n, ctx->dt, ctx->substates, c, ctx->win_parts, ctx->gs,
                                     static_cast<unsigned long long>(ctx->ev_cap), shells ? 1 : 0, shell_cfg,
                                     ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride, bands);
+                                // The second stream starts once every window's states are written.
+                                if (two) {
+                                    KHIP_TRY(ctx, hipEventRecord(ctx->unit2.ev_go, s));
+                                    KHIP_TRY(ctx, hipStreamWaitEvent(ctx->unit2.stream, ctx->unit2.ev_go, 0));
+                                }
                             }
+                            // With two streams every window writes its own record
+                            // (no record is passed between windows of different streams).
+                            const bool on2 = two && (q & 1);
                             FinishArgs mine{};
-                            if (!enqueue_screen(ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1,
-                                                info, q, nw, spw,
-                                                fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
-                                                q > 0 ? &pending : nullptr, q < nw - 1 ? &mine : nullptr,
-                                                shells ? q : -1, bands))
-                                return false;
+                            if (on2) swap_unit2(ctx);
+                            const bool ok = enqueue_screen(
+                                ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1, info, q, nw, spw,
+                                fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
+                                !two && q > 0 ? &pending : nullptr, !two && q < nw - 1 ? &mine : nullptr,
+                                shells ? q : -1, bands);
+                            if (on2) swap_unit2(ctx);
+                            if (!ok) return false;
                             pending = mine;
                         }
-                        k_reverify<<<EXACT_BLOCKS, BLOCK, 0, s>>>(ctx->gs, ctx->events,
-                                                                  static_cast<unsigned long long>(ctx->ev_cap), pc,
-                                                                  ctx->radius, m13, info, h, sub,
-                                                                  ctx->cfg.threshold_km, vmin2, ctx->rstats, bands);
+                        // Every event found is solved again on plain trajectories;
+                        // with two streams each stream its own events, then the
+                        // second stream's go into the step's events.
+                        auto reverify = [&]() {
+                            k_reverify<<<EXACT_BLOCKS, BLOCK, 0, ctx->stream>>>(
+                                ctx->gs, ctx->events, static_cast<unsigned long long>(ctx->ev_cap), pc, ctx->radius,
+                                m13, info, h, sub, ctx->cfg.threshold_km, vmin2, ctx->rstats, bands);
+                        };
+                        reverify();
+                        if (two) {
+                            swap_unit2(ctx);
+                            reverify();
+                            swap_unit2(ctx);
+                            KHIP_TRY(ctx, hipEventRecord(ctx->unit2.ev_done, ctx->unit2.stream));
+                            KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->unit2.ev_done, 0));
+                            k_merge_events<<<1, TBLOCK, 0, s>>>(ctx->gs, ctx->events,
+                                                                static_cast<unsigned long long>(ctx->ev_cap),
+                                                                ctx->unit2.gs, ctx->unit2.events,
+                                                                static_cast<unsigned long long>(ctx->unit2.ev_cap));
+                        }
                         if (follow) KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->ev_join, 0));
                         if (bk) {
                             // The step's confirmed collisions, in (t, i, j) order:
@@ -4487,7 +4694,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             };
             if (ctx->use_graph && !ctx->timing) {
                 std::vector<int64_t> key = {static_cast<int64_t>(ctx->gen), len, cur0, screen, ctx->passes, n,
-                                            shells ? ctx->elaunch : 0, follow ? ctx->sample_n : -1,
+                                            two ? 1 : 0, shells ? ctx->elaunch : 0, follow ? ctx->sample_n : -1,
                                             __builtin_bit_cast(int64_t, ctx->sample_t0), bk ? n0 : -1,
                                             bk ? reserve : -1, bk ? ctx->sample_cap : -1};
                 key.insert(key.end(), pattern.begin(), pattern.end());
@@ -4498,7 +4705,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             const int cur = cur_end;
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
-            if (!drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow)) return false;
+            const bool drained = drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow);
+            ctx->unit2_used = false;
+            if (!drained) return false;
             bu::BreakupState bs{};
             if (bk) {
                 KHIP_TRY(ctx, hipMemcpy(&bs, ctx->bstate, sizeof(bs), hipMemcpyDeviceToHost));
@@ -5438,6 +5647,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     khip_ctx* ctx = new khip_ctx;
     ctx->device = device;
     if (const char* f = std::getenv("KHIP_LEARNED_FUSED")) ctx->split_learned = std::atoi(f) == 0;
+    if (const char* f = std::getenv("KHIP_WINDOW_STREAMS")) ctx->window_streams = std::atoi(f) != 0;
 
 
     ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -5474,6 +5684,7 @@ void khip_destroy(khip_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    free_unit2(ctx);
     free_all(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);

@@ -332,3 +332,34 @@ def test_split_kernels_equal_fused(kamd, trained, monkeypatch):
     assert e0.tobytes() == e1.tobytes() and t0.tobytes() == t1.tobytes()
     assert r0 == r1
     assert len(e0) > 50 and r0["learned"] > 0 and r0["base"] > 0 and r0["plain"] > 0
+
+
+def test_window_streams_equal_one_stream(kamd, trained, monkeypatch):
+    """Screening a step's windows on two streams at once (window q on the
+    second stream when q is odd, its events appended to the step's after
+    both finish) gives the one-stream run's results bit for bit: states,
+    flags, events, per-sub-step statistics and method counts, for two and
+    three windows, with and without radial shells, eager and replayed from
+    captured graphs."""
+    _, _, params, norm = trained
+    pop = kamd.generate(60000, seed=12, mix=MIX_DENSE)
+    for nw, shell, graph in ((2, 16.0, False), (3, 16.0, False), (2, 0.0, False), (2, 16.0, True)):
+        out = {}
+        for ws in ("0", "1"):
+            monkeypatch.setenv("KHIP_WINDOW_STREAMS", ws)
+            eng = kamd.Engine(0)
+            try:
+                eng.net_set(params, norm)
+                eng.set_graph(graph)
+                s, a, ev, st = _run(eng, pop, 2, 25, 60.0, True, windows=nw, shell_km=shell)
+                out[ws] = (s, a, ev, st, eng.learned_report())
+            finally:
+                eng.close()
+        (s0, a0, e0, t0, r0), (s1, a1, e1, t1, r1) = out["0"], out["1"]
+        print(f"windows {nw}, shells {shell} km, graph {graph}: {len(e0)} events, "
+              f"{int(t0['candidate_pairs'].sum())} candidate pairs")
+        assert np.array_equal(s0, s1) and np.array_equal(a0, a1), (nw, shell, graph)
+        assert e0.tobytes() == e1.tobytes(), (nw, shell, graph)
+        assert t0.tobytes() == t1.tobytes(), (nw, shell, graph)
+        assert r0 == r1
+        assert len(e0) > 50
