@@ -572,15 +572,23 @@ def ensemble_workload(kamd, args):
 
 
 def cpu_model():
-    """The host CPU's model name (/proc/cpuinfo)."""
+    """The host CPU's model name, with its family and model numbers (/proc/cpuinfo;
+    a virtualised host may report only the product line in the name)."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if fields:
+                        break
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                fields.setdefault(k, v)
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    return f"{name} ({', '.join(extra)})" if extra else name
 
 
 def catalogue_workload(kamd, args):
