@@ -713,13 +713,131 @@ __global__ void __launch_bounds__(BLOCK) k_check_plain(Planes cur, Planes nxt, u
     }
 }
 
+// A check's verdict on one object of the corrector's or base step's domain
+// (cls) from its candidate state (rb, vb: corrected or base) and its plain
+// state (rp, vp): it keeps the cheaper method only within the position and
+// semi-major-axis tolerances (the semi-major axis separately, since a
+// per-step error in it grows into a quadratic along-track drift).
+__device__ __forceinline__ int check_verdict(D3 rb, D3 vb, D3 rp, D3 vp, int cls, const LearnedParams& lp,
+                                             double& dev_max, unsigned int& rejected) {
+    const double dx = rb.x - rp.x, dy = rb.y - rp.y, dz = rb.z - rp.z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double da = fabs(semi_major(rb, vb) - semi_major(rp, vp));
+    if (d <= lp.tol_km && da <= lp.tol_a_km) {
+        dev_max = d > dev_max ? d : dev_max;
+        return cls;
+    }
+    ++rejected;
+    return net::CLASS_PLAIN;
+}
+
+// A block's check tallies, one record per block (k_build_lists combines
+// them): atomics from every wave on the one CheckStats line serialize.
+__device__ __forceinline__ void check_partial(const unsigned int (&cnt)[3], unsigned int rejected, double dev_max,
+                                              CheckStats* out) {
+    __shared__ CheckStats s_p[TWAVES];
+    const int lane = static_cast<int>(__lane_id());
+    const int wave = threadIdx.x / WAVE;
+    CheckStats p;
+    for (int q = 0; q < 3; ++q) p.count[q] = wave_sum(cnt[q]);
+    p.rejected = wave_sum(rejected);
+    p.dev_max = wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(dev_max)));
+    if (lane == 0) s_p[wave] = p;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    CheckStats t = s_p[0];
+    for (int w = 1; w < static_cast<int>(blockDim.x) / WAVE; ++w) {
+        for (int q = 0; q < 3; ++q) t.count[q] += s_p[w].count[q];
+        t.rejected += s_p[w].rejected;
+        t.dev_max = s_p[w].dev_max > t.dev_max ? s_p[w].dev_max : t.dev_max;
+    }
+    *out = t;
+}
+
+// A check's first pass, after k_check_plain (plain states in `nxt`, `alive`
+// = alive and survived the plain sub-steps). Block b takes objects
+// [b * chunk, (b + 1) * chunk): dead objects and those outside both domains
+// take the plain method; the base step's domain is decided here (the mixed
+// step against the plain state); the corrector's domain gets its mixed step
+// written to `base` and is listed for k_net_step<true> (segment b of `list`,
+// in object order, its length in segcnt[b]), so the corrector runs on full
+// waves of its own objects only.
+constexpr int CHECK_BLOCKS = 256;
+
+__global__ void __launch_bounds__(TBLOCK) k_check_base(Planes cur, Planes nxt, Planes base, const uint8_t* alive,
+                                                        uint8_t* mode, int n, int chunk,
+                                                        const MoonSample* __restrict__ moon13, LearnedParams lp,
+                                                        int* list, int* segcnt, CheckStats* parts) {
+    __shared__ int s_wave[TWAVES];
+    const int b0 = blockIdx.x * chunk;
+    const int b1 = b0 + chunk < n ? b0 + chunk : n;
+    unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
+    double dev_max = 0.0;
+    int run = 0;
+    for (int k0 = b0; k0 < b1; k0 += TBLOCK) {
+        const int i = k0 + static_cast<int>(threadIdx.x);
+        bool listed = false;
+        if (i < b1) {
+            const bool on = alive[i] != 0;
+            D3 r{0.0, 0.0, 0.0}, v{0.0, 0.0, 0.0};
+            if (on) {
+                r = D3{cur.x[i], cur.y[i], cur.z[i]};
+                v = D3{cur.vx[i], cur.vy[i], cur.vz[i]};
+            }
+            const int cls = on ? net::domain_class(r, v, lp.e_max, lp.a_max) : net::CLASS_PLAIN;
+            if (cls == net::CLASS_PLAIN) {
+                mode[i] = static_cast<uint8_t>(net::CLASS_PLAIN);
+                if (on) ++cnt[net::CLASS_PLAIN];
+            } else {
+                rk4(r, v, lp.tau, moon13[0], moon13[6], moon13[12], AccelMixed());
+                if (cls == net::CLASS_BASE) {
+                    const D3 rp{nxt.x[i], nxt.y[i], nxt.z[i]}, vp{nxt.vx[i], nxt.vy[i], nxt.vz[i]};
+                    const int m = check_verdict(r, v, rp, vp, cls, lp, dev_max, rejected);
+                    mode[i] = static_cast<uint8_t>(m);
+                    ++cnt[m];
+                } else {
+                    store_state(base, i, r, v);
+                    listed = true;
+                }
+            }
+        }
+        int tot;
+        const int ex = block_exclusive_scan<TWAVES>(listed ? 1 : 0, tot, s_wave);
+        if (listed) list[b0 + run + ex] = i;
+        run += tot;
+    }
+    if (threadIdx.x == 0) segcnt[blockIdx.x] = run;
+    check_partial(cnt, rejected, dev_max, parts + blockIdx.x);
+}
+
+// A check's last pass over the corrector's domain: block b takes segment b
+// of the list, the corrected states k_net_step<true> wrote to `base`, the
+// plain ones in `nxt`; the verdict at full occupancy.
+__global__ void __launch_bounds__(TBLOCK) k_check_verdict(Planes base, Planes nxt, uint8_t* mode,
+                                                           const int* __restrict__ list,
+                                                           const int* __restrict__ segcnt, int chunk,
+                                                           LearnedParams lp, CheckStats* parts) {
+    unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
+    double dev_max = 0.0;
+    const int m_seg = segcnt[blockIdx.x];
+    for (int j = threadIdx.x; j < m_seg; j += TBLOCK) {
+        const int i = list[blockIdx.x * chunk + j];
+        const D3 rb{base.x[i], base.y[i], base.z[i]}, vb{base.vx[i], base.vy[i], base.vz[i]};
+        const D3 rp{nxt.x[i], nxt.y[i], nxt.z[i]}, vp{nxt.vx[i], nxt.vy[i], nxt.vz[i]};
+        const int m = check_verdict(rb, vb, rp, vp, net::CLASS_LEARNED, lp, dev_max, rejected);
+        mode[i] = static_cast<uint8_t>(m);
+        ++cnt[m];
+    }
+    check_partial(cnt, rejected, dev_max, parts + CHECK_BLOCKS + blockIdx.x);
+}
+
 // k_net_step: the corrector on top of k_base_step's states. CHECK false:
 // the corrected list, base states in `nxt`, corrected in place (re-entry
-// test after the correction). CHECK true: every object after k_check_plain
-// (the plain states in `nxt`, `alive` = alive and survived the plain
-// sub-steps), base states in `base`: an object keeps the corrected or base
-// method if it stayed within the tolerances of its plain state, and the
-// check tallies and method flags are written as k_learned<KIND_CHECK> does.
+// test after the correction). CHECK true: the corrector's domain as
+// k_check_base listed it (segments of `list`, lengths in segcnt, `chunk`
+// apart), base states in `base`, corrected in place for k_check_verdict.
+// With k_check_base and k_check_verdict the check's tallies and method
+// flags are those k_learned<KIND_CHECK> writes.
 constexpr int NET_NT = 2;          // tiles per pass in k_net_step
 constexpr int NET_BLOCK = 256;     // two blocks share a CU's LDS: two waves per SIMD
 constexpr int NET_WAVES = NET_BLOCK / WAVE;
@@ -730,7 +848,8 @@ __global__ void __launch_bounds__(NET_BLOCK) __attribute__((amdgpu_waves_per_eu(
                                                      const int* __restrict__ counts, int n,
                                                      const MoonSample* __restrict__ moon13,
                                                      const uint16_t* __restrict__ ffwd, const float* __restrict__ P,
-                                                     net::NetNorm nn, LearnedParams lp, CheckStats* cs) {
+                                                     net::NetNorm nn, LearnedParams lp, CheckStats* cs,
+                                                     const int* __restrict__ segcnt, int chunk) {
     __shared__ uint4 s_frag[net::NFRAG_FWD * net::FRAG_HALVES / 8];
     {
         const uint4* src = reinterpret_cast<const uint4*>(ffwd);
@@ -738,23 +857,63 @@ __global__ void __launch_bounds__(NET_BLOCK) __attribute__((amdgpu_waves_per_eu(
         __syncthreads();
     }
     const uint16_t* frags = reinterpret_cast<const uint16_t*>(s_frag);
-    const int count = CHECK ? n : counts[net::CLASS_LEARNED];
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
-    unsigned int cnt[3] = {0, 0, 0}, rejected = 0;
-    double dev_max = 0.0;
+    // At checks: the listed objects in tiles of 64, segment by segment
+    // (segment b has ceil(segcnt[b] / 64) tiles). Every wave keeps the
+    // tiles' exclusive prefix in registers, segments 4 lane .. 4 lane + 3 on
+    // lane `lane` (no LDS beyond the fragments: two blocks per CU).
+    constexpr int SEG_PER_LANE = CHECK_BLOCKS / WAVE;
+    int tpre[SEG_PER_LANE];
+    int count;
+    if (CHECK) {
+        int own = 0;
+#pragma unroll
+        for (int q = 0; q < SEG_PER_LANE; ++q) {
+            tpre[q] = own;
+            own += (segcnt[SEG_PER_LANE * lane + q] + WAVE - 1) / WAVE;
+        }
+        int incl = own;
+        for (int off = 1; off < WAVE; off <<= 1) {
+            const int o = __shfl_up(incl, off, WAVE);
+            if (lane >= off) incl += o;
+        }
+#pragma unroll
+        for (int q = 0; q < SEG_PER_LANE; ++q) tpre[q] += incl - own;
+        count = __shfl(incl, WAVE - 1, WAVE) * WAVE;
+    } else {
+        count = counts[net::CLASS_LEARNED];
+    }
     for (int tile = blockIdx.x * NET_WAVES + wave; tile * WAVE < count; tile += gridDim.x * NET_WAVES) {
-        const int k = tile * WAVE + lane;
-        const bool active = k < count;
-        const int i = active ? (CHECK ? k : list[k]) : 0;
+        int k = tile * WAVE + lane;
+        bool active = k < count;
+        if (CHECK) {
+            // The tile's segment: the last one whose first tile is at or
+            // before it (wave-uniform), then the lane's place in it.
+            const unsigned long long at = __ballot(tpre[0] <= tile);
+            const int owner = 63 - __clzll(at);
+            int seg = SEG_PER_LANE * owner, first = __shfl(tpre[0], owner, WAVE);
+#pragma unroll
+            for (int q = 1; q < SEG_PER_LANE; ++q) {
+                const int f = __shfl(tpre[q], owner, WAVE);
+                if (f <= tile) {
+                    seg = SEG_PER_LANE * owner + q;
+                    first = f;
+                }
+            }
+            const int in_seg = (tile - first) * WAVE + lane;
+            active = in_seg < segcnt[seg];
+            k = seg * chunk + in_seg;
+        }
+        const int i = active ? list[k] : 0;
         const bool on = active && alive[i] != 0;
         D3 r{0.0, 0.0, 0.0}, v{0.0, 0.0, 0.0};
         if (on) {
             r = D3{cur.x[i], cur.y[i], cur.z[i]};
             v = D3{cur.vx[i], cur.vy[i], cur.vz[i]};
         }
-        const int cls = !on ? net::CLASS_PLAIN
-                            : (CHECK ? net::domain_class(r, v, lp.e_max, lp.a_max) : net::CLASS_LEARNED);
+        // At checks the list holds live objects of the corrector's domain only (k_check_base).
+        const int cls = !on ? net::CLASS_PLAIN : net::CLASS_LEARNED;
         // The base state and the correction frame are taken after the
         // network (fewer registers live across it; the frame is the same
         // arithmetic on the same inputs).
@@ -789,34 +948,9 @@ __global__ void __launch_bounds__(NET_BLOCK) __attribute__((amdgpu_waves_per_eu(
                 store_state(nxt, i, rb, vb);
                 if (sqrt(rb.x * rb.x + rb.y * rb.y + rb.z * rb.z) < dev::R_EARTH) kill(cur, alive, i, rb, vb);
             }
-        } else if (active) {
-            int m = net::CLASS_PLAIN;
-            if (cls != net::CLASS_PLAIN) {
-                const D3 rp{nxt.x[i], nxt.y[i], nxt.z[i]}, vp{nxt.vx[i], nxt.vy[i], nxt.vz[i]};
-                const double dx = rb.x - rp.x, dy = rb.y - rp.y, dz = rb.z - rp.z;
-                const double d = sqrt(dx * dx + dy * dy + dz * dz);
-                const double da = fabs(semi_major(rb, vb) - semi_major(rp, vp));
-                if (d <= lp.tol_km && da <= lp.tol_a_km) {
-                    m = cls;
-                    dev_max = d > dev_max ? d : dev_max;
-                } else {
-                    ++rejected;
-                }
-            }
-            mode[i] = static_cast<uint8_t>(m);
-            if (on) ++cnt[m];
+        } else if (on) {
+            store_state(base, i, rb, vb);  // judged by k_check_verdict
         }
-    }
-    if (CHECK) {
-        for (int q = 0; q < 3; ++q) {
-            const unsigned int s = wave_sum(cnt[q]);
-            if (lane == 0 && s) atomicAdd(&cs->count[q], s);
-        }
-        const unsigned int rj = wave_sum(rejected);
-        if (lane == 0 && rj) atomicAdd(&cs->rejected, rj);
-        const unsigned long long dm =
-            wave_max_u64(static_cast<unsigned long long>(__double_as_longlong(dev_max)));
-        if (lane == 0) atomicMax(&cs->dev_max, dm);
     }
 }
 
@@ -829,9 +963,29 @@ __global__ void __launch_bounds__(NET_BLOCK) __attribute__((amdgpu_waves_per_eu(
 constexpr int LIST_BLOCKS = 256;
 
 __global__ void __launch_bounds__(BLOCK) k_build_lists(const uint8_t* alive, const uint8_t* mode, int n,
-                                                        int* lists, int cap, int* counts) {
+                                                        int* lists, int cap, int* counts,
+                                                        const CheckStats* __restrict__ parts, CheckStats* cs) {
     __shared__ int s_base[3];
     __shared__ int s_wave[3][WAVES_PER_BLOCK];
+    if (parts && blockIdx.x == 0) {
+        // The check's tallies: k_check_base's and k_check_verdict's block records.
+        unsigned int c[3] = {0, 0, 0}, rj = 0;
+        unsigned long long dm = 0;
+        for (int k = threadIdx.x; k < 2 * CHECK_BLOCKS; k += BLOCK) {
+            for (int q = 0; q < 3; ++q) c[q] += parts[k].count[q];
+            rj += parts[k].rejected;
+            dm = parts[k].dev_max > dm ? parts[k].dev_max : dm;
+        }
+        CheckStats p;
+        for (int q = 0; q < 3; ++q) p.count[q] = wave_sum(c[q]);
+        p.rejected = wave_sum(rj);
+        p.dev_max = wave_max_u64(dm);
+        if (__lane_id() == 0) {
+            for (int q = 0; q < 3; ++q) atomicAdd(&cs->count[q], p.count[q]);
+            atomicAdd(&cs->rejected, p.rejected);
+            atomicMax(&cs->dev_max, p.dev_max);
+        }
+    }
     const int chunk = ((n + gridDim.x - 1) / gridDim.x + BLOCK - 1) / BLOCK * BLOCK;
     const int i0 = blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     const int wave = threadIdx.x / WAVE;
@@ -3216,6 +3370,8 @@ struct khip_ctx {
     int* lists = nullptr;        // 3 * cap: learned, plain, base
     int* snap_lists = nullptr;
     int* counts = nullptr;       // 3 list sizes
+    int* segcnt = nullptr;       // a check's corrector-domain list: objects per segment (k_check_base)
+    CheckStats* cparts = nullptr;  // a check's per-block tallies (k_check_base, k_check_verdict)
     CheckStats* cstats = nullptr;
     ReverifyStats* rstats = nullptr;
     double* substates = nullptr; // (substeps - 1) * 6 * cap: dense-output states
@@ -3343,7 +3499,7 @@ void free_all(khip_ctx* ctx) {
                     ctx->step_parts, ctx->win_parts, ctx->pair_parts, ctx->dense_parts, ctx->dense, ctx->items,
                     ctx->exact_parts, ctx->surv, ctx->net_p, ctx->net_m1, ctx->net_m2,
                     ctx->net_ffwd, ctx->net_fbwd, ctx->mode, ctx->snap_mode, ctx->lists,
-                    ctx->snap_lists, ctx->counts, ctx->cstats, ctx->rstats, ctx->substates,
+                    ctx->snap_lists, ctx->counts, ctx->segcnt, ctx->cparts, ctx->cstats, ctx->rstats, ctx->substates,
                     ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tile_base, ctx->sample, ctx->snap_sample,
                     ctx->sample_dev, ctx->bands};
     for (void* p : ptrs)
@@ -4189,7 +4345,8 @@ bool ensure_learned(khip_ctx* ctx) {
     }
     if (!dev_alloc(ctx, &ctx->mode, c) || !dev_alloc(ctx, &ctx->snap_mode, c) ||
         !dev_alloc(ctx, &ctx->lists, 3 * c) || !dev_alloc(ctx, &ctx->snap_lists, 3 * c + 3) ||
-        !dev_alloc(ctx, &ctx->counts, 3) || !dev_alloc(ctx, &ctx->cstats, 1) || !dev_alloc(ctx, &ctx->rstats, 1) ||
+        !dev_alloc(ctx, &ctx->counts, 3) || !dev_alloc(ctx, &ctx->segcnt, CHECK_BLOCKS) ||
+        !dev_alloc(ctx, &ctx->cparts, 2 * CHECK_BLOCKS) || !dev_alloc(ctx, &ctx->cstats, 1) || !dev_alloc(ctx, &ctx->rstats, 1) ||
         !dev_alloc(ctx, &ctx->substates, static_cast<size_t>(std::max(1, sub - 1)) * 6 * c) ||
         !dev_alloc(ctx, &ctx->win_parts, static_cast<size_t>(FUSED_WINDOWS_MAX) * PART_BLOCKS))
         return false;
@@ -4543,21 +4700,28 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     const Planes pc = ctx->planes(cur), pn = ctx->planes(nxt);
                     if (ctx->split_learned && pattern[static_cast<size_t>(k)]) {
                         const Planes pw = dense_planes(ctx, 0);  // free until the screening's windows
+                        // Plain sub-steps for everyone; then the base step's domain
+                        // decided and the corrector's domain listed (in the learned
+                        // list's room, rebuilt below); then the corrector on that list.
+                        const int chunk = (n + CHECK_BLOCKS - 1) / CHECK_BLOCKS;
                         k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
-                        k_base_step<KIND_CHECK><<<grid_of(n), BLOCK, 0, s>>>(pc, pw, ctx->alive, nullptr,
-                                                                             ctx->counts, n, m13, lp);
                         k_check_plain<<<grid_of(n), BLOCK, 0, s>>>(pc, pn, ctx->alive, n, m13, lp);
-                        k_net_step<true><<<std::min((n + NET_BLOCK - 1) / NET_BLOCK, 2 * g_net), NET_BLOCK, 0, s>>>(
-                            pc, pn, pw, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd,
-                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                        k_check_base<<<CHECK_BLOCKS, TBLOCK, 0, s>>>(pc, pn, pw, ctx->alive, ctx->mode, n, chunk, m13,
+                                                                    lp, ctx->lists, ctx->segcnt, ctx->cparts);
+                        k_net_step<true><<<2 * g_net, NET_BLOCK, 0, s>>>(
+                            pc, pn, pw, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats, ctx->segcnt, chunk);
+                        k_check_verdict<<<CHECK_BLOCKS, TBLOCK, 0, s>>>(pw, pn, ctx->mode, ctx->lists, ctx->segcnt,
+                                                                        chunk, lp, ctx->cparts);
                         k_build_lists<<<std::max(1, std::min(grid_of(n), LIST_BLOCKS)), BLOCK, 0, s>>>(
-                            ctx->alive, ctx->mode, n, ctx->lists, static_cast<int>(c), ctx->counts);
+                            ctx->alive, ctx->mode, n, ctx->lists, static_cast<int>(c), ctx->counts, ctx->cparts,
+                            ctx->cstats);
                     } else if (ctx->split_learned) {
                         k_base_step<net::CLASS_LEARNED><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists,
                                                                                 ctx->counts, n, m13, lp);
                         k_net_step<false><<<2 * g_net, NET_BLOCK, 0, s>>>(
                             pc, pn, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
-                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
+                            ctx->net_p, ctx->net_norm, lp, ctx->cstats, nullptr, 0);
                         k_base_step<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists + 2 * c,
                                                                              ctx->counts, n, m13, lp);
                         k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
@@ -4569,7 +4733,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             pc, pn, ctx->alive, ctx->mode, nullptr, ctx->counts, n, m13, ctx->net_ffwd, ctx->net_p,
                             ctx->net_norm, lp, ctx->cstats);
                         k_build_lists<<<std::max(1, std::min(grid_of(n), LIST_BLOCKS)), BLOCK, 0, s>>>(ctx->alive, ctx->mode, n, ctx->lists,
-                                                                   static_cast<int>(c), ctx->counts);
+                                                                   static_cast<int>(c), ctx->counts, nullptr, ctx->cstats);
                     } else {
                         k_learned<net::CLASS_LEARNED><<<g_net, BLOCK, 0, s>>>(
                             pc, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
