@@ -45,13 +45,16 @@
 // The learned form (KHIP_PRECISION_LEARNED) replaces k_propagate by the
 // method kernels over per-method lists (k_base_step: the mixed 60 s step at
 // full occupancy; k_net_step: the corrector on MFMA; k_learned<PLAIN>: the
-// plain sub-steps; at checks k_check_plain for everyone), follows a sample of
-// objects on plain twins beside the screening (k_sample_step, side stream),
-// screens each step in windows on the step's cubic dense output (k_windows:
-// window boundary states, reductions and, with radial shells, each object's
-// shells; k_keys_shell: one sorted element per (object, shell)), and solves
-// every event again on plain FP64 trajectories (k_reverify). In-run
-// breakups (k_bu_*) follow a step's screening in every form.
+// plain sub-steps; at checks k_check_plain for everyone, then k_check_base,
+// k_net_step<true> on the corrector's domain and k_check_verdict), follows a
+// sample of objects on plain twins beside the screening (k_sample_step, side
+// stream), screens each step in windows on the step's cubic dense output
+// (k_windows: window boundary states, reductions and, with radial shells,
+// each object's shells; k_keys_shell: one sorted element per (object,
+// shell)), window q on a second stream with its own grid buffers when q is
+// odd, and solves every event again on plain FP64 trajectories (k_reverify,
+// each stream its own events, k_merge_events joining them). In-run breakups
+// (k_bu_*) follow a step's screening in every form.
 //
 // The host only enqueues; it synchronises once per chunk of steps to drain
 // events and statistics.
