@@ -3318,7 +3318,6 @@ struct khip_ctx {
     int bu_reserve = 0;
     bool reset_breakups = true;    // breakup ids restart with a new population
     std::vector<khip_breakup_event> host_breakups;
-    size_t last_chunk_events = 0;  // events the last drained chunk appended
     int4* coltab = nullptr;  // column table, 1 << COL_BITS entries
     int* scen = nullptr;     // scenario id per object (concatenated scenarios)
     int sbits = 0;
@@ -3433,7 +3432,15 @@ struct khip_ctx {
     int64_t last_rerun = 0;  // reasons of the last re-run (KHIP_RERUN_* bits)
 
     std::vector<khip_event> host_events;
-    std::vector<khip_event> raw_events;  // one chunk's events as the device wrote them
+    // One drained chunk's events as the device wrote them, in pinned host
+    // memory (a full-speed copy), ordered into host_events later: the chunk
+    // loops do that while the next chunk runs on the device.
+    khip_event* ev_pinned = nullptr;
+    int64_t ev_pinned_cap = 0;
+    struct {
+        bool on = false;
+        int64_t step0 = 0, steps = 0, n = 0;
+    } pend;
     std::vector<khip_stats> host_stats;
 
     bool timing = false;
@@ -3917,7 +3924,54 @@ void accumulate_timing_learned(khip_ctx* ctx, int64_t steps, int sub, int nw, bo
 // device buffer (grown here), or a step needed more radix passes than were
 // launched (raised here). The pass count also shrinks to what the chunk
 // needed, so the next chunk launches no idle passes.
-bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& rerun) {
+// Orders a drained chunk's events (pinned staging) into host_events.
+void flush_pending(khip_ctx* ctx) {
+    if (!ctx->pend.on) return;
+    ctx->pend.on = false;
+    const int64_t step0 = ctx->pend.step0, steps = ctx->pend.steps;
+    const khip_event* raw = ctx->ev_pinned;
+    const int64_t ne = ctx->pend.n;
+    // (step, t, i, j) order, as Screener::finish returns each step's events
+    // (collision.cpp:739-747): counting sort on the chunk's step index, then
+    // each step's events sorted on their own, steps in parallel. Events the
+    // plain re-verification of the learned form rejected (pad 1) are left out.
+    std::vector<int64_t> start(static_cast<size_t>(steps) + 1, 0);
+    int64_t nk = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        if (raw[e].pad == 1) continue;
+        ++start[static_cast<size_t>(raw[e].step - step0) + 1];
+        ++nk;
+    }
+    for (int64_t k = 0; k < steps; ++k) start[k + 1] += start[k];
+    std::vector<khip_event>& host = ctx->host_events;
+    const size_t out0 = host.size();
+    if (out0 + static_cast<size_t>(nk) > host.capacity())
+        host.reserve(std::max(out0 + static_cast<size_t>(nk), 2 * host.capacity()));
+    host.resize(out0 + static_cast<size_t>(nk));
+    khip_event* out = host.data() + out0;
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (int64_t e = 0; e < ne; ++e)
+        if (raw[e].pad != 1) out[fill[static_cast<size_t>(raw[e].step - step0)]++] = raw[e];
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t k = 0; k < steps; ++k)
+        std::sort(out + start[k], out + start[k + 1], [](const khip_event& a, const khip_event& b) {
+            if (a.t != b.t) return a.t < b.t;
+            if (a.i != b.i) return a.i < b.i;
+            return a.j < b.j;
+        });
+}
+
+// A drained chunk that has to run again: its staged events are dropped.
+void discard_pending(khip_ctx* ctx) { ctx->pend.on = false; }
+
+// Orders the staged events when the caller returns (every exit path).
+struct FlushOnExit {
+    khip_ctx* ctx;
+    ~FlushOnExit() { flush_pending(ctx); }
+};
+
+bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& rerun, bool defer = false) {
+    flush_pending(ctx);  // the staging buffer is about to be reused
     GridState g;
     KHIP_TRY(ctx, hipMemcpyAsync(&g, ctx->gs, sizeof(g), hipMemcpyDeviceToHost, ctx->stream));
     KHIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -3977,35 +4031,24 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
     }
     if (!screen) return true;
     const int64_t ne = static_cast<int64_t>(g.events);
-    std::vector<khip_event>& raw = ctx->raw_events;
-    raw.resize(static_cast<size_t>(ne));
+    if (ne > ctx->ev_pinned_cap) {
+        if (ctx->ev_pinned) (void)hipHostFree(ctx->ev_pinned);
+        ctx->ev_pinned = nullptr;
+        ctx->ev_pinned_cap = 0;
+        const int64_t want = ne + ne / 4 + 1024;
+        KHIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->ev_pinned), sizeof(khip_event) * want));
+        ctx->ev_pinned_cap = want;
+    }
     if (ne > 0)
-        KHIP_TRY(ctx, hipMemcpy(raw.data(), ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
-    // (step, t, i, j) order, as Screener::finish returns each step's events
-    // (collision.cpp:739-747): counting sort on the chunk's step index, then
-    // each step's events sorted on their own, steps in parallel.
-    std::vector<int64_t> start(static_cast<size_t>(steps) + 1, 0);
-    // Events the plain re-verification of the learned form rejected (pad 1).
-    raw.erase(std::remove_if(raw.begin(), raw.end(), [](const khip_event& e) { return e.pad == 1; }), raw.end());
-    const int64_t nk = static_cast<int64_t>(raw.size());
-    for (const khip_event& e : raw) ++start[static_cast<size_t>(e.step - step0) + 1];
-    for (int64_t k = 0; k < steps; ++k) start[k + 1] += start[k];
-    const size_t out0 = ctx->host_events.size();
-    ctx->host_events.resize(out0 + static_cast<size_t>(nk));
-    ctx->last_chunk_events = static_cast<size_t>(nk);
-    khip_event* out = ctx->host_events.data() + out0;
-    std::vector<int64_t> fill(start.begin(), start.end() - 1);
-    for (const khip_event& e : raw) out[fill[static_cast<size_t>(e.step - step0)]++] = e;
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int64_t k = 0; k < steps; ++k)
-        std::sort(out + start[k], out + start[k + 1], [](const khip_event& a, const khip_event& b) {
-            if (a.t != b.t) return a.t < b.t;
-            if (a.i != b.i) return a.i < b.i;
-            return a.j < b.j;
-        });
+        KHIP_TRY(ctx, hipMemcpy(ctx->ev_pinned, ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
     std::vector<khip_stats> st(static_cast<size_t>(steps));
     KHIP_TRY(ctx, hipMemcpy(st.data(), ctx->stats, sizeof(khip_stats) * steps, hipMemcpyDeviceToHost));
     ctx->host_stats.insert(ctx->host_stats.end(), st.begin(), st.end());
+    ctx->pend.on = true;
+    ctx->pend.step0 = step0;
+    ctx->pend.steps = steps;
+    ctx->pend.n = ne;
+    if (!defer) flush_pending(ctx);
     return true;
 }
 
@@ -4086,6 +4129,7 @@ bool ensure_steptab(khip_ctx* ctx, int64_t len) {
 }
 
 bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
+    FlushOnExit flush_guard{ctx};
     hipStream_t s = ctx->stream;
     if (ctx->n == 0) {
         for (int64_t k = 0; k < nsteps; ++k) {
@@ -4228,20 +4272,22 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             } else if (!enqueue_chunk()) {
                 return false;
             }
+            // The previous chunk's events are ordered while this one runs.
+            flush_pending(ctx);
             // The chunk's end (a replayed graph runs no host code).
             t = t_chunk;
             for (int64_t k = 0; k < len; ++k) t += ctx->dt;
             cur = (len & 1) ? cur0 ^ 1 : cur0;
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
-            if (!drain_chunk(ctx, ctx->step + done, len, screen, overflow)) return false;
+            if (!drain_chunk(ctx, ctx->step + done, len, screen, overflow, true)) return false;
             bu::BreakupState bs{};
             if (bk) {
                 KHIP_TRY(ctx, hipMemcpy(&bs, ctx->bstate, sizeof(bs), hipMemcpyDeviceToHost));
                 if (bs.jobs_short || bs.samples_short || bs.reserve_short) {
                     if (!overflow) {
                         // drain_chunk kept this chunk's events and statistics: drop them.
-                        ctx->host_events.resize(ctx->host_events.size() - ctx->last_chunk_events);
+                        discard_pending(ctx);
                         ctx->host_stats.resize(ctx->host_stats.size() - static_cast<size_t>(len));
                     }
                     overflow = true;
@@ -4504,6 +4550,7 @@ bool ensure_sample(khip_ctx* ctx) {
 }
 
 bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
+    FlushOnExit flush_guard{ctx};
     if (!ctx->net_ready) {
         ctx->error = "learned form: no corrector loaded or trained";
         return false;
@@ -4869,10 +4916,12 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
             } else if (!enqueue_chunk()) {
                 return false;
             }
+            // The previous chunk's events are ordered while this one runs.
+            flush_pending(ctx);
             const int cur = cur_end;
             KHIP_TRY(ctx, hipGetLastError());
             bool overflow = false;
-            const bool drained = drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow);
+            const bool drained = drain_chunk(ctx, (ctx->step + done) * sub, len * sub, screen, overflow, true);
             ctx->unit2_used = false;
             if (!drained) return false;
             bu::BreakupState bs{};
@@ -4881,7 +4930,7 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                 if (bs.jobs_short || bs.samples_short || bs.reserve_short) {
                     if (!overflow) {
                         // drain_chunk kept this chunk's events and statistics: drop them.
-                        ctx->host_events.resize(ctx->host_events.size() - ctx->last_chunk_events);
+                        discard_pending(ctx);
                         ctx->host_stats.resize(ctx->host_stats.size() - static_cast<size_t>(len * sub));
                     }
                     overflow = true;
@@ -5853,6 +5902,7 @@ void khip_destroy(khip_ctx* ctx) {
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     free_unit2(ctx);
     free_all(ctx);
+    if (ctx->ev_pinned) (void)hipHostFree(ctx->ev_pinned);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
