@@ -1305,13 +1305,14 @@ __device__ __forceinline__ void finish_window(GridState* gs, DevStats* out, int 
         // blocks. Loads go in batches of FIN_BATCH per thread, so a thread
         // waits once per batch rather than once per record.
         constexpr int FIN_BATCH = 8;
+        const int nt = static_cast<int>(blockDim.x);
         const int np = nparts > npair ? nparts : npair;
-        for (int k0 = threadIdx.x; k0 < np; k0 += TBLOCK * FIN_BATCH) {
+        for (int k0 = threadIdx.x; k0 < np; k0 += nt * FIN_BATCH) {
             unsigned int mc[FIN_BATCH];
             unsigned long long cd[FIN_BATCH];
 #pragma unroll
             for (int u = 0; u < FIN_BATCH; ++u) {
-                const int k = k0 + u * TBLOCK;
+                const int k = k0 + u * nt;
                 mc[u] = k < nparts ? parts[k].max_cell : 0u;
                 cd[u] = k < npair ? parts[k].cand : 0ULL;
             }
@@ -1321,8 +1322,8 @@ __device__ __forceinline__ void finish_window(GridState* gs, DevStats* out, int 
                 a += cd[u];
             }
         }
-        for (int k = threadIdx.x; k < ndparts; k += TBLOCK) dn += dparts[k].cand;
-        for (int k = threadIdx.x; k < neparts; k += TBLOCK) {
+        for (int k = threadIdx.x; k < ndparts; k += nt) dn += dparts[k].cand;
+        for (int k = threadIdx.x; k < neparts; k += nt) {
             b += eparts[k].refined;
             c += eparts[k].co;
             sv += eparts[k].cand;
@@ -1345,7 +1346,8 @@ __device__ __forceinline__ void finish_window(GridState* gs, DevStats* out, int 
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    for (int w = 1; w < TWAVES; ++w) {
+    const int waves = static_cast<int>(blockDim.x) / WAVE;  // at most TWAVES
+    for (int w = 1; w < waves; ++w) {
         s_red[0][0] += s_red[w][0];
         s_red[0][1] += s_red[w][1];
         s_red[0][2] += s_red[w][2];
@@ -2829,7 +2831,7 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
                                                      Planes s0, const double* radius,
                                                      const MoonSample* __restrict__ moon13, const StepInfo* info,
                                                      double h, int sub, double thr_report, double vmin2,
-                                                     ReverifyStats* rs, const Bands* bands) {
+                                                     ReverifyStats* rs, const Bands* bands, FinishArgs fin) {
     if (bands) {
         thr_report = bands->thr_report;
         vmin2 = bands->vmin2_report;
@@ -2892,6 +2894,11 @@ __global__ void __launch_bounds__(BLOCK) k_reverify(GridState* gs, DevEvent* eve
         if (drop) atomicAdd(&rs->dropped, drop);
         if (kept) atomicAdd(&rs->kept_unverified, kept);
     }
+    // The screening's statistics record (in place of a k_finish launch); it
+    // touches no field read above.
+    if (blockIdx.x == 0 && fin.out)
+        finish_window(gs, fin.out, 1, fin.parts, fin.nparts, fin.npair, fin.dparts, fin.ndparts, fin.eparts,
+                      fin.neparts, fin.fill);
 }
 
 __global__ void k_zero_reverify(ReverifyStats* rs) {
@@ -4217,6 +4224,7 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
             k_zero_events<<<1, 64, 0, s>>>(ctx->gs);
             t = t_chunk;
             cur = cur0;
+            FinishArgs pending{};  // step k - 1's record, written by step k's grid setup
             for (int64_t k = 0; k < len; ++k) {
                 const int nxt = cur ^ 1;
                 g_slot = static_cast<int>(k);
@@ -4232,10 +4240,13 @@ bool run_steps(khip_ctx* ctx, int64_t nsteps, int screen) {
                 mark(ctx, 1);
                 if (screen) {
                     if (bk) k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
+                    FinishArgs mine{};
                     if (!enqueue_screen(ctx, ctx->planes(cur), ctx->planes(nxt), t, ctx->step + done + k,
                                         static_cast<int>(k), ctx->dt, ctx->cfg.threshold_km, 0,
-                                        graphed ? ctx->steptab + k : nullptr))
+                                        graphed ? ctx->steptab + k : nullptr, 0, 1, 1, nullptr,
+                                        pending.out ? &pending : nullptr, k + 1 < len ? &mine : nullptr))
                         return false;
+                    pending = mine;
                     if (bk) {
                         const Planes pb = ctx->planes(cur), pa = ctx->planes(nxt);
                         int* sc = ctx->sbits ? ctx->scen : nullptr;
@@ -4815,7 +4826,9 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                         // up to FUSED_WINDOWS_MAX windows' states and reductions
                         // come from one pass over the step's start and end.
                         const bool fused = nw <= FUSED_WINDOWS_MAX;
-                        FinishArgs pending{};  // window q - 1's record, written by window q's grid setup
+                        // Per stream, the previous window's record, written by the next
+                        // window's grid setup; the stream's last one by its k_reverify.
+                        FinishArgs pending[2] = {};
                         if (!fused)
                             k_mark_events<<<1, 64, 0, s>>>(ctx->gs, static_cast<unsigned long long>(ctx->ev_cap));
                         for (int q = 0; q < nw; ++q) {
@@ -4842,29 +4855,29 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             // With two streams every window writes its own record
                             // (no record is passed between windows of different streams).
                             const bool on2 = two && (q & 1);
+                            FinishArgs& prev = pending[on2 ? 1 : 0];
                             FinishArgs mine{};
                             if (on2) swap_unit2(ctx);
                             const bool ok = enqueue_screen(
                                 ctx, pb, pa, 0.0, 0, static_cast<int>(k * sub + q * spw), hw, thr, 1, info, q, nw, spw,
                                 fused ? ctx->win_parts + static_cast<size_t>(q) * part_grid(n) : nullptr,
-                                !two && q > 0 ? &pending : nullptr, !two && q < nw - 1 ? &mine : nullptr,
-                                shells ? q : -1, bands);
+                                prev.out ? &prev : nullptr, &mine, shells ? q : -1, bands);
                             if (on2) swap_unit2(ctx);
                             if (!ok) return false;
-                            pending = mine;
+                            prev = mine;
                         }
                         // Every event found is solved again on plain trajectories;
                         // with two streams each stream its own events, then the
                         // second stream's go into the step's events.
-                        auto reverify = [&]() {
+                        auto reverify = [&](const FinishArgs& fin) {
                             k_reverify<<<EXACT_BLOCKS, BLOCK, 0, ctx->stream>>>(
                                 ctx->gs, ctx->events, static_cast<unsigned long long>(ctx->ev_cap), pc, ctx->radius,
-                                m13, info, h, sub, ctx->cfg.threshold_km, vmin2, ctx->rstats, bands);
+                                m13, info, h, sub, ctx->cfg.threshold_km, vmin2, ctx->rstats, bands, fin);
                         };
-                        reverify();
+                        reverify(pending[0]);
                         if (two) {
                             swap_unit2(ctx);
-                            reverify();
+                            reverify(pending[1]);
                             swap_unit2(ctx);
                             KHIP_TRY(ctx, hipEventRecord(ctx->unit2.ev_done, ctx->unit2.stream));
                             KHIP_TRY(ctx, hipStreamWaitEvent(s, ctx->unit2.ev_done, 0));
