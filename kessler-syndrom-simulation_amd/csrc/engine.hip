@@ -16,8 +16,8 @@
 //                 and a packed cell key (per-axis offsets from the bounding
 //                 box, as few bits as it needs: 27 at 1M objects, so the
 //                 radix sort runs three 9-bit passes); also writes the
-//                 previous screening's statistics record when one is
-//                 deferred to it (the learned form's windows).
+//                 previous step's (or window's) statistics record and resets
+//                 its accumulators.
 //   k_keys        packed cell key per object (dead objects get a sentinel
 //                 that sorts last), the object's compact FP32 record, and
 //                 the first pass's digit histogram.
@@ -40,7 +40,8 @@
 //                 dense cells: a wave per item); survivors go to a queue.
 //   k_exact       the queue: FP64 filter_pair, Hermite + golden-section TCA,
 //                 radius test; events appended with wave-aggregated atomics.
-//   k_finish      the step's statistics record, accumulators reset.
+//   k_finish      the statistics record of a chunk's last step (earlier
+//                 steps' records are written by the next k_grid_setup).
 //
 // The learned form (KHIP_PRECISION_LEARNED) replaces k_propagate by the
 // method kernels over per-method lists (k_base_step: the mixed 60 s step at
@@ -53,11 +54,13 @@
 // each object's shells; k_keys_shell: one sorted element per (object,
 // shell)), window q on a second stream with its own grid buffers when q is
 // odd, and solves every event again on plain FP64 trajectories (k_reverify,
-// each stream its own events, k_merge_events joining them). In-run breakups
-// (k_bu_*) follow a step's screening in every form.
+// each stream its own events and its last window's statistics record,
+// k_merge_events joining them). In-run breakups (k_bu_*) follow a step's
+// screening in every form.
 //
 // The host only enqueues; it synchronises once per chunk of steps to drain
-// events and statistics.
+// events (into pinned memory) and statistics, and orders a chunk's events
+// while the next chunk runs.
 
 #include <hip/hip_runtime.h>
 
