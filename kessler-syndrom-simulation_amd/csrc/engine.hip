@@ -3447,6 +3447,7 @@ struct khip_ctx {
     // loops do that while the next chunk runs on the device.
     khip_event* ev_pinned = nullptr;
     int64_t ev_pinned_cap = 0;
+    std::vector<khip_event> ev_pageable;  // the staging when pinned memory is refused
     struct {
         bool on = false;
         int64_t step0 = 0, steps = 0, n = 0;
@@ -3939,7 +3940,7 @@ void flush_pending(khip_ctx* ctx) {
     if (!ctx->pend.on) return;
     ctx->pend.on = false;
     const int64_t step0 = ctx->pend.step0, steps = ctx->pend.steps;
-    const khip_event* raw = ctx->ev_pinned;
+    const khip_event* raw = ctx->ev_pinned ? ctx->ev_pinned : ctx->ev_pageable.data();
     const int64_t ne = ctx->pend.n;
     // (step, t, i, j) order, as Screener::finish returns each step's events
     // (collision.cpp:739-747): counting sort on the chunk's step index, then
@@ -4046,11 +4047,18 @@ bool drain_chunk(khip_ctx* ctx, int64_t step0, int64_t steps, int screen, bool& 
         ctx->ev_pinned = nullptr;
         ctx->ev_pinned_cap = 0;
         const int64_t want = ne + ne / 4 + 1024;
-        KHIP_TRY(ctx, hipHostMalloc(reinterpret_cast<void**>(&ctx->ev_pinned), sizeof(khip_event) * want));
+        if (hipHostMalloc(reinterpret_cast<void**>(&ctx->ev_pinned), sizeof(khip_event) * want) == hipSuccess) {
+            ctx->ev_pageable = std::vector<khip_event>();
+        } else {
+            // Pinned memory refused: pageable staging (a slower copy).
+            (void)hipGetLastError();
+            ctx->ev_pinned = nullptr;
+            ctx->ev_pageable.resize(static_cast<size_t>(want));
+        }
         ctx->ev_pinned_cap = want;
     }
-    if (ne > 0)
-        KHIP_TRY(ctx, hipMemcpy(ctx->ev_pinned, ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
+    khip_event* stage = ctx->ev_pinned ? ctx->ev_pinned : ctx->ev_pageable.data();
+    if (ne > 0) KHIP_TRY(ctx, hipMemcpy(stage, ctx->events, sizeof(khip_event) * ne, hipMemcpyDeviceToHost));
     std::vector<khip_stats> st(static_cast<size_t>(steps));
     KHIP_TRY(ctx, hipMemcpy(st.data(), ctx->stats, sizeof(khip_stats) * steps, hipMemcpyDeviceToHost));
     ctx->host_stats.insert(ctx->host_stats.end(), st.begin(), st.end());
