@@ -44,9 +44,9 @@
 //                 steps' records are written by the next k_grid_setup).
 //
 // The learned form (KHIP_PRECISION_LEARNED) replaces k_propagate by the
-// method kernels over per-method lists (k_base_step: the mixed 60 s step at
-// full occupancy; k_net_step: the corrector on MFMA; k_learned<PLAIN>: the
-// plain sub-steps; at checks k_check_plain for everyone, then k_check_base,
+// method kernels over per-method lists (k_fp64_methods: the plain sub-steps
+// and the mixed 60 s base steps at full occupancy; k_net_step: the corrector
+// on MFMA; at checks k_check_plain for everyone, then k_check_base,
 // k_net_step<true> on the corrector's domain and k_check_verdict), follows a
 // sample of objects on plain twins beside the screening (k_sample_step, side
 // stream), screens each step in windows on the step's cubic dense output
@@ -678,32 +678,69 @@ __global__ void __launch_bounds__(BLOCK) k_learned(Planes cur, Planes nxt, uint8
     }
 }
 
-// The same step split by occupancy: the FP64 work runs in light kernels, one
+// The same step split by occupancy: the FP64 work runs in a light kernel, one
 // object per lane at full occupancy, and only the corrector (the MFMA work,
 // whose weights fill LDS) in a kernel of its own. Arithmetic and results are
 // those of k_learned, bit for bit.
 //
-// k_base_step: the mixed RK4 step of tau. KIND CLASS_LEARNED / CLASS_BASE:
-// the list's objects into `out` (= the next states; for CLASS_BASE with the
-// re-entry test, the corrected objects get theirs after the correction);
-// KIND_CHECK: every live object of the corrector's or base step's domain
-// into `out` (a scratch set of planes), the flags untouched.
-template <int KIND>
-__global__ void __launch_bounds__(BLOCK) k_base_step(Planes cur, Planes out, uint8_t* alive,
-                                                      const int* __restrict__ list, const int* __restrict__ counts,
-                                                      int n, const MoonSample* __restrict__ moon13, LearnedParams lp) {
-    const int count = KIND == KIND_CHECK ? n : counts[KIND];
-    for (int k = blockIdx.x * BLOCK + threadIdx.x; k < count; k += gridDim.x * BLOCK) {
-        const int i = KIND == KIND_CHECK ? k : list[k];
+// k_fp64_methods: the FP64 work of a step that is not a check, in one
+// launch: the plain list's sub-steps and the mixed base step of tau of the
+// corrected and base lists (into the next states; the base list with the
+// re-entry test, the corrected objects get theirs after the correction),
+// object for object k_learned's operations. The waves split between the plain list
+// and the two base lists by cost (substeps RK4 steps per plain object, one
+// per base step), each part striding over its tiles of 64; a plain object's
+// chain of sub-steps then runs beside the base steps instead of after them.
+// Lists: corrected at `lists`, plain at lists + c, base at lists + 2c.
+__global__ void __launch_bounds__(BLOCK) k_fp64_methods(Planes cur, Planes nxt, uint8_t* alive,
+                                                         const int* __restrict__ lists, size_t c,
+                                                         const int* __restrict__ counts,
+                                                         const MoonSample* __restrict__ moon13, LearnedParams lp) {
+    const int nl = counts[net::CLASS_LEARNED], nb = counts[net::CLASS_BASE], np = counts[net::CLASS_PLAIN];
+    const int lane = static_cast<int>(__lane_id());
+    const int waves = static_cast<int>(gridDim.x) * WAVES_PER_BLOCK;
+    const int w = static_cast<int>(blockIdx.x) * WAVES_PER_BLOCK + static_cast<int>(threadIdx.x) / WAVE;
+    const int tp = (np + WAVE - 1) / WAVE, tl = (nl + WAVE - 1) / WAVE, tb = (nb + WAVE - 1) / WAVE;
+    // Waves for the plain list: its share of the cost, at least one when it
+    // has objects and at most all but one when the base lists have some.
+    const long long cost_p = static_cast<long long>(lp.substeps) * np, cost_b = static_cast<long long>(nl) + nb;
+    int wp = cost_p + cost_b > 0 ? static_cast<int>((waves * cost_p + cost_p + cost_b - 1) / (cost_p + cost_b)) : 0;
+    if (tp > 0 && wp < 1) wp = 1;
+    if (tl + tb > 0 && wp > waves - 1) wp = waves - 1;
+    if (wp > tp) wp = tp;
+    if (w < wp) {
+        const int* list = lists + c;
+        const double h = lp.tau / lp.substeps;
+        for (int tile = w; tile < tp; tile += wp) {
+            const int k = tile * WAVE + lane;
+            if (k >= np) continue;
+            const int i = list[k];
+            D3 r{cur.x[i], cur.y[i], cur.z[i]}, v{cur.vx[i], cur.vy[i], cur.vz[i]};
+            if (!alive[i]) {
+                store_state(nxt, i, r, v);  // frozen
+                continue;
+            }
+            const bool survives = plain_substeps(r, v, moon13, h, lp.substeps);
+            store_state(nxt, i, r, v);
+            if (!survives) kill(cur, alive, i, r, v);
+        }
+        return;
+    }
+    const int wb = waves - wp;
+    for (int tile = w - wp; tile < tl + tb; tile += wb) {
+        const bool learned = tile < tl;
+        const int k = (learned ? tile : tile - tl) * WAVE + lane;
+        if (k >= (learned ? nl : nb)) continue;
+        const int i = learned ? lists[k] : lists[2 * c + k];
         D3 r{cur.x[i], cur.y[i], cur.z[i]}, v{cur.vx[i], cur.vy[i], cur.vz[i]};
         if (!alive[i]) {
-            if (KIND != KIND_CHECK) store_state(out, i, r, v);  // frozen
+            store_state(nxt, i, r, v);  // frozen
             continue;
         }
-        if (KIND == KIND_CHECK && net::domain_class(r, v, lp.e_max, lp.a_max) == net::CLASS_PLAIN) continue;
         rk4(r, v, lp.tau, moon13[0], moon13[6], moon13[12], AccelMixed());
-        store_state(out, i, r, v);
-        if (KIND == net::CLASS_BASE && sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) kill(cur, alive, i, r, v);
+        store_state(nxt, i, r, v);
+        // The corrected objects' re-entry test follows their correction.
+        if (!learned && sqrt(r.x * r.x + r.y * r.y + r.z * r.z) < dev::R_EARTH) kill(cur, alive, i, r, v);
     }
 }
 
@@ -837,7 +874,7 @@ __global__ void __launch_bounds__(TBLOCK) k_check_verdict(Planes base, Planes nx
     check_partial(cnt, rejected, dev_max, parts + CHECK_BLOCKS + blockIdx.x);
 }
 
-// k_net_step: the corrector on top of k_base_step's states. CHECK false:
+// k_net_step: the corrector on top of the base step's states. CHECK false:
 // the corrected list, base states in `nxt`, corrected in place (re-entry
 // test after the correction). CHECK true: the corrector's domain as
 // k_check_base listed it (segments of `list`, lengths in segcnt, `chunk`
@@ -4789,16 +4826,13 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                             ctx->alive, ctx->mode, n, ctx->lists, static_cast<int>(c), ctx->counts, ctx->cparts,
                             ctx->cstats);
                     } else if (ctx->split_learned) {
-                        k_base_step<net::CLASS_LEARNED><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists,
-                                                                                ctx->counts, n, m13, lp);
+                        // The FP64 methods in one launch, then the corrector on the
+                        // corrected list's base steps.
+                        k_fp64_methods<<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists, c, ctx->counts, m13,
+                                                               lp);
                         k_net_step<false><<<2 * g_net, NET_BLOCK, 0, s>>>(
                             pc, pn, pn, ctx->alive, ctx->mode, ctx->lists, ctx->counts, n, m13, ctx->net_ffwd,
                             ctx->net_p, ctx->net_norm, lp, ctx->cstats, nullptr, 0);
-                        k_base_step<net::CLASS_BASE><<<g_all, BLOCK, 0, s>>>(pc, pn, ctx->alive, ctx->lists + 2 * c,
-                                                                             ctx->counts, n, m13, lp);
-                        k_learned<net::CLASS_PLAIN><<<g_all, BLOCK, 0, s>>>(
-                            pc, pn, ctx->alive, ctx->mode, ctx->lists + c, ctx->counts, n, m13, ctx->net_ffwd,
-                            ctx->net_p, ctx->net_norm, lp, ctx->cstats);
                     } else if (pattern[static_cast<size_t>(k)]) {
                         k_check_begin<<<1, 64, 0, s>>>(ctx->cstats, ctx->counts);
                         k_learned<KIND_CHECK><<<std::min(grid_of(n), g_net), BLOCK, 0, s>>>(
