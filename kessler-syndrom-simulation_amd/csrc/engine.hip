@@ -427,6 +427,12 @@ struct Bands {
     float thr_f, base_f, vmin2_f, pad;
 };
 
+// Per-step values read on the device (the step table of a chunk).
+struct StepInfo {
+    double t;         // start of the step (s since epoch)
+    long long macro;  // global learned-step index
+};
+
 struct BandCfg {
     double factor;     // margin_factor (0: no band)
     double threshold;  // configured threshold (km)
@@ -1214,8 +1220,18 @@ __global__ void __launch_bounds__(BLOCK) k_windows(Planes a, Planes b, const uin
                                                     double* dense, size_t cap, StepPartial* partials,
                                                     GridState* gs, unsigned long long ev_cap, int shells,
                                                     ShellCfg sc, int* shell_lo, uint16_t* shell_n, int* tile_cnt,
-                                                    int tstride, const Bands* bands) {
-    if (bands) sc.half = 0.5 * bands->thr_search + 1e-6;
+                                                    int tstride, const unsigned long long* bdev,
+                                                    const StepInfo* binfo, BandCfg bcfg, Bands* bands) {
+    // With a drift band: the step's thresholds from the sample's drift so
+    // far (every block computes them; block 0 stores them for the kernels
+    // that follow).
+    if (bands) {
+        const double d = bdev ? __longlong_as_double(static_cast<long long>(bdev[0])) : 0.0;
+        const double dv = bdev ? __longlong_as_double(static_cast<long long>(bdev[1])) : 0.0;
+        const Bands bb = make_bands(bcfg, d, dv, binfo->t);
+        sc.half = 0.5 * bb.thr_search + 1e-6;
+        if (blockIdx.x == 0 && threadIdx.x == 0) *bands = bb;
+    }
     __shared__ StepPartial s_rows[WAVES_PER_BLOCK];
     if (blockIdx.x == 0 && threadIdx.x == 0) gs->ev_mark = gs->events < ev_cap ? gs->events : ev_cap;
     constexpr int nw = NW;
@@ -2170,11 +2186,6 @@ __global__ void __launch_bounds__(BLOCK) k_items(GridState* gs, const DenseCell*
 
 // Time and index of one learned step, read by the kernels from a per-chunk
 // device table so that a captured chunk graph replays for later chunks.
-struct StepInfo {
-    double t;         // start of the step (s since epoch)
-    long long macro;  // global learned-step index
-};
-
 struct PairParams {
     Planes before, after;
     const Bands* bands;     // learned form with a drift band: thresholds of the step (else null)
@@ -4853,18 +4864,24 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                     }
                     mark(ctx, 7);
                     // The step's thresholds, from the drift so far (before
-                    // this step's sample update is forked).
-                    if (follow) k_bands<<<1, 64, 0, s>>>(ctx->sample_dev, info, bcfg, ctx->bands);
+                    // this step's sample update is forked): computed by
+                    // k_windows when the screening's windows come from it.
+                    const bool wfused = screen && nw <= FUSED_WINDOWS_MAX;
                     // The sample's plain step runs on the side stream beside
                     // the screening (it reads the end-of-step states and
                     // flags, and writes only its own arrays); joined before
                     // anything writes the flags again.
-                    if (follow) {
-                        KHIP_TRY(ctx, hipEventRecord(ctx->ev_fork, s));
-                        KHIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                    auto fork_sample = [&](hipEvent_t ev) -> bool {
+                        KHIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ev, 0));
                         k_sample_step<<<(ctx->sample_n + BLOCK - 1) / BLOCK, BLOCK, 0, ctx->side>>>(
                             ctx->sample, ctx->sample_n, pn, ctx->alive, m13, h, sub, ctx->sample_dev);
                         KHIP_TRY(ctx, hipEventRecord(ctx->ev_join, ctx->side));
+                        return true;
+                    };
+                    if (follow && !wfused) {
+                        k_bands<<<1, 64, 0, s>>>(ctx->sample_dev, info, bcfg, ctx->bands);
+                        KHIP_TRY(ctx, hipEventRecord(ctx->ev_fork, s));
+                        if (!fork_sample(ctx->ev_fork)) return false;
                     }
                     if (screen) {
                         // Screening windows of tau / nw, each spw sub-steps long;
@@ -4890,12 +4907,14 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
                                 kw<<<part_grid(n), BLOCK, 0, s>>>(
                                     pc, pn, ctx->alive, n, ctx->dt, ctx->substates, c, ctx->win_parts, ctx->gs,
                                     static_cast<unsigned long long>(ctx->ev_cap), shells ? 1 : 0, shell_cfg,
-                                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride, bands);
-                                // The second stream starts once every window's states are written.
-                                if (two) {
-                                    KHIP_TRY(ctx, hipEventRecord(ctx->unit2.ev_go, s));
-                                    KHIP_TRY(ctx, hipStreamWaitEvent(ctx->unit2.stream, ctx->unit2.ev_go, 0));
-                                }
+                                    ctx->shell_lo, ctx->shell_n, ctx->tile_cnt, ctx->tstride,
+                                    follow ? ctx->sample_dev : nullptr, info, bcfg, follow ? ctx->bands : nullptr);
+                                // The second stream and the sample's step start once every
+                                // window's states and the thresholds are written (one event).
+                                const hipEvent_t go = follow ? ctx->ev_fork : (two ? ctx->unit2.ev_go : nullptr);
+                                if (go) KHIP_TRY(ctx, hipEventRecord(go, s));
+                                if (follow && !fork_sample(go)) return false;
+                                if (two) KHIP_TRY(ctx, hipStreamWaitEvent(ctx->unit2.stream, go, 0));
                             }
                             // With two streams every window writes its own record
                             // (no record is passed between windows of different streams).
