@@ -93,6 +93,7 @@ constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
 // enough loads in flight with a few hundred tiles on 256 CUs.
 constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
+constexpr int SHELL_TITEMS = 3;  // elements per thread of the shell entries' sort tiles
 constexpr int TWAVES = TBLOCK / 64;
 constexpr int COL_BITS = 24;  // column table index width (see k_cell_write)
 // Shell screening (learned form): a sorted element is one (object, radial
@@ -1688,6 +1689,7 @@ __global__ void __launch_bounds__(TBLOCK) k_keys_shell(Planes cur, Planes end, c
     }
 }
 
+template <int TI = TITEMS>
 __global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int pass, const GridState* gs,
                                                  int* hist, int ntiles) {
     __shared__ int s_hist[RADIX];
@@ -1696,8 +1698,8 @@ __global__ void __launch_bounds__(TBLOCK) k_hist(const uint64_t* keys, int pass,
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
     const int shift = RADIX_BITS * pass;
-    const int base = blockIdx.x * TILE;
-    for (int r = 0; r < TITEMS; ++r) {
+    const int base = blockIdx.x * (TI * TBLOCK);
+    for (int r = 0; r < TI; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
         if (e < n) atomicAdd(&s_hist[static_cast<int>((keys[e] >> shift) & DIGIT_MASK)], 1);
     }
@@ -1727,6 +1729,7 @@ __global__ void __launch_bounds__(BLOCK) k_digit_scan(int* hist, int ntiles, int
 // index order, TBLOCK per round; a wave finds the lanes holding the same
 // digit with RADIX_BITS ballots, and per-wave digit counts in LDS give the
 // cross-wave offsets.
+template <int TI = TITEMS>
 __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, const int* idx_in,
                                                      uint64_t* keys_out, int* idx_out,
                                                      int pass, const GridState* gs,
@@ -1750,17 +1753,17 @@ __global__ void __launch_bounds__(TBLOCK) k_scatter(const uint64_t* keys_in, con
     const int shift = RADIX_BITS * pass;
     const int wave = t / WAVE;
     const unsigned long long below = lanes_below();
-    const int base = blockIdx.x * TILE;
-    uint64_t keys[TITEMS];
-    int ids[TITEMS];
+    const int base = blockIdx.x * (TI * TBLOCK);
+    uint64_t keys[TI];
+    int ids[TI];
 #pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {  // all of the tile's loads in flight at once
+    for (int r = 0; r < TI; ++r) {  // all of the tile's loads in flight at once
         const int e = base + r * TBLOCK + t;
         keys[r] = e < n ? keys_in[e] : 0ULL;
         ids[r] = e < n ? idx_in[e] : 0;
     }
 #pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
+    for (int r = 0; r < TI; ++r) {
         const int e = base + r * TBLOCK + t;
         const bool valid = e < n;
         const uint64_t key = keys[r];
@@ -3818,7 +3821,14 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     const bool shells = shell_q >= 0;
     // Sorted elements the grid kernels are launched for.
     const int n = shells ? static_cast<int>(ctx->elaunch) : n_obj;
-    const int tiles = static_cast<int>((static_cast<int64_t>(n) + TILE - 1) / TILE);
+    // Radix-sort tiles: TILE elements for the plain grid (k_keys counts the
+    // first pass per object tile), SHELL_TITEMS per thread for shell entries
+    // (about 1.5 M entries at 1M objects then fit one round of blocks). The
+    // cell kernels keep TILE-element tiles (at three elements per thread
+    // k_cell_write holds one block per CU).
+    const int ti = shells ? SHELL_TITEMS : TITEMS;
+    const int tiles = static_cast<int>((static_cast<int64_t>(n) + ti * TBLOCK - 1) / (ti * TBLOCK));
+    const int ctiles = static_cast<int>((static_cast<int64_t>(n) + TILE - 1) / TILE);
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
     FinishArgs fin{};
@@ -3838,20 +3848,21 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     else
         k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
                                        ctx->idx[0], ctx->hist, tiles, ctx->rec);
+    auto hist = shells ? k_hist<SHELL_TITEMS> : k_hist<TITEMS>;
+    auto scatter = shells ? k_scatter<SHELL_TITEMS> : k_scatter<TITEMS>;
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         // k_keys counts the first pass's digits; shell entries are not in
         // object tiles, so their first pass is counted here too.
-        if (pass > 0 || shells)
-            k_hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], pass, ctx->gs, ctx->hist, tiles);
+        if (pass > 0 || shells) hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], pass, ctx->gs, ctx->hist, tiles);
         k_digit_scan<<<RADIX, BLOCK, 0, s>>>(ctx->hist, tiles, pass, ctx->gs, ctx->totals);
-        k_scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
-                                          ctx->gs, ctx->hist, ctx->totals, tiles);
+        scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
+                                        ctx->gs, ctx->hist, ctx->totals, tiles);
     }
-    k_cell_count<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
-    k_cell_write<<<tiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1],
-                                         ctx->gs, ctx->bcount, tiles, ctx->rec, ctx->cell_key,
-                                         ctx->cell_begin, ctx->cell_of, ctx->lrec, ctx->coltab);
+    k_cell_count<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
+    k_cell_write<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->gs,
+                                          ctx->bcount, ctiles, ctx->rec, ctx->cell_key, ctx->cell_begin,
+                                          ctx->cell_of, ctx->lrec, ctx->coltab);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
                                               ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
