@@ -93,7 +93,6 @@ constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
 // enough loads in flight with a few hundred tiles on 256 CUs.
 constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
-constexpr int SHELL_TITEMS = 3;  // elements per thread of the shell entries' sort tiles
 constexpr int TWAVES = TBLOCK / 64;
 constexpr int COL_BITS = 24;  // column table index width (see k_cell_write)
 // Shell screening (learned form): a sorted element is one (object, radial
@@ -1593,6 +1592,7 @@ __global__ void __launch_bounds__(TBLOCK) k_grid_setup(GridState* gs, const Step
 // with the object index in w, then velocity with the cell's packed z
 // coordinate in w. k_cell_write gathers one record (one cache line) per
 // sorted object.
+template <int TI = TITEMS>
 __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int midpoint, const uint8_t* alive,
                                                  const int* __restrict__ scen, int n, const GridState* gs,
                                                  uint64_t* keys, int* idx, int* hist, int ntiles, float4* rec) {
@@ -1601,8 +1601,8 @@ __global__ void __launch_bounds__(TBLOCK) k_keys(Planes cur, Planes end, int mid
     if (g.npasses == 0) return;
     if (threadIdx.x < RADIX) s_hist[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * TILE;
-    for (int r = 0; r < TITEMS; ++r) {
+    const int base = blockIdx.x * (TI * TBLOCK);
+    for (int r = 0; r < TI; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
         if (e < n) {
             uint64_t k = DEAD_KEY;
@@ -1807,16 +1807,17 @@ __device__ __forceinline__ const int* sorted_idx(const GridState& g, const int* 
 }
 
 // Cell heads per tile over the first `nlive` sorted elements.
+template <int TI = TITEMS>
 __global__ void __launch_bounds__(TBLOCK) k_cell_count(const uint64_t* k0, const uint64_t* k1,
                                                         const GridState* gs, int* bcount) {
     __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.nlive);
     const uint64_t* keys = sorted_keys(g, k0, k1);
-    const int base = blockIdx.x * TILE;
+    const int base = blockIdx.x * (TI * TBLOCK);
     int cnt = 0;
     if (base < A && g.npasses > 0) {
-        for (int r = 0; r < TITEMS; ++r) {
+        for (int r = 0; r < TI; ++r) {
             const int e = base + r * TBLOCK + threadIdx.x;
             if (e < A && (e == 0 || keys[e] != keys[e - 1])) ++cnt;
         }
@@ -1835,17 +1836,18 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_count(const uint64_t* k0, const
 //                 index bits;
 //   lrec[2e + 1]: velocity, w = the cell's packed z coordinate (int bits),
 //                 from which pairs get their z cell offset.
+template <int TI = TITEMS>
 __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const uint64_t* k1,
                                                        const int* i0, const int* i1,
                                                        GridState* gs, const int* bcount, int ntiles,
                                                        const float4* __restrict__ rec, uint64_t* cell_key,
                                                        int* cell_begin, int* cell_of,
                                                        float4* lrec, int4* __restrict__ coltab) {
-    __shared__ int s_seg[TITEMS * TWAVES];
+    __shared__ int s_seg[TI * TWAVES];
     __shared__ int s_wave[TWAVES];
     const GridState& g = *gs;
     const int A = static_cast<int>(g.nlive);
-    const int base = blockIdx.x * TILE;
+    const int base = blockIdx.x * (TI * TBLOCK);
     if (base >= A || g.npasses == 0) return;
     // Cells of the earlier tiles: the sum of their head counts (k_cell_count),
     // a few loads per thread; block 0 sums every tile and writes the total.
@@ -1864,12 +1866,12 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     const int lane = static_cast<int>(__lane_id());
     const int wave = threadIdx.x / WAVE;
     const unsigned long long below = lanes_below();
-    uint64_t key[TITEMS], prv[TITEMS];
-    unsigned long long bal[TITEMS];
-    bool head[TITEMS];
-    float4 p[TITEMS], v[TITEMS];
+    uint64_t key[TI], prv[TI];
+    unsigned long long bal[TI];
+    bool head[TI];
+    float4 p[TI], v[TI];
 #pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
+    for (int r = 0; r < TI; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
         const bool valid = e < A;
         key[r] = valid ? keys[e] : 0ULL;
@@ -1889,9 +1891,9 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
         }
     }
     __syncthreads();
-    int pre[TITEMS];
+    int pre[TI];
 #pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
+    for (int r = 0; r < TI; ++r) {
         for (int w = 0; w < TWAVES; ++w) {
             if (w == wave) pre[r] = run;
             run += s_seg[r * TWAVES + w];
@@ -1903,7 +1905,7 @@ __global__ void __launch_bounds__(TBLOCK) k_cell_write(const uint64_t* k0, const
     const bool use_tab = coltab != nullptr && g.sbits + g.bits[0] + g.bits[1] <= COL_BITS;
     const int bz = g.bits[2] + g.bbits;  // a column: one (x, y) (and shell) run
 #pragma unroll
-    for (int r = 0; r < TITEMS; ++r) {
+    for (int r = 0; r < TI; ++r) {
         const int e = base + r * TBLOCK + threadIdx.x;
         if (e >= A) break;
         const int c = pre[r] + __popcll(bal[r] & below) + (head[r] ? 1 : 0) - 1;
@@ -3597,7 +3599,7 @@ bool ensure_surv(khip_ctx* ctx, int64_t want) {
 bool ensure_sort_capacity(khip_ctx* ctx, int64_t want) {
     if (want <= ctx->ecap) return true;
     const int64_t cap = std::max<int64_t>(want, ctx->cap);
-    const int64_t tiles = (cap + TILE - 1) / TILE;
+    const int64_t tiles = (cap + TBLOCK - 1) / TBLOCK;  // tiles of one element per thread at most
     ctx->ecap = cap;
     return dev_alloc(ctx, &ctx->keys[0], cap) && dev_alloc(ctx, &ctx->keys[1], cap) &&
            dev_alloc(ctx, &ctx->idx[0], cap) && dev_alloc(ctx, &ctx->idx[1], cap) &&
@@ -3696,7 +3698,7 @@ bool sync_unit2(khip_ctx* ctx) {
     }
     if (u.ecap != ctx->ecap) {
         const int64_t cap = ctx->ecap;
-        const int64_t tiles = (cap + TILE - 1) / TILE;
+        const int64_t tiles = (cap + TBLOCK - 1) / TBLOCK;  // tiles of one element per thread at most
         if (!dev_alloc(ctx, &u.keys[0], cap) || !dev_alloc(ctx, &u.keys[1], cap) || !dev_alloc(ctx, &u.idx[0], cap) ||
             !dev_alloc(ctx, &u.idx[1], cap) || !dev_alloc(ctx, &u.hist, RADIX * tiles) ||
             !dev_alloc(ctx, &u.bcount, tiles) || !dev_alloc(ctx, &u.cell_key, cap) ||
@@ -3821,14 +3823,17 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     const bool shells = shell_q >= 0;
     // Sorted elements the grid kernels are launched for.
     const int n = shells ? static_cast<int>(ctx->elaunch) : n_obj;
-    // Radix-sort tiles: TILE elements for the plain grid (k_keys counts the
-    // first pass per object tile), SHELL_TITEMS per thread for shell entries
-    // (about 1.5 M entries at 1M objects then fit one round of blocks). The
-    // cell kernels keep TILE-element tiles (at three elements per thread
-    // k_cell_write holds one block per CU).
-    const int ti = shells ? SHELL_TITEMS : TITEMS;
+    // Tiles of the tile kernels: one to three elements per thread, the
+    // fewest that keep the tiles within one round of blocks (two 16-wave
+    // blocks per CU), so small populations still spread over the CUs. The
+    // radix passes (and k_keys, which counts the first pass per tile) use
+    // `ti`; the cell kernels at most two (at three k_cell_write needs 80
+    // registers and holds one block per CU).
+    const int64_t round = 2LL * ctx->n_cus * TBLOCK;
+    const int ti = n <= round ? 1 : (n <= 2 * round || !shells ? 2 : 3);
+    const int ci = ti < 2 ? ti : 2;
     const int tiles = static_cast<int>((static_cast<int64_t>(n) + ti * TBLOCK - 1) / (ti * TBLOCK));
-    const int ctiles = static_cast<int>((static_cast<int64_t>(n) + TILE - 1) / TILE);
+    const int ctiles = static_cast<int>((static_cast<int64_t>(n) + ci * TBLOCK - 1) / (ci * TBLOCK));
     hipStream_t s = ctx->stream;
     const TileCfg& tc = ctx->tile;
     FinishArgs fin{};
@@ -3846,10 +3851,11 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
             ctx->shell_n + static_cast<size_t>(shell_q) * ctx->cap, sh.tile_base, static_cast<unsigned int>(n),
             ctx->keys[0], ctx->idx[0], ctx->rec);
     else
-        k_keys<<<tiles, TBLOCK, 0, s>>>(pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0],
-                                       ctx->idx[0], ctx->hist, tiles, ctx->rec);
-    auto hist = shells ? k_hist<SHELL_TITEMS> : k_hist<TITEMS>;
-    auto scatter = shells ? k_scatter<SHELL_TITEMS> : k_scatter<TITEMS>;
+        (ti == 1 ? k_keys<1> : k_keys<2>)<<<tiles, TBLOCK, 0, s>>>(
+            pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0], ctx->idx[0], ctx->hist, tiles,
+            ctx->rec);
+    auto hist = ti == 1 ? k_hist<1> : ti == 2 ? k_hist<2> : k_hist<3>;
+    auto scatter = ti == 1 ? k_scatter<1> : ti == 2 ? k_scatter<2> : k_scatter<3>;
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         // k_keys counts the first pass's digits; shell entries are not in
@@ -3859,10 +3865,11 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
         scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
                                         ctx->gs, ctx->hist, ctx->totals, tiles);
     }
-    k_cell_count<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs, ctx->bcount);
-    k_cell_write<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->gs,
-                                          ctx->bcount, ctiles, ctx->rec, ctx->cell_key, ctx->cell_begin,
-                                          ctx->cell_of, ctx->lrec, ctx->coltab);
+    (ci == 1 ? k_cell_count<1> : k_cell_count<2>)<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs,
+                                                                           ctx->bcount);
+    (ci == 1 ? k_cell_write<1> : k_cell_write<2>)<<<ctiles, TBLOCK, 0, s>>>(
+        ctx->keys[0], ctx->keys[1], ctx->idx[0], ctx->idx[1], ctx->gs, ctx->bcount, ctiles, ctx->rec,
+        ctx->cell_key, ctx->cell_begin, ctx->cell_of, ctx->lrec, ctx->coltab);
     const unsigned int icap = static_cast<unsigned int>(std::min<int64_t>(ctx->item_cap, 0x7fffffff));
     k_neighbours<<<grid_of(n), BLOCK, 0, s>>>(ctx->gs, ctx->cell_key, ctx->cell_begin,
                                               ctx->cell_count, ctx->ranges, ctx->pair_parts, ctx->dense,
@@ -5658,7 +5665,7 @@ constexpr int64_t SCR_OBJECTS = 32768;     // objects per call
 constexpr int64_t SCR_EVENTS = 1 << 17;    // events per call
 constexpr int64_t SCR_SURVIVORS = 1 << 20; // pre-filter survivors per call
 constexpr int64_t SCR_ITEMS = 1 << 16;     // dense-cell work items per call
-constexpr size_t SCR_TILES = (SCR_OBJECTS + TILE - 1) / TILE;
+constexpr size_t SCR_TILES = (SCR_OBJECTS + TBLOCK - 1) / TBLOCK;  // tiles of one element per thread at most
 constexpr size_t SCR_BLOCKS = (SCR_OBJECTS + BLOCK - 1) / BLOCK;
 
 enum ScrRegion {
