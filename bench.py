@@ -102,7 +102,7 @@ def run_gpu(kamd, pop, steps, warmup):
         eng.run(warmup, screen=True)
         eng.take_events()
         eng.take_stats()
-    tune = eng.autotune(steps=4)
+    tune = eng.autotune(steps=48)
     reset()
     t0 = time.perf_counter()
     eng.run(steps, screen=True)
