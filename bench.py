@@ -84,9 +84,10 @@ def cascade_population(kamd):
 
 def run_gpu(kamd, pop, steps, warmup):
     """Plain FP64 run of `pop`: the warm-up steps run first, then the tiling
-    is autotuned on a few steps of the warmed state (results do not depend
-    on it; at t = 0 every breakup's fragments still share one point, which
-    is no guide to the rest of the day), then the timed run from t = 0
+    is autotuned in 48-step trials from the warmed state (results do not
+    depend on it; at t = 0 every breakup's fragments still share one point,
+    which is no guide to the rest of the day; shorter trials are too noisy
+    to tell the close candidates apart), then the timed run from t = 0
     carries no per-kernel event marks; per-kernel device times come from
     the same run repeated from the same state with marks."""
     eng = kamd.Engine(0)
