@@ -82,7 +82,7 @@ def cascade_population(kamd):
     return kamd.cascade(28340, 0, 40, 0.045)
 
 
-def run_gpu(kamd, pop, steps, warmup):
+def run_gpu(kamd, pop, steps, warmup, tiling=None):
     """Plain FP64 run of `pop`: the warm-up steps run first, then the tiling
     is autotuned in 48-step trials from the warmed state (results do not
     depend on it; at t = 0 every breakup's fragments still share one point,
@@ -103,7 +103,13 @@ def run_gpu(kamd, pop, steps, warmup):
         eng.run(warmup, screen=True)
         eng.take_events()
         eng.take_stats()
-    tune = eng.autotune(steps=48)
+    if tiling:
+        # A fixed tiling in place of the autotune (profiling runs, where the
+        # profiler's launch overhead would sway the tuner's timings).
+        eng.set_tuning(**tiling)
+        tune = [eng.tuning(), None, None]
+    else:
+        tune = eng.autotune(steps=48)
     reset()
     t0 = time.perf_counter()
     eng.run(steps, screen=True)
@@ -739,6 +745,9 @@ def main():
     ap.add_argument("--no-breakups", action="store_true", help="ensemble: collisions do not fragment")
     ap.add_argument("--train-iters", type=int, default=20000,
                     help="corrector training steps before the timed runs (fewer only for profiling runs)")
+    ap.add_argument("--tiling", default=None,
+                    help="cascade: a fixed pair tiling 'dense_min,long_range[,seg]' instead of the autotune "
+                         "(for profiling runs)")
     ap.add_argument("--workload", default="learned",
                     choices=("learned", "cascade", "precision", "ensemble", "catalogue"),
                     help="learned: the learned corrector at 1M objects vs plain FP64 (default, the headline "
@@ -769,7 +778,11 @@ def main():
         precision_workload(kamd, args)
         return
     pop = cascade_population(kamd)
-    eng, wall, tm, ev, st, tune, wall_marked = run_gpu(kamd, pop, args.steps, args.warmup)
+    tiling = None
+    if args.tiling:
+        vals = [int(v) for v in args.tiling.split(",")]
+        tiling = dict(zip(("dense_min", "long_range", "seg"), vals))
+    eng, wall, tm, ev, st, tune, wall_marked = run_gpu(kamd, pop, args.steps, args.warmup, tiling)
     obj_steps = int(st["propagated"].sum())
     cand = int(st["candidate_pairs"].sum())
     value = obj_steps / wall
