@@ -87,10 +87,12 @@ using namespace kessler::dev;
 namespace {
 
 constexpr int ITEMS = 8;               // elements per thread per tile (BLOCK threads)
-constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per tile
+constexpr int TILE = BLOCK * ITEMS;    // 2048 elements per object tile
 // The tile kernels of the grid build (keys, radix passes, cell table) take a
-// tile with TBLOCK threads, TITEMS elements each: 16 waves per tile keep
-// enough loads in flight with a few hundred tiles on 256 CUs.
+// tile with TBLOCK threads, TI elements each (a template argument: one to
+// three, chosen per screening so that the tiles fit one round of blocks;
+// TITEMS, two, at 1M objects): 16 waves per tile keep enough loads in
+// flight with a few hundred tiles on 256 CUs.
 constexpr int TBLOCK = 1024;
 constexpr int TITEMS = TILE / TBLOCK;
 constexpr int TWAVES = TBLOCK / 64;
@@ -3854,16 +3856,17 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
         (ti == 1 ? k_keys<1> : k_keys<2>)<<<tiles, TBLOCK, 0, s>>>(
             pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0], ctx->idx[0], ctx->hist, tiles,
             ctx->rec);
-    auto hist = ti == 1 ? k_hist<1> : ti == 2 ? k_hist<2> : k_hist<3>;
-    auto scatter = ti == 1 ? k_scatter<1> : ti == 2 ? k_scatter<2> : k_scatter<3>;
+    auto hist_kernel = ti == 1 ? k_hist<1> : ti == 2 ? k_hist<2> : k_hist<3>;
+    auto scatter_kernel = ti == 1 ? k_scatter<1> : ti == 2 ? k_scatter<2> : k_scatter<3>;
     for (int pass = 0; pass < ctx->passes; ++pass) {
         const int src = pass & 1, dst = src ^ 1;
         // k_keys counts the first pass's digits; shell entries are not in
         // object tiles, so their first pass is counted here too.
-        if (pass > 0 || shells) hist<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], pass, ctx->gs, ctx->hist, tiles);
+        if (pass > 0 || shells)
+            hist_kernel<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], pass, ctx->gs, ctx->hist, tiles);
         k_digit_scan<<<RADIX, BLOCK, 0, s>>>(ctx->hist, tiles, pass, ctx->gs, ctx->totals);
-        scatter<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
-                                        ctx->gs, ctx->hist, ctx->totals, tiles);
+        scatter_kernel<<<tiles, TBLOCK, 0, s>>>(ctx->keys[src], ctx->idx[src], ctx->keys[dst], ctx->idx[dst], pass,
+                                               ctx->gs, ctx->hist, ctx->totals, tiles);
     }
     (ci == 1 ? k_cell_count<1> : k_cell_count<2>)<<<ctiles, TBLOCK, 0, s>>>(ctx->keys[0], ctx->keys[1], ctx->gs,
                                                                            ctx->bcount);
