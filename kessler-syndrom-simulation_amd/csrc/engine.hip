@@ -3485,6 +3485,7 @@ struct khip_ctx {
     // pair kernels. KHIP_WINDOW_STREAMS=0: one stream.
     ScreenUnit unit2;
     bool window_streams = true;
+    int tile_items = 0;  // KHIP_TILE_ITEMS: elements per thread of the tile kernels (0: by size)
     bool unit2_used = false;  // the chunk being drained screened on unit2
     bool in_unit2 = false;    // unit2's buffers are swapped in (no timing marks)
 
@@ -3832,7 +3833,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
     // `ti`; the cell kernels at most two (at three k_cell_write needs 80
     // registers and holds one block per CU).
     const int64_t round = 2LL * ctx->n_cus * TBLOCK;
-    const int ti = n <= round ? 1 : (n <= 2 * round || !shells ? 2 : 3);
+    const int ti = ctx->tile_items > 0 ? ctx->tile_items : n <= round ? 1 : (n <= 2 * round || !shells ? 2 : 3);
     const int ci = ti < 2 ? ti : 2;
     const int tiles = static_cast<int>((static_cast<int64_t>(n) + ti * TBLOCK - 1) / (ti * TBLOCK));
     const int ctiles = static_cast<int>((static_cast<int64_t>(n) + ci * TBLOCK - 1) / (ci * TBLOCK));
@@ -3853,7 +3854,7 @@ bool enqueue_screen(khip_ctx* ctx, Planes pb, Planes pa, double t0, int64_t step
             ctx->shell_n + static_cast<size_t>(shell_q) * ctx->cap, sh.tile_base, static_cast<unsigned int>(n),
             ctx->keys[0], ctx->idx[0], ctx->rec);
     else
-        (ti == 1 ? k_keys<1> : k_keys<2>)<<<tiles, TBLOCK, 0, s>>>(
+        (ti == 1 ? k_keys<1> : ti == 2 ? k_keys<2> : k_keys<3>)<<<tiles, TBLOCK, 0, s>>>(
             pb, pa, midpoint, ctx->alive, ctx->scen, n, ctx->gs, ctx->keys[0], ctx->idx[0], ctx->hist, tiles,
             ctx->rec);
     auto hist_kernel = ti == 1 ? k_hist<1> : ti == 2 ? k_hist<2> : k_hist<3>;
@@ -5962,6 +5963,7 @@ int khip_create(int device, khip_ctx** out, char* msg, size_t msg_cap) {
     ctx->device = device;
     if (const char* f = std::getenv("KHIP_LEARNED_FUSED")) ctx->split_learned = std::atoi(f) == 0;
     if (const char* f = std::getenv("KHIP_WINDOW_STREAMS")) ctx->window_streams = std::atoi(f) != 0;
+    if (const char* f = std::getenv("KHIP_TILE_ITEMS")) ctx->tile_items = std::min(3, std::max(0, std::atoi(f)));
 
 
     ctx->n_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;

@@ -363,3 +363,30 @@ def test_window_streams_equal_one_stream(kamd, trained, monkeypatch):
         assert t0.tobytes() == t1.tobytes(), (nw, shell, graph)
         assert r0 == r1
         assert len(e0) > 50
+
+
+def test_tile_sizes_do_not_change_results(kamd, trained, monkeypatch):
+    """The tile kernels of the grid build (keys, radix passes, cell table)
+    give the same sorted order, cells and results at one, two or three
+    elements per thread (KHIP_TILE_ITEMS; by default the size picks it):
+    plain FP64 and the learned form with radial shells, states, flags,
+    events and per-sub-step statistics bit for bit."""
+    _, _, params, norm = trained
+    pop = kamd.generate(60000, seed=13, mix=MIX_DENSE)
+    for precision, dt, steps in ((0, 10.0, 30), (2, 60.0, 12)):
+        out = {}
+        for items in ("0", "1", "2", "3"):
+            monkeypatch.setenv("KHIP_TILE_ITEMS", items)
+            eng = kamd.Engine(0)
+            try:
+                eng.net_set(params, norm)
+                out[items] = _run(eng, pop, precision, steps, dt, True)
+            finally:
+                eng.close()
+        s0, a0, e0, t0 = out["0"]
+        print(f"precision {precision}: {len(e0)} events, {int(t0['candidate_pairs'].sum())} candidate pairs")
+        for items in ("1", "2", "3"):
+            s1, a1, e1, t1 = out[items]
+            assert np.array_equal(s0, s1) and np.array_equal(a0, a1), (precision, items)
+            assert e0.tobytes() == e1.tobytes() and t0.tobytes() == t1.tobytes(), (precision, items)
+        assert len(e0) > 20
