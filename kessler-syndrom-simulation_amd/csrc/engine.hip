@@ -419,9 +419,9 @@ struct SampleObj {
 };
 
 // The thresholds of one learned step's screening, from the sample's drift
-// so far (k_bands writes them once per step on the device; the screening
-// kernels read them, so captured graphs and chunk boundaries do not depend
-// on the values).
+// so far (k_windows, or k_bands when the windows are not fused, writes them
+// once per step on the device; the screening kernels read them, so captured
+// graphs and chunk boundaries do not depend on the values).
 struct Bands {
     double band, band_v;              // drift bands (km, km/s)
     double thr_search, thr_report;    // search: threshold + margin + band; report: threshold + band
@@ -4785,8 +4785,8 @@ bool run_steps_learned(khip_ctx* ctx, int64_t nsteps, int screen) {
         const double t_chunk = ctx->t;
         const bool force0 = ctx->force_check;
         // With a drift band, each step's thresholds are computed on the
-        // device (k_bands) from the sample's drift; the host's values below
-        // are then the band-free ones the layout starts from.
+        // device (k_windows or k_bands) from the sample's drift; the host's
+        // values below are then the band-free ones the layout starts from.
         const double thr = ctx->cfg.threshold_km + L.margin_km;
         const double vmin2 = ctx->cfg.min_encounter_speed_km_s * ctx->cfg.min_encounter_speed_km_s;
         const BandCfg bcfg{follow ? L.margin_factor : 0.0, ctx->cfg.threshold_km, L.margin_km,
